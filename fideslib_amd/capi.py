@@ -1,0 +1,241 @@
+"""ctypes binding of the C ABI in include/fideslib_amd.h.
+
+Loads the in-tree libfideslib_amd.so (built by fideslib_amd/build.py).  There
+is no fallback: if the library is missing or cannot be loaded, loading
+raises, so no operation can silently run elsewhere.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfideslib_amd.so")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "fideslib_amd.h")
+
+FIDES_OK = 0
+
+
+class FidesError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("logN", ctypes.c_uint32),
+        ("L", ctypes.c_uint32),
+        ("K", ctypes.c_uint32),
+        ("dnum", ctypes.c_uint32),
+        ("q", ctypes.POINTER(ctypes.c_uint64)),
+        ("p", ctypes.POINTER(ctypes.c_uint64)),
+        ("psi", ctypes.POINTER(ctypes.c_uint64)),
+    ]
+
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+
+# name -> (restype, argtypes); the list is also the export contract the CPU
+# tests check against the header.
+SIGNATURES = {
+    "fides_last_error": (ctypes.c_char_p, []),
+    "fides_context_create": (_int, [ctypes.POINTER(Params), ctypes.POINTER(_vp)]),
+    "fides_context_destroy": (None, [_vp]),
+    "fides_context_info": (_int, [_vp, _u32p, _u32p, _u32p, _u32p, _u32p]),
+    "fides_context_roots": (_int, [_vp, _u64p]),
+    "fides_device_alloc": (_int, [ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "fides_device_free": (_int, [_vp]),
+    "fides_copy_to_device": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "fides_copy_to_host": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "fides_stream_sync": (_int, [_vp]),
+    "fides_ntt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
+    "fides_intt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
+    "fides_modup": (_int, [_vp, _vp, _u32, _u32, _vp, _vp]),
+    "fides_moddown": (_int, [_vp, _vp, _u32, _vp, _vp]),
+    "fides_key_upload": (_int, [_vp, _u64p, ctypes.POINTER(_vp)]),
+    "fides_key_destroy": (None, [_vp]),
+    "fides_keyswitch": (_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "fides_hmult": (_int, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
+    "fides_square": (_int, [_vp, _vp, _u32, _vp, _vp, _vp]),
+    "fides_rescale": (_int, [_vp, _vp, _u32, _vp, _vp]),
+    "fides_rotate": (_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
+    "fides_automorph": (_int, [_vp, _vp, _u32, _u32, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FidesError("libfideslib_amd.so is not built (run fideslib_amd/build.py): " + LIB_PATH)
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != FIDES_OK:
+        msg = load().fides_last_error()
+        raise FidesError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+# ------------------------------------------------------------------ device buffers
+class DeviceBuffer:
+    """A device allocation of uint64 words owned by this object."""
+
+    def __init__(self, nwords: int):
+        self.nwords = int(nwords)
+        p = _vp()
+        check(load().fides_device_alloc(max(1, self.nwords) * 8, ctypes.byref(p)), "fides_device_alloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a: np.ndarray, stream=None) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        b = cls(a.size)
+        b.upload(a, stream)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a, dtype=np.uint64)
+        assert a.size <= self.nwords
+        check(load().fides_copy_to_device(self.ptr, a.ctypes.data, a.size * 8, stream), "copy_to_device")
+
+    def download(self, shape=None, stream=None, nwords=None) -> np.ndarray:
+        n = self.nwords if nwords is None else nwords
+        out = np.empty(n, np.uint64)
+        check(load().fides_copy_to_host(out.ctypes.data, self.ptr, n * 8, stream), "copy_to_host")
+        return out.reshape(shape) if shape is not None else out
+
+    def at(self, word_offset: int) -> int:
+        return self.ptr + 8 * int(word_offset)
+
+    def free(self) -> None:
+        if getattr(self, "ptr", None):
+            load().fides_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """A device context for one parameter set (Q primes, P primes, dnum)."""
+
+    def __init__(self, logN: int, q, p, dnum: int, psi=None):
+        lib = load()
+        self.logN = logN
+        self.N = 1 << logN
+        self.q = [int(x) for x in q]
+        self.p = [int(x) for x in p]
+        self.L, self.K, self.dnum = len(self.q), len(self.p), dnum
+        self._qa = np.ascontiguousarray(self.q, np.uint64)
+        self._pa = np.ascontiguousarray(self.p, np.uint64)
+        self._psi = None if psi is None else np.ascontiguousarray([int(x) for x in psi], np.uint64)
+        prm = Params(logN, self.L, self.K, dnum, self._qa.ctypes.data_as(_u64p), self._pa.ctypes.data_as(_u64p),
+                     self._psi.ctypes.data_as(_u64p) if self._psi is not None else None)
+        h = _vp()
+        check(lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)), "fides_context_create")
+        self.h = h.value
+        a = _u32()
+        lib.fides_context_info(self.h, None, None, None, None, ctypes.byref(a))
+        self.alpha = a.value
+        roots = np.zeros(self.L + self.K, np.uint64)
+        check(lib.fides_context_roots(self.h, roots.ctypes.data_as(_u64p)), "roots")
+        self.roots = [int(r) for r in roots]
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fides_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def digits(self, l: int) -> int:
+        return (l + self.alpha - 1) // self.alpha
+
+    # ---- raw limb ops
+    def ntt(self, buf: DeviceBuffer, pids, stream=None, inverse=False, offset=0):
+        pid = np.ascontiguousarray(pids, np.uint32)
+        f = load().fides_intt if inverse else load().fides_ntt
+        check(f(self.h, buf.at(offset), pid.ctypes.data_as(_u32p), len(pid), stream), "fides_ntt")
+
+    def intt(self, buf: DeviceBuffer, pids, stream=None, offset=0):
+        self.ntt(buf, pids, stream, inverse=True, offset=offset)
+
+    def modup(self, inp: DeviceBuffer, l: int, digit: int, out: DeviceBuffer, stream=None):
+        check(load().fides_modup(self.h, inp.ptr, l, digit, out.ptr, stream), "fides_modup")
+
+    def moddown(self, inp: DeviceBuffer, l: int, out: DeviceBuffer, stream=None):
+        check(load().fides_moddown(self.h, inp.ptr, l, out.ptr, stream), "fides_moddown")
+
+    # ---- keys
+    def upload_key(self, key: np.ndarray) -> "Key":
+        return Key(self, key)
+
+    def keyswitch(self, inp, l, key, out0, out1, stream=None):
+        check(load().fides_keyswitch(self.h, inp.ptr, l, key.h, out0.ptr, out1.ptr, stream), "fides_keyswitch")
+
+    # ---- ciphertext ops
+    def hmult(self, a, b, l, rlk, out, stream=None):
+        check(load().fides_hmult(self.h, _ptr(a), _ptr(b), l, rlk.h, _ptr(out), stream), "fides_hmult")
+
+    def square(self, a, l, rlk, out, stream=None):
+        check(load().fides_square(self.h, _ptr(a), l, rlk.h, _ptr(out), stream), "fides_square")
+
+    def rescale(self, inp, l, out, stream=None):
+        check(load().fides_rescale(self.h, _ptr(inp), l, _ptr(out), stream), "fides_rescale")
+
+    def rotate(self, inp, l, key, k, out, stream=None):
+        check(load().fides_rotate(self.h, _ptr(inp), l, key.h, k, _ptr(out), stream), "fides_rotate")
+
+    def automorph(self, inp, nlimbs, k, out, stream=None):
+        check(load().fides_automorph(self.h, _ptr(inp), nlimbs, k, _ptr(out), stream), "fides_automorph")
+
+    def sync(self, stream=None):
+        check(load().fides_stream_sync(stream), "sync")
+
+
+def _ptr(x) -> int:
+    return x.ptr if isinstance(x, DeviceBuffer) else int(x)
+
+
+class Key:
+    def __init__(self, ctx: Context, key: np.ndarray):
+        k = np.ascontiguousarray(key, np.uint64)
+        assert k.size == 2 * ctx.dnum * (ctx.L + ctx.K) * ctx.N, "key must be [2][dnum][L+K][N]"
+        h = _vp()
+        check(load().fides_key_upload(ctx.h, k.ctypes.data_as(_u64p), ctypes.byref(h)), "fides_key_upload")
+        self.h = h.value
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fides_key_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,108 @@
+// Device-side modular arithmetic on 64-bit residues for gfx950.
+//
+// CDNA has no 64x64 multiplier; every 64-bit product is built from 32x32->64
+// partial products (v_mad_u64_u32 / v_mul_lo_u32 / v_mul_hi_u32).  Three
+// reductions are used, each picked for the operand pattern of its caller:
+//
+//   shoup_lazy   x * w for a constant w with precomputed w' = floor(w 2^64 / q)
+//                (Shoup; reference src/ModMult.cuh:83-95).  x < 2^64, result
+//                in [0, 2q).  Used by every NTT butterfly and constant scale.
+//   barrett_mul  a * b for two variable operands in [0, q) (Barrett with
+//                mu = floor(2^(2b) / q), b = bit length of q; reference
+//                src/ModMult.cuh:42-69).  Result in [0, q).
+//   mont_redc    reduction of a 128-bit accumulator T (sum of products with
+//                one operand stored pre-multiplied by R = 2^64).  Used by the
+//                basis conversions and the key-switch inner product, which
+//                accumulate several 128-bit products before one reduction.
+//
+// All residues are kept below 4q (lazy reduction); q < 2^62 is required so
+// that 4q fits in 64 bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+// Per-prime constants, indexed by prime id (Q primes first, then P primes).
+struct PrimeC {
+  u64 q;         // the prime
+  u64 q2;        // 2q
+  u64 qneg_inv;  // -q^-1 mod 2^64 (Montgomery)
+  u64 mu;        // floor(2^(2*bits) / q) (Barrett)
+  u64 r2;        // R^2 mod q, R = 2^64
+  u64 ninv;      // N^-1 mod q
+  u64 ninv_sh;   // Shoup companion of ninv
+  u64 ninv_w;    // ipsi[1] * N^-1 mod q (last inverse stage, scaled)
+  u64 ninv_w_sh;
+  u32 bits;
+  u32 pad;
+};
+
+__device__ __forceinline__ void mul64wide(u64 a, u64 b, u64& hi, u64& lo) {
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+  const u64 p00 = (u64)a0 * b0;
+  const u64 p01 = (u64)a0 * b1;
+  const u64 p10 = (u64)a1 * b0;
+  const u64 p11 = (u64)a1 * b1;
+  const u64 mid = (p00 >> 32) + (u32)p01 + (u32)p10;
+  lo = (mid << 32) | (u32)p00;
+  hi = p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+}
+
+__device__ __forceinline__ u64 mulhi64(u64 a, u64 b) {
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+  const u64 p00 = (u64)a0 * b0;
+  const u64 p01 = (u64)a0 * b1;
+  const u64 p10 = (u64)a1 * b0;
+  const u64 p11 = (u64)a1 * b1;
+  const u64 mid = (p00 >> 32) + (u32)p01 + (u32)p10;
+  return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+}
+
+// x * w mod q in [0, 2q); any x < 2^64, w < q, wp = floor(w * 2^64 / q).
+__device__ __forceinline__ u64 shoup_lazy(u64 x, u64 w, u64 wp, u64 q) {
+  const u64 h = mulhi64(x, wp);
+  return x * w - h * q;
+}
+
+__device__ __forceinline__ u64 csub(u64 x, u64 m) { return x >= m ? x - m : x; }
+
+// x * w mod q in [0, q).
+__device__ __forceinline__ u64 shoup(u64 x, u64 w, u64 wp, u64 q) { return csub(shoup_lazy(x, w, wp, q), q); }
+
+// a * b mod q in [0, q) for a, b < q (Barrett, 2 corrections).
+__device__ __forceinline__ u64 barrett_mul(u64 a, u64 b, const PrimeC& P) {
+  u64 hi, lo;
+  mul64wide(a, b, hi, lo);
+  const u32 sh = P.bits - 1;  // T >> (bits-1) < 2^(bits+1) <= 2^63
+  const u64 t = (hi << (64 - sh)) | (lo >> sh);
+  u64 h2, l2;
+  mul64wide(t, P.mu, h2, l2);
+  const u32 s2 = P.bits + 1;  // (t * mu) >> (bits+1)
+  const u64 qh = (h2 << (64 - s2)) | (l2 >> s2);
+  u64 r = lo - qh * P.q;
+  r = csub(r, P.q2);
+  return csub(r, P.q);
+}
+
+// 128-bit accumulate: (hi, lo) += a * b.
+__device__ __forceinline__ void mac128(u64& hi, u64& lo, u64 a, u64 b) {
+  u64 h, l;
+  mul64wide(a, b, h, l);
+  lo += l;
+  hi += h + (lo < l);
+}
+
+// Montgomery reduction of T = hi * 2^64 + lo: returns T * 2^-64 mod q, fully
+// reduced.  Valid for hi < c * q with a small c (the result before the final
+// loop is below (c + 1) q); callers bound c by their accumulation length.
+__device__ __forceinline__ u64 mont_redc(u64 hi, u64 lo, const PrimeC& P) {
+  const u64 m = lo * P.qneg_inv;
+  u64 u = hi + mulhi64(m, P.q) + (lo != 0);
+  while (u >= P.q) u -= P.q;
+  return u;
+}
+
+__device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 q) { return csub(a + b, q); }
+__device__ __forceinline__ u64 sub_mod(u64 a, u64 b, u64 q) { return a >= b ? a - b : a + q - b; }

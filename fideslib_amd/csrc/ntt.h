@@ -1,0 +1,104 @@
+// Negacyclic NTT / INTT over 64-bit RNS limbs for gfx950.
+//
+// Transform (OpenFHE evaluation order, reference src/NTT.cu:331-695 forward,
+// :42-310 inverse; tables src/ConstantsGPU.cu:226-300):
+//   NTT(a)[k] = sum_i a_i psi^(i (2 brev(k) + 1))  mod q
+// computed by the Cooley-Tukey loop whose stage s (m = 2^s groups) uses the
+// bit-reversed twiddle psi_rev[m + group]; the inverse runs the
+// Gentleman-Sande loop with psi^-1 and folds N^-1 into its last stage.
+//
+// MI355X design: the N = 2^LOGN limb is viewed as a 2^L1 x 2^L2 matrix
+// (row-major, L1 = LOGN/2).  The first L1 stages only pair elements of the
+// same column, the last L2 stages only elements of the same row, so the
+// transform is two launches, each holding whole lines (columns or rows) of
+// one limb in LDS:
+//   column pass: a workgroup owns 16 adjacent columns (128-byte row segments,
+//                coalesced), all 2^L1 rows.
+//   row pass:    a workgroup owns 256/TPL whole rows (contiguous).
+// Inside a pass each thread keeps E = 2^A elements in VGPRs and runs A
+// stages as a radix-2^A butterfly network, exchanges through LDS once, then
+// runs the remaining B = LS - A stages on 2^(A-B) groups of 2^B elements.
+// Butterflies use Harvey's lazy reduction (values in [0, 4q) forward,
+// [0, 2q) inverse) with Shoup twiddle products, so a butterfly costs one
+// modular multiply.  Limbs (any mix of primes, any number of polynomials)
+// are the grid's y dimension, described by a LimbSpan.
+#pragma once
+#include "modarith.h"
+
+// One limb of work: pointer to N residues and its prime id.
+struct LimbTask {
+  u64* ptr;
+  u32 pid;
+  u32 aux;  // operation-specific (source limb index for fused prologues)
+};
+
+// A set of limbs: up to 8 segments of consecutive limbs (stride N) with
+// consecutive prime ids, passed by value, or a device task list.
+struct LimbSpan {
+  u64* seg_ptr[8];
+  u32 seg_pid[8];
+  u32 seg_end[8];  // cumulative limb count at the end of each segment
+  u32 nseg;
+  u32 total;
+  const LimbTask* list;  // if non-null, used instead of the segments
+};
+
+__device__ __forceinline__ void span_get(const LimbSpan& s, u32 y, u32 logN, u64*& ptr, u32& pid, u32& aux) {
+  if (s.list) {
+    const LimbTask t = s.list[y];
+    ptr = t.ptr;
+    pid = t.pid;
+    aux = t.aux;
+    return;
+  }
+  u32 g = 0, start = 0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    if (g + 1 < s.nseg && y >= s.seg_end[g]) {
+      start = s.seg_end[g];
+      ++g;
+    }
+  }
+  ptr = s.seg_ptr[g] ? s.seg_ptr[g] + ((u64)(y - start) << logN) : nullptr;
+  pid = s.seg_pid[g] + (y - start);
+  aux = y;
+}
+
+struct NttTables {
+  const u64* psi;      // [P][N] psi^brev(j)
+  const u64* psi_sh;   // Shoup companions
+  const u64* ipsi;     // [P][N] psi^-brev(j)
+  const u64* ipsi_sh;
+  const PrimeC* prm;   // [P]
+};
+
+// Fused prologue of the forward column pass.
+enum NttPro : int {
+  PRO_NONE = 0,
+  // input element = SwitchModulus(src[aux-th limb of pro_src], q_src -> q_pid)
+  // (OpenFHE NativeVector::SwitchModulus, reference src/CKKS/Rescale.cuh:47-68;
+  //  fused as in src/CKKS/LimbPartitionMGPU.cu:339-373)
+  PRO_SWITCH_MODULUS = 1,
+};
+
+// Fused epilogue of the forward row pass (final store).
+enum NttEpi : int {
+  EPI_NONE = 0,
+  // out = (x - ntt) * P^-1 (+ acc)   ModDown, reference src/CKKS/NTTfusions.cuh:237-252
+  EPI_MODDOWN = 1,
+  EPI_MODDOWN_ADD = 2,
+  // out = x * q_l^-1 + ntt * QlQlInvModqlDivqlModq   Rescale, src/CKKS/NTTfusions.cuh:214-235
+  EPI_RESCALE = 3,
+};
+
+struct NttFuse {
+  const u64* pro_src;  // PRO_SWITCH_MODULUS: coefficient-form source limb
+  u32 pro_src_pid;     //   and its prime id
+  LimbSpan x;          // EPI_*: the x operand, same limb order as the transformed span
+  LimbSpan acc;        // EPI_MODDOWN_ADD: accumulator (may alias the output)
+  LimbSpan out;        // EPI_*: destination (if nseg == 0 and list == null, in place)
+  const u64* c0;       // per-pid constant and Shoup companion
+  const u64* c0sh;
+  const u64* c1;
+  const u64* c1sh;
+};

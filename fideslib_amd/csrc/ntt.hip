@@ -1,0 +1,396 @@
+// Two-pass LDS negacyclic NTT / INTT kernels (see ntt.h for the design).
+#include "kernels.h"
+
+template <int LOGN, int PASS>
+struct NttCfg {
+  static constexpr int L1 = LOGN / 2;               // log #rows (column length)
+  static constexpr int L2 = LOGN - L1;              // log row length
+  static constexpr int LS = PASS == 0 ? L1 : L2;    // stages in this pass
+  static constexpr int A = (LS + 1) / 2;            // stages in register phase A
+  static constexpr int B = LS - A;                  // stages in register phase B
+  static constexpr int E = 1 << A;                  // elements per thread
+  static constexpr int G = 1 << (A - B);            // phase-B groups per thread
+  static constexpr int TPL = 1 << (LS - A);         // threads per line
+  static constexpr int M = 1 << LS;                 // line length
+  static constexpr int NLINES = PASS == 0 ? (1 << L2) : (1 << L1);
+  static constexpr int LPW0 = PASS == 0 ? 16 : 256 / TPL;
+  static constexpr int LPW = LPW0 < NLINES ? LPW0 : NLINES;  // lines per workgroup
+  static constexpr int THREADS = LPW * TPL;
+  static constexpr int S0 = PASS == 0 ? 0 : L1;     // first global stage
+  static constexpr int LDS_LINE = M + 1;            // padded line stride (u64)
+};
+
+// ---------------------------------------------------------------- butterflies
+// Forward Cooley-Tukey, lazy: X, Y in [0, 4q) -> [0, 4q).
+__device__ __forceinline__ void ct_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
+  const u64 x = csub(X, q2);
+  const u64 t = shoup_lazy(Y, w, wp, q);
+  X = x + t;
+  Y = x + q2 - t;
+}
+// Inverse Gentleman-Sande, lazy: X, Y in [0, 2q) -> [0, 2q).
+__device__ __forceinline__ void gs_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
+  const u64 s = csub(X + Y, q2);
+  Y = shoup_lazy(X + q2 - Y, w, wp, q);
+  X = s;
+}
+
+// ------------------------------------------------------------- prologue/epilogue
+__device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, u64 nm) {
+  // OpenFHE NativeVector::SwitchModulus (centred lift), reference
+  // src/CKKS/Rescale.cuh:14-40 (quoted) and :47-68.
+  const u64 halfQ = om >> 1;
+  if (nm > om) {
+    if (a > halfQ) a += nm - om;
+    return a;
+  }
+  const u64 diff = nm - (om % nm);
+  if (a > halfQ) a += diff;
+  if (a >= nm) a %= nm;
+  return a;
+}
+
+template <int EPI>
+__device__ __forceinline__ u64 epilogue(u64 v, u32 y, u32 idx, u32 logN, u32 pid, const PrimeC& P,
+                                        const NttFuse& F) {
+  if constexpr (EPI == EPI_NONE) {
+    return v;
+  } else {
+    u64* xp;
+    u32 xpid, xa;
+    span_get(F.x, y, logN, xp, xpid, xa);
+    const u64 x = xp[idx];
+    if constexpr (EPI == EPI_RESCALE) {
+      const u64 a = shoup(x, F.c0[pid], F.c0sh[pid], P.q);
+      const u64 b = shoup(v, F.c1[pid], F.c1sh[pid], P.q);
+      return add_mod(a, b, P.q);
+    } else {
+      u64 r = shoup(x + P.q - v, F.c0[pid], F.c0sh[pid], P.q);
+      if constexpr (EPI == EPI_MODDOWN_ADD) {
+        u64* ap;
+        u32 apid, aa;
+        span_get(F.acc, y, logN, ap, apid, aa);
+        if (ap) r = add_mod(r, ap[idx], P.q);
+      }
+      return r;
+    }
+  }
+}
+
+template <int EPI>
+__device__ __forceinline__ u64* epilogue_dst(u64* data, u32 y, u32 logN, const NttFuse& F) {
+  if constexpr (EPI == EPI_NONE) {
+    return data;
+  } else {
+    if (F.out.nseg == 0 && F.out.list == nullptr) return data;
+    u64* op;
+    u32 opid, oa;
+    span_get(F.out, y, logN, op, opid, oa);
+    return op;
+  }
+}
+
+// ------------------------------------------------------------------ forward
+// Column pass: global stages [0, L1).  Output kept lazy in [0, 4q).
+template <int LOGN, int PRO>
+__global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T, NttFuse F) {
+  using C = NttCfg<LOGN, 0>;
+  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  u64* data;
+  u32 pid, aux;
+  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  const u64 q = T.prm[pid].q, q2 = 2 * q;
+  const u64* tw = T.psi + ((u64)pid << LOGN);
+  const u64* tws = T.psi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
+  const u32 col = blockIdx.x * C::LPW + ll;
+  u64 x[C::E];
+  if constexpr (PRO == PRO_SWITCH_MODULUS) {
+    const u64 om = T.prm[F.pro_src_pid].q;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x[k] = switch_modulus(F.pro_src[((r0 + C::TPL * k) << C::L2) | col], om, q);
+  } else {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x[k] = data[((r0 + C::TPL * k) << C::L2) | col];
+  }
+#pragma unroll
+  for (int u = 0; u < C::A; ++u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << u) + (k >> (C::A - u));
+      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+  u64* L = lds + ll * C::LDS_LINE;
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = x[k];
+  __syncthreads();
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[(r1 << C::B) + k];
+#pragma unroll
+    for (int v = 0; v < C::B; ++v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << u) + (r1 << v) + (k >> (C::B - v));
+        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) data[(((r1 << C::B) + k) << C::L2) | col] = x[g * EB + k];
+  }
+}
+
+// Row pass: global stages [L1, LOGN).  Input in [0, 4q); output in [0, q).
+template <int LOGN, int EPI>
+__global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
+  using C = NttCfg<LOGN, 1>;
+  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  u64* data;
+  u32 pid, aux;
+  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  const PrimeC P = T.prm[pid];
+  const u64 q = P.q, q2 = 2 * q;
+  const u64* tw = T.psi + ((u64)pid << LOGN);
+  const u64* tws = T.psi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 r0 = tid % C::TPL, ll = tid / C::TPL;
+  const u32 row = blockIdx.x * C::LPW + ll;
+  u64* rowp = data + ((u64)row << C::L2);
+  u64 x[C::E];
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) x[k] = rowp[r0 + C::TPL * k];
+#pragma unroll
+  for (int u = 0; u < C::A; ++u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
+      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+  u64* L = lds + ll * C::LDS_LINE;
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = x[k];
+  __syncthreads();
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[(r1 << C::B) + k];
+#pragma unroll
+    for (int v = 0; v < C::B; ++v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
+        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
+  }
+  __syncthreads();
+  u64* dst = epilogue_dst<EPI>(data, blockIdx.y, LOGN, F) + ((u64)row << C::L2);
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) {
+    const u32 c = r0 + C::TPL * k;
+    u64 v = csub(csub(L[c], q2), q);
+    v = epilogue<EPI>(v, blockIdx.y, (row << C::L2) | c, LOGN, pid, P, F);
+    dst[c] = v;
+  }
+}
+
+// ------------------------------------------------------------------ inverse
+// Row pass: global stages LOGN-1 .. L1 (descending).  [0, q) -> [0, 2q).
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
+  using C = NttCfg<LOGN, 1>;
+  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  u64* data;
+  u32 pid, aux;
+  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  const u64 q = T.prm[pid].q, q2 = 2 * q;
+  const u64* tw = T.ipsi + ((u64)pid << LOGN);
+  const u64* tws = T.ipsi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 r0 = tid % C::TPL, ll = tid / C::TPL;
+  const u32 row = blockIdx.x * C::LPW + ll;
+  u64* rowp = data + ((u64)row << C::L2);
+  u64* L = lds + ll * C::LDS_LINE;
+  u64 x[C::E];
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = rowp[r0 + C::TPL * k];
+  __syncthreads();
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[(r1 << C::B) + k];
+#pragma unroll
+    for (int v = C::B - 1; v >= 0; --v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
+        gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) x[k] = L[r0 + C::TPL * k];
+#pragma unroll
+  for (int u = C::A - 1; u >= 0; --u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
+      gs_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) rowp[r0 + C::TPL * k] = x[k];
+}
+
+// Column pass: global stages L1-1 .. 0; N^-1 folded into stage 0.  [0, 2q) -> [0, q).
+template <int LOGN>
+__global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
+  using C = NttCfg<LOGN, 0>;
+  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  u64* data;
+  u32 pid, aux;
+  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  const PrimeC P = T.prm[pid];
+  const u64 q = P.q, q2 = 2 * q;
+  const u64* tw = T.ipsi + ((u64)pid << LOGN);
+  const u64* tws = T.ipsi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
+  const u32 col = blockIdx.x * C::LPW + ll;
+  u64* L = lds + ll * C::LDS_LINE;
+  u64 x[C::E];
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = data[(((r1 << C::B) + k) << C::L2) | col];
+#pragma unroll
+    for (int v = C::B - 1; v >= 0; --v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << u) + (r1 << v) + (k >> (C::B - v));
+        gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) x[k] = L[r0 + C::TPL * k];
+#pragma unroll
+  for (int u = C::A - 1; u >= 1; --u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << u) + (k >> (C::A - u));
+      gs_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+  {  // stage 0 with N^-1: X' = (X+Y) N^-1, Y' = (X-Y) psi^-brev(1) N^-1
+    constexpr int half = C::E >> 1;
+#pragma unroll
+    for (int k = 0; k < half; ++k) {
+      const u64 X = x[k], Y = x[k + half];
+      x[k] = shoup(X + Y, P.ninv, P.ninv_sh, q);
+      x[k + half] = shoup(X + q2 - Y, P.ninv_w, P.ninv_w_sh, q);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) data[((r0 + C::TPL * k) << C::L2) | col] = x[k];
+}
+
+// ------------------------------------------------------------------ launchers
+template <int LOGN>
+static void launch_fwd(const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F, hipStream_t st) {
+  using C0 = NttCfg<LOGN, 0>;
+  using C1 = NttCfg<LOGN, 1>;
+  dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
+  if (pro == PRO_SWITCH_MODULUS)
+    k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else
+    k_ntt_fwd_col<LOGN, PRO_NONE><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  switch (epi) {
+    case EPI_MODDOWN: k_ntt_fwd_row<LOGN, EPI_MODDOWN><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    case EPI_MODDOWN_ADD: k_ntt_fwd_row<LOGN, EPI_MODDOWN_ADD><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    case EPI_RESCALE: k_ntt_fwd_row<LOGN, EPI_RESCALE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    default: k_ntt_fwd_row<LOGN, EPI_NONE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+  }
+}
+
+template <int LOGN>
+static void launch_inv(const LimbSpan& s, const NttTables& T, hipStream_t st) {
+  using C0 = NttCfg<LOGN, 0>;
+  using C1 = NttCfg<LOGN, 1>;
+  dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
+  k_intt_row<LOGN><<<g1, C1::THREADS, 0, st>>>(s, T);
+  k_intt_col<LOGN><<<g0, C0::THREADS, 0, st>>>(s, T);
+}
+
+// Host entry points (called by the context code).  Return 0 or -1 for an
+// unsupported ring dimension.
+int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
+                     hipStream_t st) {
+  if (s.total == 0) return 0;
+  switch (logN) {
+    case 10: launch_fwd<10>(s, T, pro, epi, F, st); return 0;
+    case 11: launch_fwd<11>(s, T, pro, epi, F, st); return 0;
+    case 12: launch_fwd<12>(s, T, pro, epi, F, st); return 0;
+    case 13: launch_fwd<13>(s, T, pro, epi, F, st); return 0;
+    case 14: launch_fwd<14>(s, T, pro, epi, F, st); return 0;
+    case 15: launch_fwd<15>(s, T, pro, epi, F, st); return 0;
+    case 16: launch_fwd<16>(s, T, pro, epi, F, st); return 0;
+    case 17: launch_fwd<17>(s, T, pro, epi, F, st); return 0;
+    default: return -1;
+  }
+}
+
+int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st) {
+  if (s.total == 0) return 0;
+  switch (logN) {
+    case 10: launch_inv<10>(s, T, st); return 0;
+    case 11: launch_inv<11>(s, T, st); return 0;
+    case 12: launch_inv<12>(s, T, st); return 0;
+    case 13: launch_inv<13>(s, T, st); return 0;
+    case 14: launch_inv<14>(s, T, st); return 0;
+    case 15: launch_inv<15>(s, T, st); return 0;
+    case 16: launch_inv<16>(s, T, st); return 0;
+    case 17: launch_inv<17>(s, T, st); return 0;
+    default: return -1;
+  }
+}

@@ -1,0 +1,161 @@
+/*
+ * fideslib_amd.h -- C ABI of the MI355X (gfx950) CKKS key-switching core.
+ *
+ * The reference (FIDESlib) has no C ABI: its GPU seam is the C++ classes
+ * FIDESlib::CKKS::{Context, Ciphertext, KeySwitchingKey} fed by the
+ * interchange structs RawParams / RawCipherText / RawKeySwitchKey
+ * (src/CKKS/openfhe-interface/RawCiphertext.cuh:19-93).  This header is that
+ * seam as a C ABI: plain pointers and sizes, integer status returns, an
+ * explicit HIP stream on every operation, no exceptions and no C++ or torch
+ * types.  The host C++ API (fideslib::CryptoContext, ...) sits above it.
+ *
+ * Data formats
+ *   limb        N = 2^logN contiguous uint64 residues in [0, q), device memory
+ *   evaluation  OpenFHE bit-reversed NTT order:
+ *               NTT(a)[k] = sum_i a_i psi^(i (2 brev(k) + 1)) mod q
+ *               (reference src/CKKS/openfhe-interface/RawCiphertext.cu:47-73,
+ *                REV = 1 default = no reordering)
+ *   ciphertext  [2][l][N] uint64, evaluation form, l active Q limbs, limb i
+ *               under Q prime i (FIDESlib level convention l = level + 1,
+ *               api/Ciphertext.cpp:56-68)
+ *   prime ids   0..L-1 = Q primes, L..L+K-1 = special (P) primes
+ *   key         [2][dnum][L+K][N] uint64 host memory, component 0 = a,
+ *               component 1 = b (RawKeySwitchKey r_key = {A, B},
+ *               RawCiphertext.cu:649-668), evaluation form
+ *
+ * Threading: one context serves one stream at a time (its key-switch scratch
+ * buffers are shared by all calls); calls on different streams with the
+ * same context race.  Different contexts are independent.
+ */
+#ifndef FIDESLIB_AMD_H
+#define FIDESLIB_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (reference errors print and exit(0), src/CudaUtils.cuh;
+ * here every failure is returned). */
+#define FIDES_OK 0
+#define FIDES_ERR_ARG (-1)         /* invalid argument / precondition */
+#define FIDES_ERR_HIP (-2)         /* HIP runtime error */
+#define FIDES_ERR_UNSUPPORTED (-3) /* parameter outside the compiled kernels */
+#define FIDES_ERR_NOMEM (-4)       /* device allocation failed */
+
+typedef struct fides_context fides_context;
+typedef struct fides_key fides_key;
+
+/* Parameters (replaces RawParams, RawCiphertext.cuh:45-83, built by
+ * GetRawParams RawCiphertext.cu:213-643 and Parameters::adaptTo
+ * src/CKKS/Parameters.cu:113-140). */
+typedef struct fides_params {
+  uint32_t logN;       /* ring dimension N = 2^logN, 10 <= logN <= 17 */
+  uint32_t L;          /* number of Q primes (max level + 1), <= 64 */
+  uint32_t K;          /* number of special primes, 1..16 */
+  uint32_t dnum;       /* number of digits; alpha = ceil(L / dnum) <= 16 */
+  const uint64_t* q;   /* [L] Q primes, q = 1 mod 2N, q < 2^62 */
+  const uint64_t* p;   /* [K] special primes, same conditions */
+  const uint64_t* psi; /* [L+K] primitive 2N-th roots (RawParams
+                          root_of_unity / SPECIALroot_of_unity), or NULL to
+                          use g^((q-1)/2N) with g the smallest primitive root
+                          (reference fallback src/ConstantsGPU.cu:36-67) */
+} fides_params;
+
+/* Last error message of the calling thread. */
+const char* fides_last_error(void);
+
+/* Context: tables, per-level conversion constants and key-switch scratch
+ * (replaces GenCryptoContextGPU src/CKKS/Context.cu:972-990 +
+ * SetupConstants src/ConstantsGPU.cu:125-686). */
+int fides_context_create(const fides_params* params, fides_context** out);
+void fides_context_destroy(fides_context* ctx);
+int fides_context_info(const fides_context* ctx, uint32_t* logN, uint32_t* L, uint32_t* K, uint32_t* dnum,
+                       uint32_t* alpha);
+/* Roots actually used (psi, [L+K]) -- for callers that passed NULL. */
+int fides_context_roots(const fides_context* ctx, uint64_t* psi_out);
+
+/* Device memory and streams (stream = hipStream_t, NULL = default stream). */
+int fides_device_alloc(size_t bytes, void** dptr);
+int fides_device_free(void* dptr);
+int fides_copy_to_device(void* dst, const void* src, size_t bytes, void* stream);
+int fides_copy_to_host(void* dst, const void* src, size_t bytes, void* stream);
+int fides_stream_sync(void* stream);
+
+/* ---- raw limb operations (config: NTT/INTT + ModUp/ModDown) ---------- */
+
+/* In-place forward / inverse negacyclic NTT of nlimbs limbs at d_data,
+ * limb t under prime id pids[t] (host array).  Replaces
+ * LimbPartition::NTT / INTT launching NTT_ / INTT_ (src/NTT.cu:42-756). */
+int fides_ntt(fides_context* ctx, uint64_t* d_data, const uint32_t* pids, uint32_t nlimbs, void* stream);
+int fides_intt(fides_context* ctx, uint64_t* d_data, const uint32_t* pids, uint32_t nlimbs, void* stream);
+
+/* ModUp of digit `digit` of an evaluation-form polynomial with l Q limbs
+ * into Q_l u P: d_out receives l + K limbs (Q limbs 0..l-1, then P limbs),
+ * evaluation form; the digit's own limbs are copied.  Replaces
+ * RNSPoly::modup (src/CKKS/RNSPoly.cpp:283-322) ->
+ * DecompAndModUpConv (src/CKKS/Conv.cu:258-402). */
+int fides_modup(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint32_t digit, uint64_t* d_out,
+                void* stream);
+
+/* ModDown of l + K evaluation-form limbs (Q_l u P) to l limbs.  Replaces
+ * RNSPoly::moddown (src/CKKS/RNSPoly.cpp:584-624) -> ModDown2
+ * (src/CKKS/Conv.cu:26-125) + NTT_MODDOWN epilogue
+ * (src/CKKS/NTTfusions.cuh:237-252). */
+int fides_moddown(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint64_t* d_out, void* stream);
+
+/* ---- keys ------------------------------------------------------------ */
+
+/* Upload a key-switching key (relinearization, rotation or conjugation)
+ * from host memory [2][dnum][L+K][N] (a, b).  Replaces
+ * KeySwitchingKey::Initialize (src/CKKS/KeySwitchingKey.cu:20-35) fed by
+ * GetEvalKeySwitchKey / GetRotationKeySwitchKey (RawCiphertext.cu:645-760). */
+int fides_key_upload(fides_context* ctx, const uint64_t* h_key, fides_key** out);
+void fides_key_destroy(fides_key* key);
+
+/* Hybrid key switch of one evaluation-form polynomial d (l limbs):
+ * (out0, out1) with out0 + out1 * s_new = d * s_old + e.  Replaces
+ * RNSPoly::modup_ksk_moddown_mgpu (src/CKKS/RNSPoly.cpp:1112-1244,
+ * src/CKKS/LimbPartitionMGPU.cu:588-1873).  out0/out1: l limbs each. */
+int fides_keyswitch(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fides_key* key,
+                    uint64_t* d_out0, uint64_t* d_out1, void* stream);
+
+/* ---- ciphertext operations ([2][l][N] device buffers) ---------------- */
+
+/* HMult + relinearization (no rescale).  d_out may alias d_a.  Replaces
+ * Ciphertext::mult (src/CKKS/Ciphertext.cpp:458-677) behind
+ * CryptoContext::EvalMult (api/CryptoContext.cpp:1185-1210). */
+int fides_hmult(fides_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint32_t l, const fides_key* rlk,
+                uint64_t* d_out, void* stream);
+
+/* Squaring + relinearization (Ciphertext::square, Ciphertext.cpp:679-760). */
+int fides_square(fides_context* ctx, const uint64_t* d_a, uint32_t l, const fides_key* rlk, uint64_t* d_out,
+                 void* stream);
+
+/* Rescale: [2][l] -> [2][l-1] (OpenFHE DropLastElementAndScale).  d_out
+ * must not alias d_in.  Replaces Ciphertext::rescale
+ * (src/CKKS/Ciphertext.cpp:426-447) -> doubleRescaleMGPU
+ * (src/CKKS/LimbPartitionMGPU.cu:196-394). */
+int fides_rescale(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint64_t* d_out, void* stream);
+
+/* Automorphism X -> X^k (k odd) with its key: key switch of c1, add c0,
+ * then permute both components (OpenFHE EvalAutomorphism order; bit-exact
+ * with it).  Rotation by r uses k = 5^r mod 2N, conjugation k = 2N - 1.
+ * d_out must not alias d_in.  Replaces Ciphertext::rotate / conjugate
+ * (src/CKKS/Ciphertext.cpp:825-938) behind CryptoContext::EvalRotate
+ * (api/CryptoContext.cpp:1406-1430). */
+int fides_rotate(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fides_key* key, uint32_t auto_index,
+                 uint64_t* d_out, void* stream);
+
+/* Keyless evaluation-domain automorphism of nlimbs limbs (permutation only,
+ * reference automorph_ src/Rotation.cu:10-28).  d_out must not alias. */
+int fides_automorph(fides_context* ctx, const uint64_t* d_in, uint32_t nlimbs, uint32_t auto_index,
+                    uint64_t* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIDESLIB_AMD_H */

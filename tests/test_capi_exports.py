@@ -1,0 +1,62 @@
+"""CPU checks of the C-ABI boundary: the shared library loads and exports
+every function include/fideslib_amd.h declares, the Python binding covers
+exactly that set, and argument validation that needs no GPU rejects bad
+parameters.  No compute call is made (there is no GPU here)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from fideslib_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(capi.HEADER_PATH).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fides_[a-z0-9_]+)\s*\(", src)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(capi.LIB_PATH):
+        from fideslib_amd import build
+        build.build()
+    return capi.load()
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for must in ("fides_context_create", "fides_ntt", "fides_intt", "fides_modup", "fides_moddown",
+                 "fides_keyswitch", "fides_hmult", "fides_rescale", "fides_rotate", "fides_key_upload"):
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol(lib):
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+
+
+def test_binding_matches_header():
+    assert sorted(capi.SIGNATURES) == declared_functions()
+
+
+def test_context_rejects_bad_parameters(lib):
+    q = np.array([12289], np.uint64)  # 14-bit prime = 1 mod 2^12, not 1 mod 2^11 * 2 for logN=12? check below
+    p = np.array([40961], np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    h = ctypes.c_void_p()
+    # logN out of range
+    prm = capi.Params(9, 1, 1, 1, q.ctypes.data_as(u64p), p.ctypes.data_as(u64p), None)
+    assert lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)) == -3
+    # composite "prime"
+    bad = np.array([2 ** 20 * 3 + 1 + 2 ** 11 * 0], np.uint64)
+    prm = capi.Params(10, 1, 1, 1, bad.ctypes.data_as(u64p), p.ctypes.data_as(u64p), None)
+    assert lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)) == -1
+    assert b"prime" in lib.fides_last_error()
+    # K = 0
+    prm = capi.Params(10, 1, 0, 1, q.ctypes.data_as(u64p), p.ctypes.data_as(u64p), None)
+    assert lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)) == -1

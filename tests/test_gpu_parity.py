@@ -1,0 +1,297 @@
+"""GPU parity: every HIP operation, called through the C ABI, must equal the
+CPU oracle limb-for-limb (bit-exact; the arithmetic is exact modular
+integer arithmetic, so no tolerance applies).
+
+Sizes: N = 2^10 .. 2^13 for the exhaustive cases, N = 2^16 for the
+headline shapes (32 limbs of FIDESlib's 51-bit test primes; 30 Q limbs of
+60/59-bit primes with K = 10, dnum = 3).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fideslib_amd import capi, params
+from oracle import ckks_codec as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PRIMES = json.load(open(os.path.join(HERE, "golden", "reference_primes.json")))
+
+
+def rand_limbs(rng, primes, N):
+    return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+# ------------------------------------------------------------------ NTT
+@pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 16, 17])
+def test_ntt_intt_bit_exact(logN):
+    say("ntt logN=%d" % logN)
+    rng = np.random.default_rng(logN)
+    ok = [x for x in PRIMES["p64"] + PRIMES["sp64"] if (x - 1) % (2 << logN) == 0]
+    q = ok[:6]
+    p = ok[-3:]
+    octx = O.Context(logN, q, p, 2)
+    gctx = capi.Context(logN, q, p, 2)
+    assert gctx.roots == octx.roots
+    N = 1 << logN
+    pids = [0, 1, 2, 3, 4, 5, 6, 7, 8, 2, 8, 0]  # mixed, repeated, non-consecutive
+    prm = [octx.primes[i] for i in pids]
+    x = rand_limbs(rng, prm, N)
+    buf = capi.DeviceBuffer.from_array(x)
+    gctx.ntt(buf, pids)
+    got = buf.download(x.shape)
+    want = octx.ntt_limbs(x, pids)
+    assert np.array_equal(got, want)
+    gctx.intt(buf, pids)
+    assert np.array_equal(buf.download(x.shape), x)
+
+
+def test_ntt_edge_values():
+    """All-zero, all-(q-1) and single-impulse limbs."""
+    logN = 12
+    q = PRIMES["p64"][:2]
+    p = PRIMES["sp64"][:1]
+    octx = O.Context(logN, q, p, 1)
+    gctx = capi.Context(logN, q, p, 1)
+    N = 1 << logN
+    x = np.zeros((3, N), np.uint64)
+    x[1, :] = q[1] - 1
+    x[2, 5] = 1
+    pids = [0, 1, 2]
+    buf = capi.DeviceBuffer.from_array(x)
+    gctx.ntt(buf, pids)
+    assert np.array_equal(buf.download(x.shape), octx.ntt_limbs(x, pids))
+
+
+def test_ntt_many_segments_task_list():
+    """More than 8 prime runs forces the device task-list path."""
+    logN = 11
+    q = PRIMES["p64"][:12]
+    p = PRIMES["sp64"][:2]
+    octx = O.Context(logN, q, p, 3)
+    gctx = capi.Context(logN, q, p, 3)
+    rng = np.random.default_rng(1)
+    pids = [13, 0, 5, 2, 9, 11, 1, 7, 12, 3, 10, 4, 6, 8]
+    x = rand_limbs(rng, [octx.primes[i] for i in pids], 1 << logN)
+    buf = capi.DeviceBuffer.from_array(x)
+    gctx.ntt(buf, pids)
+    assert np.array_equal(buf.download(x.shape), octx.ntt_limbs(x, pids))
+
+
+# ------------------------------------------------------- ModUp / ModDown (N = 2^16, 32 + 8 limbs)
+@pytest.fixture(scope="module")
+def cfg2():
+    prm = params.config2()
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    return prm, octx, gctx
+
+
+def test_cfg2_ntt_roundtrip(cfg2):
+    prm, octx, gctx = cfg2
+    say("cfg2 ntt")
+    rng = np.random.default_rng(2)
+    x = rand_limbs(rng, prm.q, prm.N)
+    buf = capi.DeviceBuffer.from_array(x)
+    pids = list(range(prm.L))
+    gctx.ntt(buf, pids)
+    assert np.array_equal(buf.download(x.shape), octx.ntt_limbs(x, pids))
+    gctx.intt(buf, pids)
+    assert np.array_equal(buf.download(x.shape), x)
+
+
+@pytest.mark.parametrize("digit", [0, 3])
+def test_cfg2_modup_moddown(cfg2, digit):
+    prm, octx, gctx = cfg2
+    say("cfg2 modup digit %d" % digit)
+    rng = np.random.default_rng(10 + digit)
+    l = prm.L
+    x = rand_limbs(rng, prm.q, prm.N)
+    inp = capi.DeviceBuffer.from_array(x)
+    out = capi.DeviceBuffer((l + prm.K) * prm.N)
+    gctx.modup(inp, l, digit, out)
+    got = out.download((l + prm.K, prm.N))
+    want = octx.modup_digit(x, l, digit)
+    assert np.array_equal(got, want)
+    # ModDown 40 -> 32 of the ModUp output and of a random extended polynomial
+    y = rand_limbs(rng, prm.q + prm.p, prm.N)
+    for src in (want, y):
+        ib = capi.DeviceBuffer.from_array(src)
+        ob = capi.DeviceBuffer(l * prm.N)
+        gctx.moddown(ib, l, ob)
+        assert np.array_equal(ob.download((l, prm.N)), octx.moddown(src, l))
+        assert np.array_equal(ib.download(src.shape), src), "moddown must not modify its input"
+
+
+@pytest.mark.parametrize("l", [1, 7, 8, 9, 20])
+def test_partial_digits_modup_moddown(cfg2, l):
+    """Levels whose last digit is partial, and single-limb levels."""
+    prm, octx, gctx = cfg2
+    rng = np.random.default_rng(100 + l)
+    x = rand_limbs(rng, prm.q[:l], prm.N)
+    inp = capi.DeviceBuffer.from_array(x)
+    beta = gctx.digits(l)
+    for j in range(beta):
+        out = capi.DeviceBuffer((l + prm.K) * prm.N)
+        gctx.modup(inp, l, j, out)
+        assert np.array_equal(out.download((l + prm.K, prm.N)), octx.modup_digit(x, l, j)), (l, j)
+    y = rand_limbs(rng, prm.q[:l] + prm.p, prm.N)
+    ib = capi.DeviceBuffer.from_array(y)
+    ob = capi.DeviceBuffer(l * prm.N)
+    gctx.moddown(ib, l, ob)
+    assert np.array_equal(ob.download((l, prm.N)), octx.moddown(y, l))
+
+
+# ------------------------------------------------------- ciphertext ops, N = 2^13 (4 limbs)
+@pytest.fixture(scope="module")
+def cfg1():
+    prm = params.config1()
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 11)
+    return prm, octx, gctx, keys
+
+
+def _enc_pair(prm, keys, seed, l):
+    N = prm.N
+    rng = np.random.default_rng(seed)
+    z = [rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2) for _ in range(2)]
+    sc = 2.0 ** prm.scale_bits
+    return z, [keys.encrypt(C.encode(v, sc, N), l) for v in z]
+
+
+def test_cfg1_keyswitch_hmult_rescale_rotate(cfg1):
+    prm, octx, gctx, keys = cfg1
+    say("cfg1 ct ops")
+    N, l = prm.N, prm.L
+    rlk = keys.relin_key()
+    z, (c1, c2) = _enc_pair(prm, keys, 5, l)
+    g_rlk = gctx.upload_key(rlk)
+    # key switch alone
+    d = capi.DeviceBuffer.from_array(c2[1])
+    o0, o1 = capi.DeviceBuffer(l * N), capi.DeviceBuffer(l * N)
+    gctx.keyswitch(d, l, g_rlk, o0, o1)
+    w0, w1 = octx.keyswitch(c2[1], l, rlk)
+    assert np.array_equal(o0.download((l, N)), w0)
+    assert np.array_equal(o1.download((l, N)), w1)
+    # HMult + relin
+    a, b = capi.DeviceBuffer.from_array(c1), capi.DeviceBuffer.from_array(c2)
+    m = capi.DeviceBuffer(2 * l * N)
+    gctx.hmult(a, b, l, g_rlk, m)
+    want_m = octx.hmult_relin(c1, c2, l, rlk)
+    assert np.array_equal(m.download((2, l, N)), want_m)
+    # in-place HMult (out aliases a)
+    gctx.hmult(a, b, l, g_rlk, a)
+    assert np.array_equal(a.download((2, l, N)), want_m)
+    # Rescale
+    r = capi.DeviceBuffer(2 * (l - 1) * N)
+    gctx.rescale(m, l, r)
+    want_r = octx.rescale_ct(want_m, l)
+    assert np.array_equal(r.download((2, l - 1, N)), want_r)
+    # Rotate by +1, -1 and conjugate at l - 1
+    for k in (5, pow(5, -1, 2 * N), 2 * N - 1):
+        key = keys.rotation_key(k)
+        gk = gctx.upload_key(key)
+        o = capi.DeviceBuffer(2 * (l - 1) * N)
+        gctx.rotate(r, l - 1, gk, k, o)
+        assert np.array_equal(o.download((2, l - 1, N)), octx.rotate(want_r, l - 1, key, k)), k
+    # decrypted result against FP64
+    sc2 = (2.0 ** prm.scale_bits) ** 2 / prm.q[l - 1]
+    dec = C.decode(keys.decrypt_coef(r.download((2, l - 1, N)), l - 1), sc2, N)
+    assert C.precision_bits(dec, z[0] * z[1]) > 20
+
+
+def test_cfg1_square_and_lower_levels(cfg1):
+    prm, octx, gctx, keys = cfg1
+    N = prm.N
+    rlk = keys.relin_key()
+    g_rlk = gctx.upload_key(rlk)
+    for l in (1, 2, 3):
+        z, (c1, c2) = _enc_pair(prm, keys, 20 + l, l)
+        a = capi.DeviceBuffer.from_array(c1)
+        o = capi.DeviceBuffer(2 * l * N)
+        gctx.square(a, l, g_rlk, o)
+        assert np.array_equal(o.download((2, l, N)), octx.hmult_relin(c1, c1, l, rlk)), l
+
+
+def test_automorph_keyless(cfg1):
+    prm, octx, gctx, keys = cfg1
+    rng = np.random.default_rng(4)
+    x = rand_limbs(rng, prm.q, prm.N)
+    for k in (5, 25, 2 * prm.N - 1, 3):
+        ib = capi.DeviceBuffer.from_array(x)
+        ob = capi.DeviceBuffer(x.size)
+        gctx.automorph(ib, prm.L, k, ob)
+        assert np.array_equal(ob.download(x.shape), octx.automorph(x, k)), k
+
+
+def test_error_paths(cfg1):
+    prm, octx, gctx, keys = cfg1
+    N = prm.N
+    a = capi.DeviceBuffer(2 * prm.L * N)
+    with pytest.raises(capi.FidesError):
+        gctx.rescale(a, 1, capi.DeviceBuffer(2 * N))  # l < 2
+    with pytest.raises(capi.FidesError):
+        gctx.rescale(a, 2, a)  # aliasing
+    with pytest.raises(capi.FidesError):
+        gctx.automorph(a, 1, 4, capi.DeviceBuffer(N))  # even index
+    with pytest.raises(capi.FidesError):
+        gctx.ntt(a, [99])  # prime id out of range
+
+
+# ------------------------------------------------------- headline shape, N = 2^16, L = 29 (30 limbs)
+@pytest.fixture(scope="module")
+def cfg3():
+    prm = params.config3()
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 29)
+    return prm, octx, gctx, keys
+
+
+@pytest.mark.parametrize("l", [30, 16, 6])
+def test_cfg3_hmult_rescale_bit_exact(cfg3, l):
+    prm, octx, gctx, keys = cfg3
+    say("cfg3 hmult l=%d" % l)
+    N = prm.N
+    rlk = keys.relin_key()
+    z, (c1, c2) = _enc_pair(prm, keys, 300 + l, l)
+    g_rlk = gctx.upload_key(rlk)
+    a, b = capi.DeviceBuffer.from_array(c1), capi.DeviceBuffer.from_array(c2)
+    m = capi.DeviceBuffer(2 * l * N)
+    r = capi.DeviceBuffer(2 * (l - 1) * N)
+    gctx.hmult(a, b, l, g_rlk, m)
+    gctx.rescale(m, l, r)
+    want = octx.rescale_ct(octx.hmult_relin(c1, c2, l, rlk), l)
+    got = r.download((2, l - 1, N))
+    assert np.array_equal(got, want)
+    sc2 = (2.0 ** prm.scale_bits) ** 2 / prm.q[l - 1]
+    dec = C.decode(keys.decrypt_coef(got, l - 1), sc2, N)
+    assert C.precision_bits(dec, z[0] * z[1]) > 30
+
+
+def test_cfg3_rotate_bit_exact(cfg3):
+    prm, octx, gctx, keys = cfg3
+    say("cfg3 rotate")
+    N, l = prm.N, 30
+    rng = np.random.default_rng(77)
+    r = int(rng.integers(1, 1 << 15))
+    k = pow(5, r, 2 * N)
+    key = keys.rotation_key(k)
+    z, (c1, _) = _enc_pair(prm, keys, 301, l)
+    g = gctx.upload_key(key)
+    a = capi.DeviceBuffer.from_array(c1)
+    o = capi.DeviceBuffer(2 * l * N)
+    gctx.rotate(a, l, g, k, o)
+    got = o.download((2, l, N))
+    assert np.array_equal(got, octx.rotate(c1, l, key, k))
+    dec = C.decode(keys.decrypt_coef(got, l), 2.0 ** prm.scale_bits, N)
+    assert C.precision_bits(dec, np.roll(z[0], -r)) > 30
