@@ -1,0 +1,244 @@
+"""Benchmark of the hybrid key-switching hot path on one MI355X.
+
+Workload (the headline configuration): N = 2^16, 30 Q limbs (one 60-bit +
+29 59-bit primes, "L = 29"), K = 10 special 60-bit primes, dnum = 3.  One
+step processes a batch of 64 ciphertexts whose limb counts are drawn
+uniformly from 6..30 (levels 5..29) with a fixed seed; each ciphertext gets
+HMult + relinearize + Rescale and a Rotate by an index drawn uniformly from
+[1, 2^15).  Every HMult and every Rotate is one key switch, so a step is
+128 key-switch operations; `value` is key-switch ops/s with all inputs and
+keys resident in HBM.  Ciphertext and key limbs are synthetic (seeded,
+uniform mod q_i): throughput does not depend on their values.
+
+The roofline entry is for the forward NTT (column + row pass), timed live
+with HIP events on the stream it runs on; the CPU baseline is the repo's
+C oracle (own CPU reference, not OpenFHE) on a bounded sample of the same
+operation.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from fideslib_amd import capi, params  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20261203
+NCT = 64
+
+
+# ----------------------------------------------------------------- HIP events
+class HipEvents:
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream=None):
+        assert self.hip.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def rand_limbs(rng, primes, N):
+    return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in primes])
+
+
+# ----------------------------------------------------------------- workload
+class Workload:
+    def __init__(self, prm, ctx, rng):
+        N, L = prm.N, prm.L
+        self.levels = [int(x) for x in rng.integers(6, L + 1, NCT)]
+        self.rots = [int(x) for x in rng.integers(1, 1 << 15, NCT)]
+        self.kidx = [pow(5, r, 2 * N) for r in self.rots]
+        pool_a = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])  # [2][L][N]
+        pool_b = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])
+        self.a, self.b, self.m, self.r, self.o = [], [], [], [], []
+        for l in self.levels:
+            self.a.append(capi.DeviceBuffer.from_array(np.ascontiguousarray(pool_a[:, :l])))
+            self.b.append(capi.DeviceBuffer.from_array(np.ascontiguousarray(pool_b[:, :l])))
+            self.m.append(capi.DeviceBuffer(2 * l * N))
+            self.r.append(capi.DeviceBuffer(2 * (l - 1) * N))
+            self.o.append(capi.DeviceBuffer(2 * (l - 1) * N))
+        LK = prm.L + prm.K
+        key_host = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)])
+                             for _ in range(2)])  # [2][dnum][L+K][N]
+        assert key_host.shape == (2, prm.dnum, LK, N)
+        self.rlk = ctx.upload_key(key_host)
+        # one resident key per rotation index (same synthetic content, distinct HBM copies)
+        self.rot_keys = [ctx.upload_key(key_host) for _ in range(NCT)]
+
+    def step_batched(self, ctx, stream=None):
+        ctx.hmult_batch(self.a, self.b, self.levels, self.rlk, self.r, rescale=True, stream=stream)
+        ctx.rotate_batch(self.r, [l - 1 for l in self.levels], self.rot_keys, self.kidx, self.o, stream=stream)
+
+    def step_plain(self, ctx, stream=None):
+        for i, l in enumerate(self.levels):
+            ctx.hmult(self.a[i], self.b[i], l, self.rlk, self.m[i], stream)
+            ctx.rescale(self.m[i], l, self.r[i], stream)
+            ctx.rotate(self.r[i], l - 1, self.rot_keys[i], self.kidx[i], self.o[i], stream)
+
+
+def ntt_roofline(ctx, prm, ev, reps=20):
+    """Forward NTT (column + row pass) over 30 limbs, timed with HIP events."""
+    rng = np.random.default_rng(SEED + 1)
+    nl = 30
+    x = rand_limbs(rng, prm.q[:nl], prm.N)
+    buf = capi.DeviceBuffer.from_array(x)
+    pids = list(range(nl))
+    for _ in range(3):
+        ctx.ntt(buf, pids)
+    ctx.sync()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0)
+    for _ in range(reps):
+        ctx.ntt(buf, pids)
+    ev.record(e1)
+    ms = ev.elapsed_ms(e0, e1) / reps
+    alg_bytes = 2 * prm.N * 8 * nl
+    achieved = alg_bytes / (ms * 1e-3) / 1e9
+    return {
+        "kernel": "ntt_fwd (k_ntt_fwd_col + k_ntt_fwd_row), 30 limbs x 2^16",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "launch_us": round(ms * 1e3, 2),
+        "alg_bytes_per_launch": alg_bytes,
+    }
+
+
+def cpu_baseline(prm, budget_s=12.0):
+    """Oracle HMult + relin + Rescale at the same parameters, levels drawn as
+    in the workload, for about budget_s seconds (own CPU reference)."""
+    from oracle import oracle as O
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(SEED + 2)
+    N = prm.N
+    key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    levels = [int(x) for x in rng.integers(6, prm.L + 1, 64)]
+    done, t0 = 0, time.time()
+    ops = []
+    for l in levels:
+        a = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        b = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        t = time.time()
+        octx.rescale_ct(octx.hmult_relin(a, b, l, key), l)
+        ops.append(time.time() - t)
+        done += 1
+        if time.time() - t0 > budget_s:
+            break
+    el = sum(ops)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": round(done / el, 3),
+        "unit": "key-switch ops/s (HMult+relin+Rescale)",
+        "cores": cores,
+        "kind": "port",
+        "sample": "%d HMult+relin+Rescale at N=2^16, limbs %s.. (levels U{5..29}), %.1f s; "
+                  "own CPU oracle (OpenMP), not OpenFHE" % (done, levels[:done][:4], el),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mode", default="batched", choices=["plain", "batched"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(batches are independent per GPU and shard by ciphertext).", file=sys.stderr)
+        sys.exit(2)
+    prm = params.config3()
+    ctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(SEED)
+    print("bench: building workload (64 ciphertexts, 65 keys)", file=sys.stderr, flush=True)
+    w = Workload(prm, ctx, rng)
+    ev = HipEvents()
+
+    def timed(step):
+        for _ in range(args.warmup):
+            step(ctx)
+        ctx.sync()
+        e0, e1 = ev.event(), ev.event()
+        t0 = time.time()
+        ev.record(e0)
+        for _ in range(args.steps):
+            step(ctx)
+        ev.record(e1)
+        ms = ev.elapsed_ms(e0, e1)
+        ctx.sync()
+        return ms / args.steps, time.time() - t0
+
+    ms_plain, wall_plain = timed(w.step_plain)
+    print("bench: plain %.3f ms/step" % ms_plain, file=sys.stderr, flush=True)
+    ms_batch, wall_batch = timed(w.step_batched)
+    print("bench: batched %.3f ms/step" % ms_batch, file=sys.stderr, flush=True)
+    ops_per_step = 2 * NCT
+    ms_step = ms_batch if args.mode == "batched" else ms_plain
+    wall = wall_batch if args.mode == "batched" else wall_plain
+    value = ops_per_step / (ms_step * 1e-3)
+    roof = ntt_roofline(ctx, prm, ev)
+    out = {
+        "metric": "KeySwitch ops/s N=2^16 L=29 batched vs plain, bit-exact; serve p50/p99 latency",
+        "value": round(value, 2),
+        "unit": "key-switch ops/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "wall_s": round(wall, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded uniform residues; values do not affect timing)",
+        "config": {
+            "workload": "cfg3: N=2^16, 30 Q limbs (60+29x59 bit), K=10 x 60-bit special, dnum=3; "
+                        "64 ciphertexts, limbs U{6..30}; per ct HMult+relin+Rescale and Rotate r~U[1,2^15)",
+            "mode": args.mode,
+            "ops_per_step": ops_per_step,
+            "plain_ops_per_s": round(ops_per_step / (ms_plain * 1e-3), 2),
+            "batched_ops_per_s": round(ops_per_step / (ms_batch * 1e-3), 2),
+            "batched_over_plain": round(ms_plain / ms_batch, 3),
+            "K": prm.K,
+            "seed": SEED,
+        },
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(prm)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -52,6 +52,7 @@ SIGNATURES = {
     "fides_device_free": (_int, [_vp]),
     "fides_copy_to_device": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "fides_copy_to_host": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "fides_copy_device": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "fides_stream_sync": (_int, [_vp]),
     "fides_ntt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
     "fides_intt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
@@ -65,6 +66,10 @@ SIGNATURES = {
     "fides_rescale": (_int, [_vp, _vp, _u32, _vp, _vp]),
     "fides_rotate": (_int, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "fides_automorph": (_int, [_vp, _vp, _u32, _u32, _vp, _vp]),
+    "fides_hmult_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, _vp, _int,
+                                 ctypes.POINTER(_vp), _vp]),
+    "fides_rotate_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _u32p,
+                                  ctypes.POINTER(_vp), _vp]),
 }
 
 _lib = None
@@ -211,6 +216,25 @@ class Context:
 
     def automorph(self, inp, nlimbs, k, out, stream=None):
         check(load().fides_automorph(self.h, _ptr(inp), nlimbs, k, _ptr(out), stream), "fides_automorph")
+
+    def hmult_batch(self, a, b, levels, rlk, out, rescale=True, stream=None):
+        n = len(levels)
+        A = (_vp * n)(*[_ptr(x) for x in a])
+        B = (_vp * n)(*[_ptr(x) for x in b])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_hmult_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), rlk.h, int(bool(rescale)), O,
+                                       stream), "fides_hmult_batch")
+
+    def rotate_batch(self, inp, levels, keys, kidx, out, stream=None):
+        n = len(levels)
+        I = (_vp * n)(*[_ptr(x) for x in inp])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        Kh = (_vp * n)(*[k.h for k in keys])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        kx = np.ascontiguousarray(kidx, np.uint32)
+        check(load().fides_rotate_batch(self.h, n, I, lv.ctypes.data_as(_u32p), Kh, kx.ctypes.data_as(_u32p), O,
+                                        stream), "fides_rotate_batch")
 
     def sync(self, stream=None):
         check(load().fides_stream_sync(stream), "sync")

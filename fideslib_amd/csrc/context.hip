@@ -1,5 +1,11 @@
-// Host side of the C ABI: context tables, key storage and the key-switch
-// pipelines that sequence the HIP kernels.
+// Host side of the C ABI: context tables, key storage and the executor that
+// sequences the HIP kernels of the key-switching pipelines.
+//
+// Every ciphertext operation (single or batched) is a list of "jobs"; the
+// executor builds the per-limb task records of every pipeline stage on the
+// host, uploads them in one copy through a pinned ring of task buffers, and
+// issues one launch per stage for the whole list.  A single-ciphertext call
+// is a list of one job, so both paths run the same kernels.
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -22,11 +28,17 @@ static int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(x)                                                                        \
-  do {                                                                                   \
-    hipError_t e_ = (x);                                                                 \
-    if (e_ != hipSuccess)                                                                \
-      return fail(FIDES_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));       \
+#define HIPCHK(x)                                                                  \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess)                                                          \
+      return fail(FIDES_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));  \
+  } while (0)
+
+#define RCHK(x)               \
+  do {                        \
+    int r_ = (x);             \
+    if (r_ != FIDES_OK) return r_; \
   } while (0)
 
 extern "C" const char* fides_last_error(void) { return g_err.c_str(); }
@@ -141,12 +153,23 @@ static u64 default_root(u64 q, u32 logN) {
 static u64 shoup_of(u64 w, u64 q) { return (u64)(((u128)w << 64) / q); }
 static u32 bitlen(u64 x) { return 64 - __builtin_clzll(x); }
 
+
 // ======================================================================
 // context
 // ======================================================================
 struct fides_key {
   fides_context* ctx;
   u64* d;  // [2][dnum][L+K][N] Montgomery form
+};
+
+constexpr int RING = 16;  // task-buffer slots in flight
+
+struct TaskSlot {
+  char* host = nullptr;  // pinned
+  char* dev = nullptr;
+  size_t cap = 0;
+  hipEvent_t done = nullptr;
+  bool used = false;
 };
 
 struct fides_context {
@@ -157,18 +180,19 @@ struct fides_context {
   PrimeC* prm = nullptr;
   u64* tw = nullptr;  // psi | psi_sh | ipsi | ipsi_sh, each [(L+K) N]
   NttTables T{};
-  u64* pinv = nullptr;     // [L] P^-1 mod q_i, and Shoup at [L..2L)
-  u64* resc = nullptr;     // [(L+1)][4][L]: qlinv, qlinv_sh, QlQlInv, QlQlInv_sh rows per level l
   ConvDesc* modup = nullptr;    // [(L+1) * dnum]
   ConvDesc* moddown = nullptr;  // [(L+1)]
   std::vector<ConvDesc> hmodup, hmoddown;
-  LimbTask* ext_tasks = nullptr;  // per level: NTT task list of the ModUp outputs
-  std::vector<u32> ext_task_off, ext_task_cnt;
-  LimbTask* tmp_tasks = nullptr;  // scratch task list for fides_ntt with many segments
-  u32 tmp_tasks_cap = 0;
-  // key-switch scratch (sized for l = L)
-  u64 *Dc = nullptr, *EXT = nullptr, *T0 = nullptr, *T1 = nullptr, *Y = nullptr, *D = nullptr,
-      *TMP = nullptr, *RS = nullptr;
+  // host copies of per-limb scalars
+  std::vector<u64> pinv, pinv_sh;                    // [L]
+  std::vector<u64> qlinv, qlinv_sh, qlql, qlql_sh;   // [(L+1) * L], row l = dropping q_{l-1}
+  // scratch arena (grows on demand) and task ring
+  u64* arena = nullptr;
+  size_t arena_words = 0;
+  TaskSlot ring[RING];
+  int ring_next = 0;
+  NttTask* raw_tasks = nullptr;  // fides_ntt with more than 8 prime runs
+  u32 raw_tasks_cap = 0;
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -194,6 +218,11 @@ static void span_add(LimbSpan& s, u64* ptr, u32 pid, u32 n) {
   s.total += n;
   s.seg_end[s.nseg] = s.total;
   s.nseg++;
+}
+static NttFuse fuse0() {
+  NttFuse F;
+  memset(&F, 0, sizeof(F));
+  return F;
 }
 
 // Conversion descriptor for source primes src -> targets dst (dst_off given).
@@ -231,12 +260,25 @@ static void digit_range(const fides_context* c, u32 l, u32 j, u32& st, u32& en) 
 }
 static u32 num_digits(const fides_context* c, u32 l) { return (l + c->alpha - 1) / c->alpha; }
 
+extern "C" void fides_context_destroy(fides_context* c) {
+  if (!c) return;
+  (void)hipDeviceSynchronize();
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->arena) (void)hipFree(c->arena);
+  for (auto& s : c->ring) {
+    if (s.host) (void)hipHostFree(s.host);
+    if (s.dev) (void)hipFree(s.dev);
+    if (s.done) (void)hipEventDestroy(s.done);
+  }
+  delete c;
+}
+
 extern "C" int fides_context_create(const fides_params* P, fides_context** out) {
   if (!P || !out) return fail(FIDES_ERR_ARG, "null argument");
   *out = nullptr;
   if (P->logN < 10 || P->logN > 17) return fail(FIDES_ERR_UNSUPPORTED, "logN must be in [10, 17]");
-  if (P->L == 0 || P->L > 64 || P->K == 0 || P->K > CONV_MAXS || P->dnum == 0)
-    return fail(FIDES_ERR_ARG, "need 1 <= L <= 64, 1 <= K <= 16, dnum >= 1");
+  if (!P->q || !P->p || P->L == 0 || P->L > 64 || P->K == 0 || P->K > CONV_MAXS || P->dnum == 0)
+    return fail(FIDES_ERR_ARG, "need 1 <= L <= 64, 1 <= K <= 16, dnum >= 1 and prime arrays");
   const u32 alpha = (P->L + P->dnum - 1) / P->dnum;
   if (alpha > CONV_MAXS) return fail(FIDES_ERR_UNSUPPORTED, "digit size alpha = ceil(L/dnum) must be <= 16");
   if (P->L + P->K > CONV_MAXT) return fail(FIDES_ERR_UNSUPPORTED, "L + K too large");
@@ -258,17 +300,17 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   c->dnum = P->dnum;
   c->alpha = alpha;
   c->primes = pr;
-  const u32 NP = P->L + P->K, N = c->N;
+  const u32 NP = P->L + P->K, N = c->N, L = c->L, K = c->K;
   c->roots.resize(NP);
   for (u32 i = 0; i < NP; ++i) {
-    u64 psi = P->psi ? P->psi[i] : default_root(pr[i], P->logN);
-    if (powm(psi, N, pr[i]) != pr[i] - 1) {
+    const u64 psi = P->psi ? P->psi[i] : default_root(pr[i], P->logN);
+    if (psi == 0 || powm(psi, N, pr[i]) != pr[i] - 1) {
       delete c;
       return fail(FIDES_ERR_ARG, "psi[" + std::to_string(i) + "] is not a primitive 2N-th root of unity");
     }
     c->roots[i] = psi;
   }
-  // ---- per-prime constants and twiddle tables
+  // ---- per-prime constants and twiddle tables (psi^brev(j), psi^-brev(j), Shoup companions)
   std::vector<u64> tw((size_t)4 * NP * N);
   c->hprm.resize(NP);
   auto work = [&](u32 i) {
@@ -277,7 +319,7 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     memset(&p, 0, sizeof(p));
     p.q = q;
     p.q2 = 2 * q;
-    u64 inv = q;  // Newton: q^-1 mod 2^64
+    u64 inv = q;  // Newton iteration for q^-1 mod 2^64
     for (int it = 0; it < 6; ++it) inv *= 2 - q * inv;
     p.qneg_inv = (u64)0 - inv;
     p.bits = bitlen(q);
@@ -287,8 +329,7 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     const u64 ipsi = invm(c->roots[i], q);
     p.ninv = invm(N % q, q);
     p.ninv_sh = shoup_of(p.ninv, q);
-    // psi_rev[1] = psi^brev(1) = psi^(N/2); inverse table entry ipsi^(N/2)
-    const u64 w1 = powm(ipsi, N / 2, q);
+    const u64 w1 = powm(ipsi, N / 2, q);  // psi^-brev(1)
     p.ninv_w = mulm(w1, p.ninv, q);
     p.ninv_w_sh = shoup_of(p.ninv_w, q);
     u64* t_psi = tw.data() + ((size_t)0 * NP + i) * N;
@@ -319,20 +360,20 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     for (auto& t : th) t.join();
   }
   int rc;
-#define ALLOC(ptr, bytes)                                          \
-  do {                                                             \
-    if ((rc = dalloc(c, (void**)&(ptr), (bytes))) != FIDES_OK) {   \
-      fides_context_destroy(c);                                    \
-      return rc;                                                   \
-    }                                                              \
+#define ALLOC(ptr, bytes)                                        \
+  do {                                                           \
+    if ((rc = dalloc(c, (void**)&(ptr), (bytes))) != FIDES_OK) { \
+      fides_context_destroy(c);                                  \
+      return rc;                                                 \
+    }                                                            \
   } while (0)
-#define H2D(dst, src, bytes)                                                         \
-  do {                                                                               \
-    hipError_t e_ = hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice);         \
-    if (e_ != hipSuccess) {                                                          \
-      fides_context_destroy(c);                                                      \
-      return fail(FIDES_ERR_HIP, std::string("upload: ") + hipGetErrorString(e_));   \
-    }                                                                                \
+#define H2D(dst, src, bytes)                                                       \
+  do {                                                                             \
+    hipError_t e_ = hipMemcpy((dst), (src), (bytes), hipMemcpyHostToDevice);       \
+    if (e_ != hipSuccess) {                                                        \
+      fides_context_destroy(c);                                                    \
+      return fail(FIDES_ERR_HIP, std::string("upload: ") + hipGetErrorString(e_)); \
+    }                                                                              \
   } while (0)
   ALLOC(c->prm, sizeof(PrimeC) * NP);
   H2D(c->prm, c->hprm.data(), sizeof(PrimeC) * NP);
@@ -343,95 +384,79 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   c->T.ipsi = c->tw + (size_t)2 * NP * N;
   c->T.ipsi_sh = c->tw + (size_t)3 * NP * N;
   c->T.prm = c->prm;
-  // ---- ModDown / rescale scalars
-  const u32 L = c->L, K = c->K;
-  {
-    std::vector<u64> pinv(2 * L);
-    for (u32 i = 0; i < L; ++i) {
-      const u64 qi = pr[i];
-      u64 Pm = 1;
-      for (u32 k = 0; k < K; ++k) Pm = mulm(Pm, pr[L + k] % qi, qi);
-      pinv[i] = invm(Pm, qi);
-      pinv[L + i] = shoup_of(pinv[i], qi);
-    }
-    ALLOC(c->pinv, pinv.size() * 8);
-    H2D(c->pinv, pinv.data(), pinv.size() * 8);
-    // rescale rows for l = 2..L: dropping q_{l-1}
-    std::vector<u64> rs((size_t)(L + 1) * 4 * L, 0);
-    for (u32 l = 2; l <= L; ++l) {
-      const u64 qt = pr[l - 1];
-      for (u32 i = 0; i + 1 < l; ++i) {
-        const u64 qi = pr[i];
-        const u64 qtinv = invm(qt % qi, qi);
-        // QlQlInvModqlDivqlModq_i = floor(Q' [Q'^-1]_qt / qt) mod q_i = -qt^-1 mod q_i
-        // (Q' = prod_{i<l-1} q_i; Q' [Q'^-1]_qt = 1 mod qt and = 0 mod q_i)
-        const u64 big = (qi - qtinv) % qi;
-        u64* row = rs.data() + (size_t)l * 4 * L;
-        row[0 * L + i] = qtinv;
-        row[1 * L + i] = shoup_of(qtinv, qi);
-        row[2 * L + i] = big;
-        row[3 * L + i] = shoup_of(big, qi);
-      }
-    }
-    ALLOC(c->resc, rs.size() * 8);
-    H2D(c->resc, rs.data(), rs.size() * 8);
+  // ---- ModDown and rescale scalars
+  c->pinv.resize(L);
+  c->pinv_sh.resize(L);
+  for (u32 i = 0; i < L; ++i) {
+    const u64 qi = pr[i];
+    u64 Pm = 1;
+    for (u32 k = 0; k < K; ++k) Pm = mulm(Pm, pr[L + k] % qi, qi);
+    c->pinv[i] = invm(Pm, qi);
+    c->pinv_sh[i] = shoup_of(c->pinv[i], qi);
   }
-  // ---- conversion descriptors
+  c->qlinv.assign((size_t)(L + 1) * L, 0);
+  c->qlinv_sh = c->qlql = c->qlql_sh = c->qlinv;
+  for (u32 l = 2; l <= L; ++l) {
+    const u64 qt = pr[l - 1];
+    for (u32 i = 0; i + 1 < l; ++i) {
+      const u64 qi = pr[i];
+      const u64 qtinv = invm(qt % qi, qi);
+      // QlQlInvModqlDivqlModq_i = floor(Q' [Q'^-1]_qt / qt) mod q_i with
+      // Q' = prod_{i<l-1} q_i: Q' [Q'^-1]_qt is 1 mod qt and 0 mod q_i, so the
+      // floor is (Q' [Q'^-1]_qt - 1) / qt = -qt^-1 mod q_i.
+      const u64 big = (qi - qtinv) % qi;
+      const size_t o = (size_t)l * L + i;
+      c->qlinv[o] = qtinv;
+      c->qlinv_sh[o] = shoup_of(qtinv, qi);
+      c->qlql[o] = big;
+      c->qlql_sh[o] = shoup_of(big, qi);
+    }
+  }
+  // ---- conversion descriptors per level (ModUp per digit, ModDown)
   c->hmodup.resize((size_t)(L + 1) * c->dnum);
   c->hmoddown.resize(L + 1);
-  std::vector<LimbTask> tasks;
-  c->ext_task_off.assign(L + 1, 0);
-  c->ext_task_cnt.assign(L + 1, 0);
-  {
-    // scratch first (task lists point into EXT)
-    ALLOC(c->Dc, (size_t)L * N * 8);
-    ALLOC(c->EXT, (size_t)c->dnum * (L + K) * N * 8);
-    ALLOC(c->T0, (size_t)(L + K) * N * 8);
-    ALLOC(c->T1, (size_t)(L + K) * N * 8);
-    ALLOC(c->Y, (size_t)2 * L * N * 8);
-    ALLOC(c->D, (size_t)L * N * 8);
-    ALLOC(c->TMP, (size_t)2 * L * N * 8);
-    ALLOC(c->RS, (size_t)2 * N * 8);
-  }
   for (u32 l = 1; l <= L; ++l) {
     const u32 beta = num_digits(c, l);
-    c->ext_task_off[l] = (u32)tasks.size();
     for (u32 j = 0; j < beta; ++j) {
       u32 st, en;
       digit_range(c, l, j, st, en);
       std::vector<u32> src, dst, off;
       for (u32 i = st; i < en; ++i) src.push_back(i);
       for (u32 i = 0; i < l; ++i)
-        if (i < st || i >= en) { dst.push_back(i); off.push_back(i); }
-      for (u32 k = 0; k < K; ++k) { dst.push_back(L + k); off.push_back(l + k); }
+        if (i < st || i >= en) {
+          dst.push_back(i);
+          off.push_back(i);
+        }
+      for (u32 k = 0; k < K; ++k) {
+        dst.push_back(L + k);
+        off.push_back(l + k);
+      }
       make_conv(c, src, dst, off, c->hmodup[(size_t)l * c->dnum + j]);
-      u64* ext = c->EXT + (size_t)j * (L + K) * N;
-      for (size_t t = 0; t < dst.size(); ++t) tasks.push_back(LimbTask{ext + (size_t)off[t] * N, dst[t], 0});
     }
-    c->ext_task_cnt[l] = (u32)tasks.size() - c->ext_task_off[l];
     std::vector<u32> src, dst, off;
     for (u32 k = 0; k < K; ++k) src.push_back(L + k);
-    for (u32 i = 0; i < l; ++i) { dst.push_back(i); off.push_back(i); }
+    for (u32 i = 0; i < l; ++i) {
+      dst.push_back(i);
+      off.push_back(i);
+    }
     make_conv(c, src, dst, off, c->hmoddown[l]);
   }
   ALLOC(c->modup, sizeof(ConvDesc) * c->hmodup.size());
   H2D(c->modup, c->hmodup.data(), sizeof(ConvDesc) * c->hmodup.size());
   ALLOC(c->moddown, sizeof(ConvDesc) * c->hmoddown.size());
   H2D(c->moddown, c->hmoddown.data(), sizeof(ConvDesc) * c->hmoddown.size());
-  ALLOC(c->ext_tasks, sizeof(LimbTask) * tasks.size());
-  H2D(c->ext_tasks, tasks.data(), sizeof(LimbTask) * tasks.size());
-  c->tmp_tasks_cap = 4096;
-  ALLOC(c->tmp_tasks, sizeof(LimbTask) * c->tmp_tasks_cap);
+  c->raw_tasks_cap = 4096;
+  ALLOC(c->raw_tasks, sizeof(NttTask) * c->raw_tasks_cap);
+  for (auto& s : c->ring) {
+    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+      fides_context_destroy(c);
+      return fail(FIDES_ERR_HIP, "event creation");
+    }
+  }
 #undef ALLOC
 #undef H2D
   *out = c;
   return FIDES_OK;
-}
-
-extern "C" void fides_context_destroy(fides_context* c) {
-  if (!c) return;
-  for (void* p : c->allocs) (void)hipFree(p);
-  delete c;
 }
 
 extern "C" int fides_context_info(const fides_context* c, uint32_t* logN, uint32_t* L, uint32_t* K, uint32_t* dnum,
@@ -474,8 +499,310 @@ extern "C" int fides_copy_to_host(void* dst, const void* src, size_t bytes, void
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   return FIDES_OK;
 }
+extern "C" int fides_copy_device(void* dst, const void* src, size_t bytes, void* stream) {
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)stream));
+  return FIDES_OK;
+}
 extern "C" int fides_stream_sync(void* stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return FIDES_OK;
+}
+
+// ======================================================================
+// executor: task blob + arena
+// ======================================================================
+// Host-side builder of all task lists of one operation.  Lists are appended
+// to one byte blob; `off` values are byte offsets resolved against the
+// device copy after upload.
+struct Blob {
+  std::vector<char> b;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    size_t o = (b.size() + 255) & ~(size_t)255;
+    b.resize(o + v.size() * sizeof(T));
+    if (!v.empty()) memcpy(b.data() + o, v.data(), v.size() * sizeof(T));
+    return o;
+  }
+};
+
+// Upload a blob through the next ring slot; returns its device base.
+static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char** dev, int* slot_out) {
+  TaskSlot& s = c->ring[c->ring_next];
+  *slot_out = c->ring_next;
+  c->ring_next = (c->ring_next + 1) % RING;
+  if (s.used) HIPCHK(hipEventSynchronize(s.done));  // slot's previous operation has finished
+  const size_t need = std::max<size_t>(blob.b.size(), 1);
+  if (need > s.cap) {
+    if (s.host) HIPCHK(hipHostFree(s.host));
+    if (s.dev) HIPCHK(hipFree(s.dev));
+    s.host = nullptr;
+    s.dev = nullptr;
+    const size_t cap = std::max<size_t>(need * 2, 1 << 20);
+    HIPCHK(hipHostMalloc((void**)&s.host, cap, hipHostMallocDefault));
+    HIPCHK(hipMalloc((void**)&s.dev, cap));
+    s.cap = cap;
+  }
+  memcpy(s.host, blob.b.data(), blob.b.size());
+  HIPCHK(hipMemcpyAsync(s.dev, s.host, blob.b.size(), hipMemcpyHostToDevice, st));
+  *dev = s.dev;
+  return FIDES_OK;
+}
+
+static int finish_slot(fides_context* c, int slot, hipStream_t st) {
+  TaskSlot& s = c->ring[slot];
+  HIPCHK(hipEventRecord(s.done, st));
+  s.used = true;
+  return FIDES_OK;
+}
+
+static int ensure_arena(fides_context* c, size_t words, hipStream_t st) {
+  if (words <= c->arena_words) return FIDES_OK;
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipDeviceSynchronize());
+  if (c->arena) HIPCHK(hipFree(c->arena));
+  c->arena = nullptr;
+  c->arena_words = 0;
+  const size_t w = words + words / 4;
+  hipError_t e = hipMalloc((void**)&c->arena, w * 8);
+  if (e != hipSuccess) return fail(FIDES_ERR_NOMEM, "scratch arena allocation");
+  c->arena_words = w;
+  return FIDES_OK;
+}
+
+template <class T>
+static const T* at(char* dev, size_t off) {
+  return (const T*)(dev + off);
+}
+
+static int launch_ntt_list(fides_context* c, const NttTask* list, u32 n, int pro, int epi, hipStream_t st) {
+  for (u32 o = 0; o < n; o += 65535) {
+    LimbSpan s = span0();
+    s.list = list + o;
+    s.total = std::min<u32>(65535, n - o);
+    if (fides_launch_ntt(c->logN, s, c->T, pro, epi, fuse0(), st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
+  }
+  return FIDES_OK;
+}
+static int launch_intt_list(fides_context* c, const NttTask* list, u32 n, hipStream_t st) {
+  for (u32 o = 0; o < n; o += 65535) {
+    LimbSpan s = span0();
+    s.list = list + o;
+    s.total = std::min<u32>(65535, n - o);
+    if (fides_launch_intt(c->logN, s, c->T, st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
+  }
+  return FIDES_OK;
+}
+static int launch_conv_list(fides_context* c, const ConvTask* list, u32 n, u32 ns, hipStream_t st) {
+  for (u32 o = 0; o < n; o += 65535)
+    if (fides_launch_conv_batch(list + o, std::min<u32>(65535, n - o), ns, c->prm, c->logN, st))
+      return fail(FIDES_ERR_UNSUPPORTED, "conversion size");
+  return FIDES_OK;
+}
+
+static NttTask ntask(u64* data, u32 pid) {
+  NttTask t;
+  memset(&t, 0, sizeof(t));
+  t.data = data;
+  t.pid = pid;
+  return t;
+}
+
+// ======================================================================
+// key-switch jobs
+// ======================================================================
+struct KsJob {
+  const u64* d;  // l limbs, evaluation form
+  u32 l;
+  const fides_key* key;
+  u64* out0;
+  u64* out1;
+  const u64* acc0;  // nullable
+  const u64* acc1;  // nullable
+};
+
+// Arena words needed by the key switch of one job.
+static size_t ks_words(const fides_context* c, u32 l) {
+  const size_t N = c->N;
+  const u32 beta = num_digits(c, l);
+  return N * ((size_t)l + (size_t)beta * (l + c->K) + 2 * (size_t)(l + c->K) + 2 * (size_t)l);
+}
+
+// Appends the task lists of the batched key switch (reference pipeline
+// modup_ksk_moddown_mgpu, src/CKKS/LimbPartitionMGPU.cu:588-1873):
+//   1 INTT of the input (out of place into Dc)
+//   2 ModUp conversion of every digit (grouped by digit size)
+//   3 NTT of every conversion output
+//   4 inner product with the key
+//   5 INTT of the special limbs of both products
+//   6 ModDown conversion P -> Q_l of both products
+//   7 NTT fused with (x - conv) * P^-1 [+ acc] into the outputs
+struct KsLists {
+  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md;
+  u32 n_intt_in = 0, n_conv_up[CONV_MAXS + 1] = {}, n_ntt_ext = 0, n_dot = 0, n_intt_sp = 0, n_conv_down = 0,
+      n_ntt_md = 0;
+  bool any_acc = false;
+};
+
+static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scratch, Blob& blob, KsLists& K_) {
+  const u32 N = c->N, K = c->K, L = c->L;
+  std::vector<NttTask> intt_in, ntt_ext, intt_sp, ntt_md;
+  std::vector<ConvTask> conv_up[CONV_MAXS + 1], conv_down;
+  std::vector<DotTask> dot;
+  u64* w = scratch;
+  for (const KsJob& J : jobs) {
+    const u32 l = J.l, beta = num_digits(c, l), ext_stride = l + K;
+    u64* Dc = w;
+    u64* EXT = Dc + (size_t)l * N;
+    u64* T0 = EXT + (size_t)beta * ext_stride * N;
+    u64* T1 = T0 + (size_t)(l + K) * N;
+    u64* Y = T1 + (size_t)(l + K) * N;
+    w = Y + (size_t)2 * l * N;
+    for (u32 i = 0; i < l; ++i) {
+      NttTask t = ntask(c->limb(Dc, i), i);
+      t.src = c->limb(J.d, i);
+      intt_in.push_back(t);
+    }
+    for (u32 j = 0; j < beta; ++j) {
+      u32 st, en;
+      digit_range(c, l, j, st, en);
+      const u32 ns = en - st;
+      const ConvDesc& hd = c->hmodup[(size_t)l * c->dnum + j];
+      u64* slab = EXT + (size_t)j * ext_stride * N;
+      for (u32 t0 = 0; t0 < hd.nt; t0 += CONV_TCHUNK)
+        conv_up[ns].push_back(ConvTask{c->limb(Dc, st), slab, c->modup + (size_t)l * c->dnum + j, t0,
+                                       std::min(hd.nt, t0 + CONV_TCHUNK)});
+      for (u32 t = 0; t < hd.nt; ++t) ntt_ext.push_back(ntask(c->limb(slab, hd.dst_off[t]), hd.dst_pid[t]));
+    }
+    const u64* ka = J.key->d;
+    const u64* kb = J.key->d + (size_t)c->dnum * (L + K) * N;
+    for (u32 t = 0; t < l + K; ++t) {
+      const u32 pid = t < l ? t : L + (t - l);
+      DotTask d;
+      d.d = t < l ? c->limb(J.d, t) : nullptr;
+      d.e = c->limb(EXT, t);
+      d.ka = c->limb(ka, pid);
+      d.kb = c->limb(kb, pid);
+      d.o0 = c->limb(T0, t);
+      d.o1 = c->limb(T1, t);
+      d.beta = beta;
+      d.jown = t < l ? t / c->alpha : 0xffffffffu;
+      d.pid = pid;
+      d.estride = ext_stride;
+      dot.push_back(d);
+    }
+    u64* Tp[2] = {T0, T1};
+    u64* outp[2] = {J.out0, J.out1};
+    const u64* accp[2] = {J.acc0, J.acc1};
+    const ConvDesc& md = c->hmoddown[l];
+    for (u32 p = 0; p < 2; ++p) {
+      for (u32 k = 0; k < K; ++k) intt_sp.push_back(ntask(c->limb(Tp[p], l + k), L + k));
+      u64* Yp = Y + (size_t)p * l * N;
+      for (u32 t0 = 0; t0 < md.nt; t0 += CONV_TCHUNK)
+        conv_down.push_back(ConvTask{c->limb(Tp[p], l), Yp, c->moddown + l, t0, std::min(md.nt, t0 + CONV_TCHUNK)});
+      for (u32 i = 0; i < l; ++i) {
+        NttTask t = ntask(c->limb(Yp, i), i);
+        t.x = c->limb(Tp[p], i);
+        t.acc = accp[p] ? c->limb(accp[p], i) : nullptr;
+        t.out = c->limb(outp[p], i);
+        t.c0 = c->pinv[i];
+        t.c0sh = c->pinv_sh[i];
+        ntt_md.push_back(t);
+        if (accp[p]) K_.any_acc = true;
+      }
+    }
+  }
+  // Group limbs of the same prime so a prime's twiddle tables stay in L2
+  // while its limbs are transformed (order does not affect results).
+  auto by_prime = [](std::vector<NttTask>& v) {
+    std::stable_sort(v.begin(), v.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
+  };
+  by_prime(intt_in);
+  by_prime(ntt_ext);
+  by_prime(intt_sp);
+  by_prime(ntt_md);
+  K_.intt_in = blob.add(intt_in);
+  K_.n_intt_in = (u32)intt_in.size();
+  for (u32 ns = 1; ns <= CONV_MAXS; ++ns) {
+    K_.conv_up[ns] = blob.add(conv_up[ns]);
+    K_.n_conv_up[ns] = (u32)conv_up[ns].size();
+  }
+  K_.ntt_ext = blob.add(ntt_ext);
+  K_.n_ntt_ext = (u32)ntt_ext.size();
+  K_.dot = blob.add(dot);
+  K_.n_dot = (u32)dot.size();
+  K_.intt_sp = blob.add(intt_sp);
+  K_.n_intt_sp = (u32)intt_sp.size();
+  K_.conv_down = blob.add(conv_down);
+  K_.n_conv_down = (u32)conv_down.size();
+  K_.ntt_md = blob.add(ntt_md);
+  K_.n_ntt_md = (u32)ntt_md.size();
+}
+
+static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t st) {
+  RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_in), K_.n_intt_in, st));
+  for (u32 ns = 1; ns <= CONV_MAXS; ++ns)
+    if (K_.n_conv_up[ns]) RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_up[ns]), K_.n_conv_up[ns], ns, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_ext), K_.n_ntt_ext, PRO_NONE, EPI_NONE, st));
+  for (u32 o = 0; o < K_.n_dot; o += 65535)
+    fides_launch_dot_batch(at<DotTask>(dev, K_.dot) + o, std::min<u32>(65535, K_.n_dot - o), c->prm, c->logN,
+                           c->L + c->K, st);
+  RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_sp), K_.n_intt_sp, st));
+  RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_down), K_.n_conv_down, c->K, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_md), K_.n_ntt_md, PRO_NONE,
+                       K_.any_acc ? EPI_MODDOWN_ADD : EPI_MODDOWN, st));
+  return FIDES_OK;
+}
+
+// Rescale jobs: in [2][l] -> out [2][l-1] (OpenFHE DropLastElementAndScale;
+// reference doubleRescaleMGPU, src/CKKS/LimbPartitionMGPU.cu:196-394):
+//   INTT of both top limbs (out of place), then per remaining limb
+//   NTT(SwitchModulus(top)) fused with x * q_top^-1 + . * QlQlInvModqlDivqlModq.
+struct RsJob {
+  const u64* in;
+  u32 l;
+  u64* out;
+};
+struct RsLists {
+  size_t intt, ntt;
+  u32 n_intt = 0, n_ntt = 0;
+};
+static void rs_build(fides_context* c, const std::vector<RsJob>& jobs, u64* scratch, Blob& blob, RsLists& R) {
+  const u32 N = c->N, L = c->L;
+  std::vector<NttTask> intt, ntt;
+  u64* w = scratch;
+  for (const RsJob& J : jobs) {
+    const u32 l = J.l;
+    u64* RS = w;
+    w += 2 * (size_t)N;
+    for (u32 p = 0; p < 2; ++p) {
+      const u64* inp = J.in + (size_t)p * l * N;
+      u64* outp = J.out + (size_t)p * (l - 1) * N;
+      NttTask t = ntask(RS + (size_t)p * N, l - 1);
+      t.src = c->limb(inp, l - 1);
+      intt.push_back(t);
+      for (u32 i = 0; i + 1 < l; ++i) {
+        NttTask u = ntask(c->limb(outp, i), i);
+        u.src = RS + (size_t)p * N;
+        u.src_pid = l - 1;
+        u.x = c->limb(inp, i);
+        const size_t o = (size_t)l * L + i;
+        u.c0 = c->qlinv[o];
+        u.c0sh = c->qlinv_sh[o];
+        u.c1 = c->qlql[o];
+        u.c1sh = c->qlql_sh[o];
+        ntt.push_back(u);
+      }
+    }
+  }
+  std::stable_sort(ntt.begin(), ntt.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
+  R.intt = blob.add(intt);
+  R.n_intt = (u32)intt.size();
+  R.ntt = blob.add(ntt);
+  R.n_ntt = (u32)ntt.size();
+}
+static int rs_launch(fides_context* c, char* dev, const RsLists& R, hipStream_t st) {
+  RCHK(launch_intt_list(c, at<NttTask>(dev, R.intt), R.n_intt, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, R.ntt), R.n_ntt, PRO_SWITCH_MODULUS, EPI_RESCALE, st));
   return FIDES_OK;
 }
 
@@ -488,70 +815,45 @@ static int build_span(fides_context* c, u64* data, const uint32_t* pids, uint32_
   const u32 NP = c->L + c->K;
   for (u32 t = 0; t < n; ++t)
     if (pids[t] >= NP) return fail(FIDES_ERR_ARG, "prime id out of range");
-  // group consecutive prime ids into segments
-  std::vector<std::pair<u32, u32>> segs;  // (start limb, count)
+  std::vector<std::pair<u32, u32>> segs;  // (first limb, count) of runs of consecutive prime ids
   for (u32 t = 0; t < n; ++t) {
-    if (!segs.empty() && pids[t] == pids[segs.back().first] + segs.back().second) {
+    if (!segs.empty() && pids[t] == pids[segs.back().first] + segs.back().second)
       segs.back().second++;
-    } else {
+    else
       segs.push_back({t, 1});
-    }
   }
   if (segs.size() <= 8) {
     for (auto& sg : segs) span_add(s, c->limb(data, sg.first), pids[sg.first], sg.second);
     return FIDES_OK;
   }
-  if (n > c->tmp_tasks_cap) return fail(FIDES_ERR_ARG, "too many limbs in one call");
-  std::vector<LimbTask> tasks(n);
-  for (u32 t = 0; t < n; ++t) tasks[t] = LimbTask{c->limb(data, t), pids[t], t};
-  HIPCHK(hipMemcpyAsync(c->tmp_tasks, tasks.data(), n * sizeof(LimbTask), hipMemcpyHostToDevice, st));
-  s.list = c->tmp_tasks;
+  if (n > c->raw_tasks_cap) return fail(FIDES_ERR_ARG, "too many limbs in one call");
+  std::vector<NttTask> tasks(n);
+  for (u32 t = 0; t < n; ++t) tasks[t] = ntask(c->limb(data, t), pids[t]);
+  HIPCHK(hipMemcpyAsync(c->raw_tasks, tasks.data(), n * sizeof(NttTask), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // the host vector goes out of scope
+  s.list = c->raw_tasks;
   s.total = n;
   return FIDES_OK;
 }
 
 extern "C" int fides_ntt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
-  if (!c || (!d && n) || (!pids && n)) return fail(FIDES_ERR_ARG, "null argument");
+  if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   LimbSpan s;
-  int rc = build_span(c, d, pids, n, (hipStream_t)stream, s);
-  if (rc) return rc;
-  NttFuse F;
-  memset(&F, 0, sizeof(F));
-  if (fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, F, (hipStream_t)stream))
+  RCHK(build_span(c, d, pids, n, (hipStream_t)stream, s));
+  if (fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, fuse0(), (hipStream_t)stream))
     return fail(FIDES_ERR_UNSUPPORTED, "ring dimension");
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
 extern "C" int fides_intt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
-  if (!c || (!d && n) || (!pids && n)) return fail(FIDES_ERR_ARG, "null argument");
+  if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   LimbSpan s;
-  int rc = build_span(c, d, pids, n, (hipStream_t)stream, s);
-  if (rc) return rc;
+  RCHK(build_span(c, d, pids, n, (hipStream_t)stream, s));
   if (fides_launch_intt(c->logN, s, c->T, (hipStream_t)stream)) return fail(FIDES_ERR_UNSUPPORTED, "ring dimension");
   HIPCHK(hipGetLastError());
-  return FIDES_OK;
-}
-
-// ModUp of digit j into ext (l+K limbs, Q_l u P order), target limbs only
-// (the digit's own limbs are not written).  Input d: l limbs, evaluation.
-static int modup_digit(fides_context* c, const u64* d, u32 l, u32 j, u64* ext, bool intt_done, hipStream_t st) {
-  const u32 N = c->N;
-  u32 stt, en;
-  digit_range(c, l, j, stt, en);
-  const u32 sz = en - stt;
-  if (!intt_done) {
-    HIPCHK(hipMemcpyAsync(c->limb(c->Dc, stt), c->limb(d, stt), (size_t)sz * N * 8, hipMemcpyDeviceToDevice, st));
-    LimbSpan s = span0();
-    span_add(s, c->limb(c->Dc, stt), stt, sz);
-    fides_launch_intt(c->logN, s, c->T, st);
-  }
-  const ConvDesc& hd = c->hmodup[(size_t)l * c->dnum + j];
-  if (fides_launch_conv(c->limb(c->Dc, stt), ext, c->modup + (size_t)l * c->dnum + j, hd.ns, hd.nt, c->prm,
-                        c->logN, st))
-    return fail(FIDES_ERR_UNSUPPORTED, "digit size");
   return FIDES_OK;
 }
 
@@ -560,54 +862,33 @@ extern "C" int fides_modup(fides_context* c, const uint64_t* d, uint32_t l, uint
   if (!c || !d || !out) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L || j >= num_digits(c, l)) return fail(FIDES_ERR_ARG, "level / digit out of range");
   hipStream_t st = (hipStream_t)stream;
-  int rc = modup_digit(c, d, l, j, out, false, st);
-  if (rc) return rc;
+  const u32 N = c->N;
   u32 stt, en;
   digit_range(c, l, j, stt, en);
-  LimbSpan s = span0();
-  span_add(s, out, 0, stt);
-  span_add(s, c->limb(out, en), en, l - en);
-  span_add(s, c->limb(out, l), c->L, c->K);
-  NttFuse F;
-  memset(&F, 0, sizeof(F));
-  fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, F, st);
-  HIPCHK(hipMemcpyAsync(c->limb(out, stt), c->limb(d, stt), (size_t)(en - stt) * c->N * 8, hipMemcpyDeviceToDevice,
-                        st));
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
-}
-
-// ModDown of npoly polynomials src[p] (each l+K limbs, evaluation form) into
-// dst[p] (l limbs); with acc[p] != null the result is added to acc[p]
-// (dst[p] may alias acc[p]).  src special limbs are clobbered.
-static int moddown_multi(fides_context* c, u64* const* src, u32 npoly, u32 l, u64* const* dst, const u64* const* acc,
-                         hipStream_t st) {
-  const u32 N = c->N, K = c->K;
-  LimbSpan sp = span0();
-  for (u32 p = 0; p < npoly; ++p) span_add(sp, c->limb(src[p], l), c->L, K);
-  fides_launch_intt(c->logN, sp, c->T, st);
-  const ConvDesc& hd = c->hmoddown[l];
-  for (u32 p = 0; p < npoly; ++p)
-    fides_launch_conv(c->limb(src[p], l), c->Y + (size_t)p * c->L * N, c->moddown + l, hd.ns, hd.nt, c->prm, c->logN,
-                      st);
-  LimbSpan ys = span0();
-  NttFuse F;
-  memset(&F, 0, sizeof(F));
-  F.x = span0();
-  F.out = span0();
-  F.acc = span0();
-  bool any_acc = false;
-  for (u32 p = 0; p < npoly; ++p) {
-    span_add(ys, c->Y + (size_t)p * c->L * N, 0, l);
-    span_add(F.x, src[p], 0, l);
-    span_add(F.out, dst[p], 0, l);
-    span_add(F.acc, (u64*)(acc ? acc[p] : nullptr), 0, l);
-    if (acc && acc[p]) any_acc = true;
+  const u32 sz = en - stt;
+  RCHK(ensure_arena(c, (size_t)sz * N, st));
+  Blob blob;
+  std::vector<NttTask> intt, ntt;
+  for (u32 i = 0; i < sz; ++i) {
+    NttTask t = ntask(c->limb(c->arena, i), stt + i);
+    t.src = c->limb(d, stt + i);
+    intt.push_back(t);
   }
-  F.c0 = c->pinv;
-  F.c0sh = c->pinv + c->L;
-  fides_launch_ntt(c->logN, ys, c->T, PRO_NONE, any_acc ? EPI_MODDOWN_ADD : EPI_MODDOWN, F, st);
-  (void)N;
+  const ConvDesc& hd = c->hmodup[(size_t)l * c->dnum + j];
+  std::vector<ConvTask> conv;
+  for (u32 t0 = 0; t0 < hd.nt; t0 += CONV_TCHUNK)
+    conv.push_back(ConvTask{c->arena, out, c->modup + (size_t)l * c->dnum + j, t0, std::min(hd.nt, t0 + CONV_TCHUNK)});
+  for (u32 t = 0; t < hd.nt; ++t) ntt.push_back(ntask(c->limb(out, hd.dst_off[t]), hd.dst_pid[t]));
+  const size_t o_intt = blob.add(intt), o_conv = blob.add(conv), o_ntt = blob.add(ntt);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(launch_intt_list(c, at<NttTask>(dev, o_intt), sz, st));
+  RCHK(launch_conv_list(c, at<ConvTask>(dev, o_conv), (u32)conv.size(), sz, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, o_ntt), (u32)ntt.size(), PRO_NONE, EPI_NONE, st));
+  HIPCHK(hipMemcpyAsync(c->limb(out, stt), c->limb(d, stt), (size_t)sz * N * 8, hipMemcpyDeviceToDevice, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
@@ -615,18 +896,44 @@ extern "C" int fides_moddown(fides_context* c, const uint64_t* d, uint32_t l, ui
   if (!c || !d || !out) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   hipStream_t st = (hipStream_t)stream;
-  // work on a copy so the caller's input is preserved
-  HIPCHK(hipMemcpyAsync(c->T0, d, (size_t)(l + c->K) * c->N * 8, hipMemcpyDeviceToDevice, st));
-  u64* src[1] = {c->T0};
-  u64* dst[1] = {out};
-  int rc = moddown_multi(c, src, 1, l, dst, nullptr, st);
-  if (rc) return rc;
+  const u32 N = c->N, K = c->K, L = c->L;
+  // scratch: copy of the special limbs (INTT in place) + conversion output
+  RCHK(ensure_arena(c, (size_t)(K + l) * N, st));
+  u64* sp = c->arena;
+  u64* Y = c->arena + (size_t)K * N;
+  Blob blob;
+  std::vector<NttTask> intt, ntt;
+  for (u32 k = 0; k < K; ++k) {
+    NttTask t = ntask(c->limb(sp, k), L + k);
+    t.src = c->limb(d, l + k);
+    intt.push_back(t);
+  }
+  const ConvDesc& md = c->hmoddown[l];
+  std::vector<ConvTask> conv;
+  for (u32 t0 = 0; t0 < md.nt; t0 += CONV_TCHUNK)
+    conv.push_back(ConvTask{sp, Y, c->moddown + l, t0, std::min(md.nt, t0 + CONV_TCHUNK)});
+  for (u32 i = 0; i < l; ++i) {
+    NttTask t = ntask(c->limb(Y, i), i);
+    t.x = c->limb(d, i);
+    t.out = c->limb(out, i);
+    t.c0 = c->pinv[i];
+    t.c0sh = c->pinv_sh[i];
+    ntt.push_back(t);
+  }
+  const size_t o_intt = blob.add(intt), o_conv = blob.add(conv), o_ntt = blob.add(ntt);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(launch_intt_list(c, at<NttTask>(dev, o_intt), K, st));
+  RCHK(launch_conv_list(c, at<ConvTask>(dev, o_conv), (u32)conv.size(), K, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, o_ntt), l, PRO_NONE, EPI_MODDOWN, st));
+  RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
 // ======================================================================
-// keys and key switching
+// keys
 // ======================================================================
 extern "C" int fides_key_upload(fides_context* c, const uint64_t* h, fides_key** out) {
   if (!c || !h || !out) return fail(FIDES_ERR_ARG, "null argument");
@@ -659,108 +966,124 @@ extern "C" void fides_key_destroy(fides_key* k) {
   delete k;
 }
 
-// Key switch of d (l limbs, evaluation) -> out0/out1 (l limbs), optionally
-// accumulating acc0/acc1 (null = none).  Pipeline of the reference
-// modup_ksk_moddown_mgpu (src/CKKS/LimbPartitionMGPU.cu:588-1873):
-// INTT -> per-digit ModUp conversion -> NTT of all conversion outputs ->
-// inner product with the key -> ModDown of both components (INTT of the
-// special limbs, conversion, NTT fused with (x - conv) P^-1 [+ acc]).
-static int keyswitch_core(fides_context* c, const u64* d, u32 l, const fides_key* key, u64* out0, u64* out1,
-                          const u64* acc0, const u64* acc1, hipStream_t st) {
-  const u32 N = c->N, K = c->K, L = c->L;
-  const u32 beta = num_digits(c, l);
-  // 1. coefficient-form copy of the input
-  HIPCHK(hipMemcpyAsync(c->Dc, d, (size_t)l * N * 8, hipMemcpyDeviceToDevice, st));
-  {
-    LimbSpan s = span0();
-    span_add(s, c->Dc, 0, l);
-    fides_launch_intt(c->logN, s, c->T, st);
-  }
-  // 2. ModUp conversions of every digit
-  for (u32 j = 0; j < beta; ++j) {
-    int rc = modup_digit(c, d, l, j, c->EXT + (size_t)j * (L + K) * N, true, st);
-    if (rc) return rc;
-  }
-  // 3. NTT of all conversion outputs (one launch, precomputed task list)
-  {
-    LimbSpan s = span0();
-    s.list = c->ext_tasks + c->ext_task_off[l];
-    s.total = c->ext_task_cnt[l];
-    NttFuse F;
-    memset(&F, 0, sizeof(F));
-    fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, F, st);
-  }
-  // 4. inner product with the key
-  {
-    KsDotArgs A;
-    A.d = d;
-    A.ext = c->EXT;
-    A.key_a = key->d;
-    A.key_b = key->d + (size_t)c->dnum * (L + K) * N;
-    A.o0 = c->T0;
-    A.o1 = c->T1;
-    A.prm = c->prm;
-    A.logN = c->logN;
-    A.l = l;
-    A.K = K;
-    A.L = L;
-    A.beta = beta;
-    A.alpha = c->alpha;
-    A.ext_stride = L + K;
-    A.key_stride = L + K;
-    fides_launch_ksk_dot(A, st);
-  }
-  // 5. ModDown of both components
-  u64* src[2] = {c->T0, c->T1};
-  u64* dst[2] = {out0, out1};
-  const u64* acc[2] = {acc0, acc1};
-  return moddown_multi(c, src, 2, l, dst, (acc0 || acc1) ? acc : nullptr, st);
-}
-
-static int check_ks(fides_context* c, const fides_key* key, u32 l) {
-  if (!c || !key) return fail(FIDES_ERR_ARG, "null argument");
+// ======================================================================
+// ciphertext operations (single = batch of one)
+// ======================================================================
+static int check_key(fides_context* c, const fides_key* key, u32 l) {
+  if (!key) return fail(FIDES_ERR_ARG, "null key");
   if (key->ctx != c) return fail(FIDES_ERR_ARG, "key belongs to another context");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   return FIDES_OK;
 }
 
+static int run_keyswitch(fides_context* c, const std::vector<KsJob>& jobs, hipStream_t st) {
+  size_t words = 0;
+  for (const KsJob& J : jobs) words += ks_words(c, J.l);
+  RCHK(ensure_arena(c, words, st));
+  Blob blob;
+  KsLists K_;
+  ks_build(c, jobs, c->arena, blob, K_);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(ks_launch(c, dev, K_, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
 extern "C" int fides_keyswitch(fides_context* c, const uint64_t* d, uint32_t l, const fides_key* key, uint64_t* o0,
                                uint64_t* o1, void* stream) {
-  int rc = check_ks(c, key, l);
-  if (rc) return rc;
+  if (!c) return fail(FIDES_ERR_ARG, "null context");
+  RCHK(check_key(c, key, l));
   if (!d || !o0 || !o1) return fail(FIDES_ERR_ARG, "null buffer");
-  rc = keyswitch_core(c, d, l, key, o0, o1, nullptr, nullptr, (hipStream_t)stream);
-  if (rc) return rc;
+  return run_keyswitch(c, {KsJob{d, l, key, o0, o1, nullptr, nullptr}}, (hipStream_t)stream);
+}
+
+// HMult (+ relinearize [+ rescale]) of n ciphertext pairs.
+static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* const* b, const u32* lv,
+                       const fides_key* rlk, bool square, bool rescale, u64* const* out, hipStream_t st) {
+  const u32 N = c->N;
+  for (u32 i = 0; i < n; ++i) {
+    RCHK(check_key(c, rlk, lv[i]));
+    if (!a[i] || (!square && !b[i]) || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (rescale && lv[i] < 2) return fail(FIDES_ERR_ARG, "rescale needs l >= 2");
+  }
+  // arena: [key-switch scratch | per job: c1d1 (l) | product [2][l] if rescaling | rescale top limbs (2)]
+  size_t ksw = 0, extra = 0;
+  for (u32 i = 0; i < n; ++i) {
+    ksw += ks_words(c, lv[i]);
+    extra += (size_t)N * (lv[i] + (rescale ? 2 * lv[i] + 2 : 0));
+  }
+  RCHK(ensure_arena(c, ksw + extra, st));
+  u64* w = c->arena + ksw;
+  std::vector<TensorTask> tens;
+  std::vector<KsJob> jobs;
+  std::vector<RsJob> rjobs;
+  for (u32 i = 0; i < n; ++i) {
+    const u32 l = lv[i];
+    u64* D = w;
+    w += (size_t)l * N;
+    u64* prod = out[i];
+    if (rescale) {
+      prod = w;
+      w += (size_t)2 * l * N;
+    }
+    const size_t S = (size_t)l * N;
+    for (u32 t = 0; t < l; ++t) {
+      TensorTask tt;
+      tt.a0 = c->limb(a[i], t);
+      tt.a1 = c->limb(a[i] + S, t);
+      tt.b0 = square ? nullptr : c->limb(b[i], t);
+      tt.b1 = square ? nullptr : c->limb(b[i] + S, t);
+      tt.o0 = c->limb(prod, t);
+      tt.o1 = c->limb(prod + S, t);
+      tt.o2 = c->limb(D, t);
+      tt.pid = t;
+      tt.square = square ? 1u : 0u;
+      tens.push_back(tt);
+    }
+    jobs.push_back(KsJob{D, l, rlk, prod, prod + S, prod, prod + S});
+    if (rescale) rjobs.push_back(RsJob{prod, l, out[i]});
+  }
+  Blob blob;
+  const size_t o_tens = blob.add(tens);
+  KsLists K_;
+  ks_build(c, jobs, c->arena, blob, K_);
+  RsLists R;
+  if (rescale) rs_build(c, rjobs, w, blob, R);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  for (u32 o = 0; o < tens.size(); o += 65535)
+    fides_launch_tensor_batch(at<TensorTask>(dev, o_tens) + o, std::min<u32>(65535, (u32)tens.size() - o), c->prm,
+                              c->logN, st);
+  RCHK(ks_launch(c, dev, K_, st));
+  if (rescale) RCHK(rs_launch(c, dev, R, st));
+  RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
 extern "C" int fides_hmult(fides_context* c, const uint64_t* a, const uint64_t* b, uint32_t l, const fides_key* rlk,
                            uint64_t* out, void* stream) {
-  int rc = check_ks(c, rlk, l);
-  if (rc) return rc;
-  if (!a || !b || !out) return fail(FIDES_ERR_ARG, "null buffer");
-  hipStream_t st = (hipStream_t)stream;
-  const size_t S = (size_t)l * c->N;
-  fides_launch_tensor(a, a + S, b, b + S, out, out + S, c->D, c->prm, c->logN, l, false, st);
-  rc = keyswitch_core(c, c->D, l, rlk, out, out + S, out, out + S, st);
-  if (rc) return rc;
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
+  if (!c) return fail(FIDES_ERR_ARG, "null context");
+  return hmult_batch(c, 1, &a, &b, &l, rlk, false, false, &out, (hipStream_t)stream);
 }
 
 extern "C" int fides_square(fides_context* c, const uint64_t* a, uint32_t l, const fides_key* rlk, uint64_t* out,
                             void* stream) {
-  int rc = check_ks(c, rlk, l);
-  if (rc) return rc;
-  if (!a || !out) return fail(FIDES_ERR_ARG, "null buffer");
-  hipStream_t st = (hipStream_t)stream;
-  const size_t S = (size_t)l * c->N;
-  fides_launch_tensor(a, a + S, nullptr, nullptr, out, out + S, c->D, c->prm, c->logN, l, true, st);
-  rc = keyswitch_core(c, c->D, l, rlk, out, out + S, out, out + S, st);
-  if (rc) return rc;
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
+  if (!c) return fail(FIDES_ERR_ARG, "null context");
+  const uint64_t* b = nullptr;
+  return hmult_batch(c, 1, &a, &b, &l, rlk, true, false, &out, (hipStream_t)stream);
+}
+
+extern "C" int fides_hmult_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
+                                 const uint32_t* l, const fides_key* rlk, int rescale, uint64_t* const* out,
+                                 void* stream) {
+  if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (n == 0) return FIDES_OK;
+  return hmult_batch(c, n, a, b, l, rlk, false, rescale != 0, out, (hipStream_t)stream);
 }
 
 extern "C" int fides_rescale(fides_context* c, const uint64_t* in, uint32_t l, uint64_t* out, void* stream) {
@@ -768,36 +1091,15 @@ extern "C" int fides_rescale(fides_context* c, const uint64_t* in, uint32_t l, u
   if (l < 2 || l > c->L) return fail(FIDES_ERR_ARG, "rescale needs 2 <= l <= L");
   if (in == out) return fail(FIDES_ERR_ARG, "rescale output must not alias its input");
   hipStream_t st = (hipStream_t)stream;
-  const u32 N = c->N, L = c->L;
-  const size_t S = (size_t)l * N, So = (size_t)(l - 1) * N;
-  // top limbs of both components -> coefficient form
-  HIPCHK(hipMemcpyAsync(c->RS, in + (size_t)(l - 1) * N, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
-  HIPCHK(hipMemcpyAsync(c->RS + N, in + S + (size_t)(l - 1) * N, (size_t)N * 8, hipMemcpyDeviceToDevice, st));
-  {
-    LimbSpan s = span0();
-    span_add(s, c->RS, l - 1, 1);
-    span_add(s, c->RS + N, l - 1, 1);
-    fides_launch_intt(c->logN, s, c->T, st);
-  }
-  // per component: NTT(SwitchModulus(top)) fused with x * q^-1 + . * QlQlInv
-  const u64* row = c->resc + (size_t)l * 4 * L;
-  for (u32 p = 0; p < 2; ++p) {
-    LimbSpan s = span0();
-    span_add(s, out + p * So, 0, l - 1);
-    NttFuse F;
-    memset(&F, 0, sizeof(F));
-    F.pro_src = c->RS + (size_t)p * N;
-    F.pro_src_pid = l - 1;
-    F.x = span0();
-    span_add(F.x, (u64*)in + p * S, 0, l - 1);
-    F.out = span0();
-    F.acc = span0();
-    F.c0 = row;
-    F.c0sh = row + L;
-    F.c1 = row + 2 * L;
-    F.c1sh = row + 3 * L;
-    fides_launch_ntt(c->logN, s, c->T, PRO_SWITCH_MODULUS, EPI_RESCALE, F, st);
-  }
+  RCHK(ensure_arena(c, 2 * (size_t)c->N, st));
+  Blob blob;
+  RsLists R;
+  rs_build(c, {RsJob{in, l, out}}, c->arena, blob, R);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(rs_launch(c, dev, R, st));
+  RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
@@ -807,23 +1109,73 @@ extern "C" int fides_automorph(fides_context* c, const uint64_t* in, uint32_t n,
   if (!c || !in || !out) return fail(FIDES_ERR_ARG, "null argument");
   if ((k & 1) == 0 || k >= 2 * c->N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
   if (in == out) return fail(FIDES_ERR_ARG, "automorphism output must not alias its input");
-  fides_launch_automorph(in, out, c->logN, n, k, (hipStream_t)stream);
+  hipStream_t st = (hipStream_t)stream;
+  Blob blob;
+  std::vector<AutoTask> at_;
+  for (u32 t = 0; t < n; ++t) at_.push_back(AutoTask{c->limb(in, t), c->limb(out, t), k, 0});
+  const size_t o = blob.add(at_);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  for (u32 x = 0; x < n; x += 65535)
+    fides_launch_automorph_batch(at<AutoTask>(dev, o) + x, std::min<u32>(65535, n - x), c->logN, st);
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+// Rotations of n ciphertexts: key switch of c1 with ModDown fused with the
+// c0 add, then the permutation of both components (OpenFHE EvalAutomorphism
+// order).
+static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32* lv, const fides_key* const* keys,
+                        const u32* kidx, u64* const* out, hipStream_t st) {
+  const u32 N = c->N;
+  size_t ksw = 0, extra = 0;
+  for (u32 i = 0; i < n; ++i) {
+    RCHK(check_key(c, keys[i], lv[i]));
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (in[i] == out[i]) return fail(FIDES_ERR_ARG, "rotate output must not alias its input");
+    if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
+    ksw += ks_words(c, lv[i]);
+    extra += (size_t)2 * lv[i] * N;
+  }
+  RCHK(ensure_arena(c, ksw + extra, st));
+  u64* w = c->arena + ksw;
+  std::vector<KsJob> jobs;
+  std::vector<AutoTask> au;
+  for (u32 i = 0; i < n; ++i) {
+    const u32 l = lv[i];
+    const size_t S = (size_t)l * N;
+    u64* tmp = w;
+    w += 2 * S;
+    jobs.push_back(KsJob{in[i] + S, l, keys[i], tmp, tmp + S, in[i], nullptr});
+    for (u32 t = 0; t < 2 * l; ++t) au.push_back(AutoTask{c->limb(tmp, t), c->limb(out[i], t), kidx[i], 0});
+  }
+  Blob blob;
+  KsLists K_;
+  ks_build(c, jobs, c->arena, blob, K_);
+  const size_t o_au = blob.add(au);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(ks_launch(c, dev, K_, st));
+  for (u32 x = 0; x < au.size(); x += 65535)
+    fides_launch_automorph_batch(at<AutoTask>(dev, o_au) + x, std::min<u32>(65535, (u32)au.size() - x), c->logN, st);
+  RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
 extern "C" int fides_rotate(fides_context* c, const uint64_t* in, uint32_t l, const fides_key* key, uint32_t k,
                             uint64_t* out, void* stream) {
-  int rc = check_ks(c, key, l);
-  if (rc) return rc;
-  if (!in || !out) return fail(FIDES_ERR_ARG, "null buffer");
-  if ((k & 1) == 0 || k >= 2 * c->N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
-  if (in == out) return fail(FIDES_ERR_ARG, "rotate output must not alias its input");
-  hipStream_t st = (hipStream_t)stream;
-  const size_t S = (size_t)l * c->N;
-  rc = keyswitch_core(c, in + S, l, key, c->TMP, c->TMP + S, in, nullptr, st);
-  if (rc) return rc;
-  fides_launch_automorph(c->TMP, out, c->logN, 2 * l, k, st);
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
+  if (!c) return fail(FIDES_ERR_ARG, "null context");
+  return rotate_batch(c, 1, &in, &l, &key, &k, &out, (hipStream_t)stream);
+}
+
+extern "C" int fides_rotate_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
+                                  const fides_key* const* keys, const uint32_t* k, uint64_t* const* out,
+                                  void* stream) {
+  if (!c || (n && (!in || !l || !keys || !k || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (n == 0) return FIDES_OK;
+  return rotate_batch(c, n, in, l, keys, k, out, (hipStream_t)stream);
 }

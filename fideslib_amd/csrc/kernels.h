@@ -20,26 +20,52 @@ struct ConvDesc {
   u64 mat[CONV_MAXT][CONV_MAXS];  // (Qhat / q_i) mod p_t, times 2^64 mod p_t
 };
 
-struct KsDotArgs {
-  const u64* d;      // [l][N]  evaluation-form input (digit limbs)
-  const u64* ext;    // [beta][ext_stride][N]  ModUp outputs per digit, limb t of Q_l u P
-  const u64* key_a;  // [dnum][key_stride][N] Montgomery form
-  const u64* key_b;
-  u64* o0;           // [l+K][N]
-  u64* o1;
-  const PrimeC* prm;
-  u32 logN, l, K, L, beta, alpha, ext_stride, key_stride;
+// ---- task records of the batched kernels (one record per workgroup row)
+
+// One (job, digit or polynomial, target chunk) of a basis conversion.
+struct ConvTask {
+  const u64* in;         // ns contiguous source limbs (coefficient form)
+  u64* out;              // target t goes to out + desc->dst_off[t] * N
+  const ConvDesc* desc;
+  u32 t0, t1;            // target range of this workgroup row
 };
 
-void fides_launch_tensor(const u64* a0, const u64* a1, const u64* b0, const u64* b1, u64* o0, u64* o1, u64* o2,
-                         const PrimeC* prm, u32 logN, u32 l, bool square, hipStream_t st);
-int fides_launch_conv(const u64* in, u64* out, const ConvDesc* d, u32 ns, u32 nt, const PrimeC* prm, u32 logN,
-                      hipStream_t st);
-void fides_launch_ksk_dot(const KsDotArgs& A, hipStream_t st);
+// One output limb t of the key-switch inner product (for one job).
+struct DotTask {
+  const u64* d;     // original evaluation-form limb t (used by the digit owning t), or null
+  const u64* e;     // ModUp output of digit 0 at limb t; digit j at e + j * estride * N
+  const u64* ka;    // key a, digit 0, this limb's prime; digit j at + j * kstride * N
+  const u64* kb;    // key b, likewise
+  u64* o0;          // sum_j e_j * b_j
+  u64* o1;          // sum_j e_j * a_j
+  u32 beta;         // active digits
+  u32 jown;         // digit owning limb t (Q limbs), 0xffffffff for special limbs
+  u32 pid;
+  u32 estride;      // limbs between consecutive digits' ModUp outputs
+};
+
+// One limb of the tensor product (a0,a1) x (b0,b1).
+struct TensorTask {
+  const u64 *a0, *a1, *b0, *b1;
+  u64 *o0, *o1, *o2;
+  u32 pid;
+  u32 square;
+};
+
+// One limb of an evaluation-domain automorphism.
+struct AutoTask {
+  const u64* in;
+  u64* out;
+  u32 kidx;
+  u32 pad;
+};
+
+int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, const PrimeC* prm, u32 logN, hipStream_t st);
+void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, u32 kstride,
+                            hipStream_t st);
+void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
+void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);
-void fides_launch_automorph(const u64* in, u64* out, u32 logN, u32 nlimbs, u32 kidx, hipStream_t st);
-void fides_launch_add(const u64* a, const u64* b, u64* o, const PrimeC* prm, u32 logN, u32 nlimbs, u32 pid0,
-                      hipStream_t st);
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st);
 int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);

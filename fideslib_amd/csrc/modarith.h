@@ -50,23 +50,35 @@ __device__ __forceinline__ void mul64wide(u64 a, u64 b, u64& hi, u64& lo) {
   hi = p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
 }
 
+// High 64 bits of a * b: one mul_hi plus three v_mad_u64_u32 whose 64-bit
+// addends carry the partial sums (no intermediate can overflow 64 bits).
 __device__ __forceinline__ u64 mulhi64(u64 a, u64 b) {
   const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
-  const u64 p00 = (u64)a0 * b0;
-  const u64 p01 = (u64)a0 * b1;
-  const u64 p10 = (u64)a1 * b0;
-  const u64 p11 = (u64)a1 * b1;
-  const u64 mid = (p00 >> 32) + (u32)p01 + (u32)p10;
-  return p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+  const u64 t = (u64)a0 * b1 + (u64)__umulhi(a0, b0);  // < 2^64
+  const u64 u = (u64)a1 * b0 + (u32)t;                 // < 2^64
+  return (u64)a1 * b1 + (t >> 32) + (u >> 32);
 }
 
-// x * w mod q in [0, 2q); any x < 2^64, w < q, wp = floor(w * 2^64 / q).
+// Low 64 bits of a * b + c: one v_mad_u64_u32 for the low word and the two
+// cross products folded into the high word with 32-bit arithmetic.
+__device__ __forceinline__ u64 mullo64_add(u64 a, u64 b, u64 c) {
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+  const u64 p = (u64)a0 * b0 + c;
+  const u32 hi = (u32)(p >> 32) + a0 * b1 + a1 * b0;
+  return ((u64)hi << 32) | (u32)p;
+}
+
+// x * w mod q in [0, 2q); any x < 2^64, w < q, wp = floor(w * 2^64 / q),
+// qn = 2^64 - q.  r = x*w + h*(-q) mod 2^64 needs no final subtraction.
 __device__ __forceinline__ u64 shoup_lazy(u64 x, u64 w, u64 wp, u64 q) {
   const u64 h = mulhi64(x, wp);
-  return x * w - h * q;
+  return mullo64_add(h, (u64)0 - q, x * w);
 }
 
-__device__ __forceinline__ u64 csub(u64 x, u64 m) { return x >= m ? x - m : x; }
+__device__ __forceinline__ u64 csub(u64 x, u64 m) {
+  const u64 t = x + (0 - m);  // one 64-bit add, no carry chain
+  return x >= m ? t : x;
+}
 
 // x * w mod q in [0, q).
 __device__ __forceinline__ u64 shoup(u64 x, u64 w, u64 wp, u64 q) { return csub(shoup_lazy(x, w, wp, q), q); }

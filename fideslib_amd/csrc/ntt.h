@@ -25,32 +25,33 @@
 #pragma once
 #include "modarith.h"
 
-// One limb of work: pointer to N residues and its prime id.
-struct LimbTask {
-  u64* ptr;
-  u32 pid;
-  u32 aux;  // operation-specific (source limb index for fused prologues)
+// One limb of NTT work.  Device task lists carry everything a launch needs
+// per limb, so one launch can cover any mix of ciphertexts, levels and
+// primes (the cross-ciphertext batch).
+struct NttTask {
+  u64* data;       // limb transformed in place; final destination unless `out` is set
+  const u64* src;  // input of the first pass (null: data).  PRO_SWITCH_MODULUS:
+                   // coefficient-form source limb under prime src_pid
+  const u64* x;    // epilogue operand
+  const u64* acc;  // epilogue accumulator (null: none)
+  u64* out;        // epilogue destination (null: data)
+  u64 c0, c0sh, c1, c1sh;  // epilogue constants of this limb (with Shoup companions)
+  u32 pid;         // prime of the transform
+  u32 src_pid;
 };
 
-// A set of limbs: up to 8 segments of consecutive limbs (stride N) with
-// consecutive prime ids, passed by value, or a device task list.
+// A set of limbs given inline as up to 8 segments of consecutive limbs
+// (stride N) with consecutive prime ids, or a device task list.
 struct LimbSpan {
   u64* seg_ptr[8];
   u32 seg_pid[8];
   u32 seg_end[8];  // cumulative limb count at the end of each segment
   u32 nseg;
   u32 total;
-  const LimbTask* list;  // if non-null, used instead of the segments
+  const NttTask* list;  // if non-null, used instead of the segments
 };
 
-__device__ __forceinline__ void span_get(const LimbSpan& s, u32 y, u32 logN, u64*& ptr, u32& pid, u32& aux) {
-  if (s.list) {
-    const LimbTask t = s.list[y];
-    ptr = t.ptr;
-    pid = t.pid;
-    aux = t.aux;
-    return;
-  }
+__device__ __forceinline__ void span_get(const LimbSpan& s, u32 y, u32 logN, u64*& ptr, u32& pid) {
   u32 g = 0, start = 0;
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
@@ -61,7 +62,6 @@ __device__ __forceinline__ void span_get(const LimbSpan& s, u32 y, u32 logN, u64
   }
   ptr = s.seg_ptr[g] ? s.seg_ptr[g] + ((u64)(y - start) << logN) : nullptr;
   pid = s.seg_pid[g] + (y - start);
-  aux = y;
 }
 
 struct NttTables {
@@ -75,7 +75,7 @@ struct NttTables {
 // Fused prologue of the forward column pass.
 enum NttPro : int {
   PRO_NONE = 0,
-  // input element = SwitchModulus(src[aux-th limb of pro_src], q_src -> q_pid)
+  // input element = SwitchModulus(task.src[i], q_src_pid -> q_pid)
   // (OpenFHE NativeVector::SwitchModulus, reference src/CKKS/Rescale.cuh:47-68;
   //  fused as in src/CKKS/LimbPartitionMGPU.cu:339-373)
   PRO_SWITCH_MODULUS = 1,
@@ -91,14 +91,51 @@ enum NttEpi : int {
   EPI_RESCALE = 3,
 };
 
+// Inline (non-list) description of a fused launch: the epilogue operand,
+// accumulator and destination as spans over the same limb order, and
+// per-prime constant tables; a single prologue source limb.
 struct NttFuse {
   const u64* pro_src;  // PRO_SWITCH_MODULUS: coefficient-form source limb
   u32 pro_src_pid;     //   and its prime id
-  LimbSpan x;          // EPI_*: the x operand, same limb order as the transformed span
-  LimbSpan acc;        // EPI_MODDOWN_ADD: accumulator (may alias the output)
-  LimbSpan out;        // EPI_*: destination (if nseg == 0 and list == null, in place)
-  const u64* c0;       // per-pid constant and Shoup companion
+  LimbSpan x;          // EPI_*: the x operand
+  LimbSpan acc;        // EPI_MODDOWN_ADD: accumulator (segments may be null)
+  LimbSpan out;        // EPI_*: destination (no segments: in place)
+  const u64* c0;       // per-pid constant tables and Shoup companions
   const u64* c0sh;
   const u64* c1;
   const u64* c1sh;
 };
+
+// Resolve the task of workgroup row y from a list or from inline spans.
+__device__ __forceinline__ NttTask resolve_task(const LimbSpan& s, const NttFuse& F, u32 y, u32 logN) {
+  if (s.list) return s.list[y];
+  NttTask t;
+  span_get(s, y, logN, t.data, t.pid);
+  t.src = F.pro_src;
+  t.src_pid = F.pro_src_pid;
+  t.x = nullptr;
+  t.acc = nullptr;
+  t.out = nullptr;
+  t.c0 = t.c0sh = t.c1 = t.c1sh = 0;
+  if (F.x.nseg) {
+    u64* p;
+    u32 q;
+    span_get(F.x, y, logN, p, q);
+    t.x = p;
+    if (F.acc.nseg) {
+      span_get(F.acc, y, logN, p, q);
+      t.acc = p;
+    }
+    if (F.out.nseg) {
+      span_get(F.out, y, logN, p, q);
+      t.out = p;
+    }
+    t.c0 = F.c0[t.pid];
+    t.c0sh = F.c0sh[t.pid];
+    if (F.c1) {
+      t.c1 = F.c1[t.pid];
+      t.c1sh = F.c1sh[t.pid];
+    }
+  }
+  return t;
+}

@@ -51,42 +51,19 @@ __device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, u64 nm) {
 }
 
 template <int EPI>
-__device__ __forceinline__ u64 epilogue(u64 v, u32 y, u32 idx, u32 logN, u32 pid, const PrimeC& P,
-                                        const NttFuse& F) {
+__device__ __forceinline__ u64 epilogue(u64 v, u32 idx, const NttTask& t, u64 q) {
   if constexpr (EPI == EPI_NONE) {
     return v;
+  } else if constexpr (EPI == EPI_RESCALE) {
+    const u64 a = shoup(t.x[idx], t.c0, t.c0sh, q);
+    const u64 b = shoup(v, t.c1, t.c1sh, q);
+    return add_mod(a, b, q);
   } else {
-    u64* xp;
-    u32 xpid, xa;
-    span_get(F.x, y, logN, xp, xpid, xa);
-    const u64 x = xp[idx];
-    if constexpr (EPI == EPI_RESCALE) {
-      const u64 a = shoup(x, F.c0[pid], F.c0sh[pid], P.q);
-      const u64 b = shoup(v, F.c1[pid], F.c1sh[pid], P.q);
-      return add_mod(a, b, P.q);
-    } else {
-      u64 r = shoup(x + P.q - v, F.c0[pid], F.c0sh[pid], P.q);
-      if constexpr (EPI == EPI_MODDOWN_ADD) {
-        u64* ap;
-        u32 apid, aa;
-        span_get(F.acc, y, logN, ap, apid, aa);
-        if (ap) r = add_mod(r, ap[idx], P.q);
-      }
-      return r;
+    u64 r = shoup(t.x[idx] + q - v, t.c0, t.c0sh, q);
+    if constexpr (EPI == EPI_MODDOWN_ADD) {
+      if (t.acc) r = add_mod(r, t.acc[idx], q);
     }
-  }
-}
-
-template <int EPI>
-__device__ __forceinline__ u64* epilogue_dst(u64* data, u32 y, u32 logN, const NttFuse& F) {
-  if constexpr (EPI == EPI_NONE) {
-    return data;
-  } else {
-    if (F.out.nseg == 0 && F.out.list == nullptr) return data;
-    u64* op;
-    u32 opid, oa;
-    span_get(F.out, y, logN, op, opid, oa);
-    return op;
+    return r;
   }
 }
 
@@ -96,9 +73,9 @@ template <int LOGN, int PRO>
 __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T, NttFuse F) {
   using C = NttCfg<LOGN, 0>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
-  u64* data;
-  u32 pid, aux;
-  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
+  u64* data = tk.data;
+  const u32 pid = tk.pid;
   const u64 q = T.prm[pid].q, q2 = 2 * q;
   const u64* tw = T.psi + ((u64)pid << LOGN);
   const u64* tws = T.psi_sh + ((u64)pid << LOGN);
@@ -107,12 +84,13 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
   const u32 col = blockIdx.x * C::LPW + ll;
   u64 x[C::E];
   if constexpr (PRO == PRO_SWITCH_MODULUS) {
-    const u64 om = T.prm[F.pro_src_pid].q;
+    const u64 om = T.prm[tk.src_pid].q;
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) x[k] = switch_modulus(F.pro_src[((r0 + C::TPL * k) << C::L2) | col], om, q);
+    for (int k = 0; k < C::E; ++k) x[k] = switch_modulus(tk.src[((r0 + C::TPL * k) << C::L2) | col], om, q);
   } else {
+    const u64* in = tk.src ? tk.src : data;
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) x[k] = data[((r0 + C::TPL * k) << C::L2) | col];
+    for (int k = 0; k < C::E; ++k) x[k] = in[((r0 + C::TPL * k) << C::L2) | col];
   }
 #pragma unroll
   for (int u = 0; u < C::A; ++u) {
@@ -155,11 +133,10 @@ template <int LOGN, int EPI>
 __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
   using C = NttCfg<LOGN, 1>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
-  u64* data;
-  u32 pid, aux;
-  span_get(span, blockIdx.y, LOGN, data, pid, aux);
-  const PrimeC P = T.prm[pid];
-  const u64 q = P.q, q2 = 2 * q;
+  const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
+  u64* data = tk.data;
+  const u32 pid = tk.pid;
+  const u64 q = T.prm[pid].q, q2 = 2 * q;
   const u64* tw = T.psi + ((u64)pid << LOGN);
   const u64* tws = T.psi_sh + ((u64)pid << LOGN);
   const u32 tid = threadIdx.x;
@@ -204,12 +181,12 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
   }
   __syncthreads();
-  u64* dst = epilogue_dst<EPI>(data, blockIdx.y, LOGN, F) + ((u64)row << C::L2);
+  u64* dst = ((EPI != EPI_NONE && tk.out) ? tk.out : data) + ((u64)row << C::L2);
 #pragma unroll
   for (int k = 0; k < C::E; ++k) {
     const u32 c = r0 + C::TPL * k;
     u64 v = csub(csub(L[c], q2), q);
-    v = epilogue<EPI>(v, blockIdx.y, (row << C::L2) | c, LOGN, pid, P, F);
+    v = epilogue<EPI>(v, (row << C::L2) | c, tk, q);
     dst[c] = v;
   }
 }
@@ -221,8 +198,17 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
   using C = NttCfg<LOGN, 1>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
   u64* data;
-  u32 pid, aux;
-  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  u32 pid;
+  const u64* src;
+  if (span.list) {
+    const NttTask& t = span.list[blockIdx.y];
+    data = t.data;
+    pid = t.pid;
+    src = t.src ? t.src : t.data;
+  } else {
+    span_get(span, blockIdx.y, LOGN, data, pid);
+    src = data;
+  }
   const u64 q = T.prm[pid].q, q2 = 2 * q;
   const u64* tw = T.ipsi + ((u64)pid << LOGN);
   const u64* tws = T.ipsi_sh + ((u64)pid << LOGN);
@@ -230,10 +216,11 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
   const u32 r0 = tid % C::TPL, ll = tid / C::TPL;
   const u32 row = blockIdx.x * C::LPW + ll;
   u64* rowp = data + ((u64)row << C::L2);
+  const u64* srow = src + ((u64)row << C::L2);
   u64* L = lds + ll * C::LDS_LINE;
   u64 x[C::E];
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = rowp[r0 + C::TPL * k];
+  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = srow[r0 + C::TPL * k];
   __syncthreads();
   constexpr int EB = 1 << C::B;
 #pragma unroll
@@ -278,8 +265,13 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
   using C = NttCfg<LOGN, 0>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
   u64* data;
-  u32 pid, aux;
-  span_get(span, blockIdx.y, LOGN, data, pid, aux);
+  u32 pid;
+  if (span.list) {
+    data = span.list[blockIdx.y].data;
+    pid = span.list[blockIdx.y].pid;
+  } else {
+    span_get(span, blockIdx.y, LOGN, data, pid);
+  }
   const PrimeC P = T.prm[pid];
   const u64 q = P.q, q2 = 2 * q;
   const u64* tw = T.ipsi + ((u64)pid << LOGN);

@@ -1,0 +1,158 @@
+// fideslib.hpp -- host C++ API of the MI355X key-switching core.
+//
+// Mirrors the hot-path part of FIDESlib's public API
+// (api/CryptoContext.hpp:24-251, api/Ciphertext.hpp:13-60) over the C ABI in
+// fideslib_amd.h.  OpenFHE objects are not available here, so the data
+// boundary is FIDESlib's own interchange format: RawParams,
+// RawCipherText and RawKeySwitchKey (src/CKKS/openfhe-interface/
+// RawCiphertext.cuh:19-93) without their OpenFHE members.  An OpenFHE
+// adapter only has to fill these structs (GetRawParams, GetRawCipherText,
+// GetEvalKeySwitchKey / GetRotationKeySwitchKey do exactly that in the
+// reference).
+//
+// Errors throw std::runtime_error (the reference throws OPENFHE_THROW for
+// precondition violations, api/CryptoContext.cpp:1922-1924).
+#ifndef FIDESLIB_AMD_FIDESLIB_HPP
+#define FIDESLIB_AMD_FIDESLIB_HPP
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+extern "C" {
+struct fides_context;
+struct fides_key;
+}
+
+namespace fideslib {
+
+enum ScalingTechnique { FIXEDMANUAL = 0, FIXEDAUTO = 1, FLEXIBLEAUTO = 2, FLEXIBLEAUTOEXT = 3 };
+
+// Parameters (RawParams, RawCiphertext.cuh:45-83).  Roots may be left empty
+// (the smallest-generator root is then used for every prime).
+struct RawParams {
+  int logN = 0;
+  std::vector<uint64_t> moduli;                 // Q primes, level 0 first
+  std::vector<uint64_t> root_of_unity;          // primitive 2N-th roots (optional)
+  std::vector<uint64_t> SPECIALmoduli;          // P primes
+  std::vector<uint64_t> SPECIALroot_of_unity;   // (optional)
+  int dnum = 1;
+  ScalingTechnique scalingTechnique = FIXEDMANUAL;
+  std::vector<double> ModReduceFactor;          // per level (optional; default = the dropped prime)
+};
+
+// Ciphertext in RNS evaluation form (RawCipherText, RawCiphertext.cuh:19-35).
+struct RawCipherText {
+  std::vector<std::vector<uint64_t>> sub_0;  // [numRes][N]
+  std::vector<std::vector<uint64_t>> sub_1;
+  int numRes = 0;
+  int N = 0;
+  double Noise = 0.0;  // scaling factor (NoiseFactor)
+  int NoiseLevel = 1;  // scale degree
+  int slots = 0;
+  std::string keyid;
+};
+
+// Key-switching key (RawKeySwitchKey, RawCiphertext.cuh:85-90):
+// r_key[0] = a, r_key[1] = b, each [dnum][L+K][N].
+struct RawKeySwitchKey {
+  std::vector<std::vector<std::vector<std::vector<uint64_t>>>> r_key;
+  std::string keyid;
+};
+
+class CryptoContextImpl;
+using CryptoContext = std::shared_ptr<CryptoContextImpl>;
+
+// A device-resident ciphertext: [2][l][N] evaluation form plus metadata
+// (reference Ciphertext, src/CKKS/Ciphertext.cuh:34-54).
+class CiphertextImpl {
+ public:
+  ~CiphertextImpl();
+  CiphertextImpl(const CiphertextImpl&) = delete;
+  CiphertextImpl& operator=(const CiphertextImpl&) = delete;
+
+  uint32_t GetLevel() const { return limbs - 1; }  // FIDESlib level = limbs - 1
+  uint32_t GetLimbs() const { return limbs; }
+  int GetNoiseScaleDeg() const { return NoiseLevel; }
+  double GetScalingFactor() const { return NoiseFactor; }
+  int GetSlots() const { return slots; }
+  uint64_t* DeviceData() const { return d; }
+
+  uint32_t limbs = 0;
+  int NoiseLevel = 1;
+  double NoiseFactor = 0.0;
+  int slots = 0;
+  std::string keyid;
+
+ private:
+  friend class CryptoContextImpl;
+  explicit CiphertextImpl(std::shared_ptr<CryptoContextImpl> cc, uint32_t limbs);
+  std::shared_ptr<CryptoContextImpl> cc;
+  uint64_t* d = nullptr;
+};
+using Ciphertext = std::shared_ptr<CiphertextImpl>;
+
+class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl> {
+ public:
+  // GenCryptoContext + LoadContext (api/CryptoContext.cpp:199-249).
+  static CryptoContext Create(const RawParams& params);
+  ~CryptoContextImpl();
+
+  uint32_t GetRingDimension() const { return N; }
+  uint32_t GetCyclotomicOrder() const { return 2 * N; }
+  ScalingTechnique GetScalingTechnique() const { return technique; }
+
+  // ---- keys (LoadContext's key upload, api/CryptoContext.cpp:229-241)
+  void AddEvalMultKey(const RawKeySwitchKey& key);
+  void AddRotationKey(int32_t index, const RawKeySwitchKey& key);  // automorphism 5^index mod 2N
+  void AddConjugateKey(const RawKeySwitchKey& key);                // automorphism 2N - 1
+  bool HasRotationKey(int32_t index) const;
+
+  // ---- data movement (LoadCiphertext / Decrypt's store, api/CryptoContext.cpp:270-288, 603-622)
+  Ciphertext LoadCiphertext(const RawCipherText& raw);
+  RawCipherText StoreCiphertext(const Ciphertext& ct);
+
+  // ---- operations (api/CryptoContext.hpp:149-181)
+  Ciphertext EvalMult(const Ciphertext& a, const Ciphertext& b);
+  void EvalMultInPlace(Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalSquare(const Ciphertext& a);
+  void EvalSquareInPlace(Ciphertext& a);
+  Ciphertext EvalRotate(const Ciphertext& a, int32_t index);
+  void EvalRotateInPlace(Ciphertext& a, int32_t index);
+  Ciphertext EvalConjugate(const Ciphertext& a);
+  Ciphertext Rescale(const Ciphertext& a);
+  void RescaleInPlace(Ciphertext& a);
+  // Drop limbs above `limbs` (OpenFHE LevelReduce / FIDESlib dropToLevel).
+  Ciphertext DropToLimbs(const Ciphertext& a, uint32_t limbs);
+
+  // ---- cross-ciphertext batching (one launch per pipeline stage)
+  std::vector<Ciphertext> EvalMultMany(const std::vector<Ciphertext>& a, const std::vector<Ciphertext>& b,
+                                       bool rescale);
+  std::vector<Ciphertext> EvalRotateMany(const std::vector<Ciphertext>& a, const std::vector<int32_t>& indices);
+
+  void Synchronize() const;
+
+  fides_context* Handle() const { return ctx; }
+
+ private:
+  CryptoContextImpl() = default;
+  Ciphertext NewCiphertext(uint32_t limbs);
+  uint32_t AutoIndex(int32_t index) const;
+  const fides_key* RotationKey(uint32_t autoIndex) const;
+  void PrepareMult(Ciphertext& a, Ciphertext& b);  // FIXEDAUTO level/scale alignment
+  void MultMetadata(CiphertextImpl& out, const CiphertextImpl& a, const CiphertextImpl& b) const;
+
+  fides_context* ctx = nullptr;
+  uint32_t logN = 0, N = 0, L = 0, K = 0, dnum = 0;
+  ScalingTechnique technique = FIXEDMANUAL;
+  std::vector<uint64_t> q;
+  std::vector<double> modReduceFactor;
+  fides_key* evalMultKey = nullptr;
+  std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
+};
+
+}  // namespace fideslib
+
+#endif

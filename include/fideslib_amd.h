@@ -82,6 +82,7 @@ int fides_device_alloc(size_t bytes, void** dptr);
 int fides_device_free(void* dptr);
 int fides_copy_to_device(void* dst, const void* src, size_t bytes, void* stream);
 int fides_copy_to_host(void* dst, const void* src, size_t bytes, void* stream);
+int fides_copy_device(void* dst, const void* src, size_t bytes, void* stream); /* async, stream-ordered */
 int fides_stream_sync(void* stream);
 
 /* ---- raw limb operations (config: NTT/INTT + ModUp/ModDown) ---------- */
@@ -148,6 +149,25 @@ int fides_rescale(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint64_t
  * (api/CryptoContext.cpp:1406-1430). */
 int fides_rotate(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fides_key* key, uint32_t auto_index,
                  uint64_t* d_out, void* stream);
+
+/* ---- batched operations (cross-ciphertext batching) ------------------ */
+
+/* n ciphertext pairs at individual limb counts l[i] in one pass: every
+ * pipeline stage (tensor, INTT, ModUp conversion, NTT, inner product,
+ * ModDown, Rescale) is one launch over the limbs of all n ciphertexts.
+ * With rescale != 0, out[i] is [2][l[i]-1] (HMult + relin + Rescale), else
+ * [2][l[i]].  Results are identical to n calls of fides_hmult (+
+ * fides_rescale).  The reference batches only inside linear transforms
+ * (src/CKKS/LimbPartitionBatch.cu:20-616); this entry point has no single
+ * reference counterpart. */
+int fides_hmult_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint64_t* const* d_b,
+                      const uint32_t* l, const fides_key* rlk, int rescale, uint64_t* const* d_out, void* stream);
+
+/* n rotations/automorphisms, ciphertext i with its own key and index.
+ * Identical to n calls of fides_rotate. */
+int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
+                       const fides_key* const* keys, const uint32_t* auto_index, uint64_t* const* d_out,
+                       void* stream);
 
 /* Keyless evaluation-domain automorphism of nlimbs limbs (permutation only,
  * reference automorph_ src/Rotation.cu:10-28).  d_out must not alias. */

@@ -295,3 +295,72 @@ def test_cfg3_rotate_bit_exact(cfg3):
     assert np.array_equal(got, octx.rotate(c1, l, key, k))
     dec = C.decode(keys.decrypt_coef(got, l), 2.0 ** prm.scale_bits, N)
     assert C.precision_bits(dec, np.roll(z[0], -r)) > 30
+
+
+# ------------------------------------------------------- batched = plain = oracle
+def test_cfg1_batch_equals_single_and_oracle(cfg1):
+    prm, octx, gctx, keys = cfg1
+    say("cfg1 batch")
+    N = prm.N
+    rlk = keys.relin_key()
+    g_rlk = gctx.upload_key(rlk)
+    levels = [4, 2, 3, 4, 2]
+    pairs = [_enc_pair(prm, keys, 40 + i, l)[1] for i, l in enumerate(levels)]
+    A = [capi.DeviceBuffer.from_array(p[0]) for p in pairs]
+    B = [capi.DeviceBuffer.from_array(p[1]) for p in pairs]
+    OB = [capi.DeviceBuffer(2 * (l - 1) * N) for l in levels]
+    gctx.hmult_batch(A, B, levels, g_rlk, OB, rescale=True)
+    for i, l in enumerate(levels):
+        want = octx.rescale_ct(octx.hmult_relin(pairs[i][0], pairs[i][1], l, rlk), l)
+        assert np.array_equal(OB[i].download((2, l - 1, N)), want), i
+    # without rescale
+    OM = [capi.DeviceBuffer(2 * l * N) for l in levels]
+    gctx.hmult_batch(A, B, levels, g_rlk, OM, rescale=False)
+    for i, l in enumerate(levels):
+        assert np.array_equal(OM[i].download((2, l, N)), octx.hmult_relin(pairs[i][0], pairs[i][1], l, rlk)), i
+    # rotations with distinct keys
+    ks = [5, 25, 2 * N - 1, pow(5, 7, 2 * N), 5]
+    hkeys = {k: keys.rotation_key(k) for k in set(ks)}
+    gkeys = {k: gctx.upload_key(v) for k, v in hkeys.items()}
+    lv2 = [l - 1 for l in levels]
+    RO = [capi.DeviceBuffer(2 * l * N) for l in lv2]
+    gctx.rotate_batch(OB, lv2, [gkeys[k] for k in ks], ks, RO)
+    for i, l in enumerate(lv2):
+        cin = OB[i].download((2, l, N))
+        want = octx.rotate(cin, l, hkeys[ks[i]], ks[i])
+        assert np.array_equal(RO[i].download((2, l, N)), want), i
+        single = capi.DeviceBuffer(2 * l * N)
+        gctx.rotate(OB[i], l, gkeys[ks[i]], ks[i], single)
+        assert np.array_equal(single.download((2, l, N)), want), i
+
+
+def test_cfg3_batch_mixed_levels_bit_exact(cfg3):
+    """16 ciphertexts at levels drawn from U{6..30} in one batched call;
+    each equals the oracle and the single-ciphertext GPU call."""
+    prm, octx, gctx, keys = cfg3
+    say("cfg3 batch")
+    N = prm.N
+    rng = np.random.default_rng(2026)
+    levels = [int(x) for x in rng.integers(6, prm.L + 1, 16)]
+    levels[0], levels[1] = 30, 6
+    rlk = keys.relin_key()
+    g_rlk = gctx.upload_key(rlk)
+    cts = []
+    for i, l in enumerate(levels):
+        a = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        b = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        cts.append((a, b))
+    A = [capi.DeviceBuffer.from_array(a) for a, _ in cts]
+    B = [capi.DeviceBuffer.from_array(b) for _, b in cts]
+    OB = [capi.DeviceBuffer(2 * (l - 1) * N) for l in levels]
+    gctx.hmult_batch(A, B, levels, g_rlk, OB, rescale=True)
+    for i in (0, 1, 7, 15):
+        l = levels[i]
+        want = octx.rescale_ct(octx.hmult_relin(cts[i][0], cts[i][1], l, rlk), l)
+        assert np.array_equal(OB[i].download((2, l - 1, N)), want), (i, l)
+    for i, l in enumerate(levels):
+        m = capi.DeviceBuffer(2 * l * N)
+        r = capi.DeviceBuffer(2 * (l - 1) * N)
+        gctx.hmult(A[i], B[i], l, g_rlk, m)
+        gctx.rescale(m, l, r)
+        assert np.array_equal(r.download((2, l - 1, N)), OB[i].download((2, l - 1, N))), (i, l)
