@@ -150,6 +150,12 @@ static u64 default_root(u64 q, u32 logN) {
   }
   return 0;
 }
+// Inverse of an odd k modulo a power of two m.
+static u32 inv_odd(u32 k, u32 m) {
+  u64 x = k;  // Newton iteration, doubling correct bits
+  for (int i = 0; i < 5; ++i) x = x * (2 - (u64)k * x);
+  return (u32)(x & (m - 1));
+}
 static u64 shoup_of(u64 w, u64 q) { return (u64)(((u128)w << 64) / q); }
 static u32 bitlen(u64 x) { return 64 - __builtin_clzll(x); }
 
@@ -332,6 +338,7 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     const u64 w1 = powm(ipsi, N / 2, q);  // psi^-brev(1)
     p.ninv_w = mulm(w1, p.ninv, q);
     p.ninv_w_sh = shoup_of(p.ninv_w, q);
+    p.one_sh = shoup_of(1, q);
     u64* t_psi = tw.data() + ((size_t)0 * NP + i) * N;
     u64* t_psish = tw.data() + ((size_t)1 * NP + i) * N;
     u64* t_ipsi = tw.data() + ((size_t)2 * NP + i) * N;
@@ -614,17 +621,20 @@ struct KsJob {
   const u64* d;  // l limbs, evaluation form
   u32 l;
   const fides_key* key;
-  u64* out0;
+  u64* out0;        // l limbs each (or, with rescale, [2][l-1] at out0)
   u64* out1;
   const u64* acc0;  // nullable
   const u64* acc1;  // nullable
+  u32 perm = 0;     // store the result through the automorphism with this k^-1 (0: none)
+  bool rescale = false;  // fuse Rescale into the ModDown: out0 = [2][l-1] result of
+                         // Rescale(acc + KeySwitch(d)); acc0/acc1 required
 };
 
 // Arena words needed by the key switch of one job.
 static size_t ks_words(const fides_context* c, u32 l) {
   const size_t N = c->N;
   const u32 beta = num_digits(c, l);
-  return N * ((size_t)l + (size_t)beta * (l + c->K) + 2 * (size_t)(l + c->K) + 2 * (size_t)l);
+  return N * ((size_t)l + (size_t)beta * (l + c->K) + 2 * (size_t)(l + c->K) + 2 * (size_t)l + 2);
 }
 
 // Appends the task lists of the batched key switch (reference pipeline
@@ -636,18 +646,21 @@ static size_t ks_words(const fides_context* c, u32 l) {
 //   5 INTT of the special limbs of both products
 //   6 ModDown conversion P -> Q_l of both products
 //   7 NTT fused with (x - conv) * P^-1 [+ acc] into the outputs
+//   (fused Rescale jobs) top limb: acc + T P^-1 -> INTT -> - conv P^-1, then
+//   one NTT per remaining limb with PRO_MDRS / EPI_MDRS (see ntt.h).
 struct KsLists {
-  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md;
+  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, axpy1, intt_top, axpy2, ntt_mdrs;
   u32 n_intt_in = 0, n_conv_up[CONV_MAXS + 1] = {}, n_ntt_ext = 0, n_dot = 0, n_intt_sp = 0, n_conv_down = 0,
-      n_ntt_md = 0;
+      n_ntt_md = 0, n_axpy1 = 0, n_intt_top = 0, n_axpy2 = 0, n_ntt_mdrs = 0;
   bool any_acc = false;
 };
 
 static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scratch, Blob& blob, KsLists& K_) {
   const u32 N = c->N, K = c->K, L = c->L;
-  std::vector<NttTask> intt_in, ntt_ext, intt_sp, ntt_md;
+  std::vector<NttTask> intt_in, ntt_ext, intt_sp, ntt_md, intt_top, ntt_mdrs;
   std::vector<ConvTask> conv_up[CONV_MAXS + 1], conv_down;
   std::vector<DotTask> dot;
+  std::vector<AxpyTask> axpy1, axpy2;
   u64* w = scratch;
   for (const KsJob& J : jobs) {
     const u32 l = J.l, beta = num_digits(c, l), ext_stride = l + K;
@@ -656,7 +669,8 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
     u64* T0 = EXT + (size_t)beta * ext_stride * N;
     u64* T1 = T0 + (size_t)(l + K) * N;
     u64* Y = T1 + (size_t)(l + K) * N;
-    w = Y + (size_t)2 * l * N;
+    u64* RS = Y + (size_t)2 * l * N;
+    w = RS + (size_t)2 * N;
     for (u32 i = 0; i < l; ++i) {
       NttTask t = ntask(c->limb(Dc, i), i);
       t.src = c->limb(J.d, i);
@@ -699,15 +713,51 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
       u64* Yp = Y + (size_t)p * l * N;
       for (u32 t0 = 0; t0 < md.nt; t0 += CONV_TCHUNK)
         conv_down.push_back(ConvTask{c->limb(Tp[p], l), Yp, c->moddown + l, t0, std::min(md.nt, t0 + CONV_TCHUNK)});
-      for (u32 i = 0; i < l; ++i) {
-        NttTask t = ntask(c->limb(Yp, i), i);
+      if (!J.rescale) {
+        for (u32 i = 0; i < l; ++i) {
+          NttTask t = ntask(c->limb(Yp, i), i);
+          t.x = c->limb(Tp[p], i);
+          t.acc = accp[p] ? c->limb(accp[p], i) : nullptr;
+          t.out = c->limb(outp[p], i);
+          t.c0 = c->pinv[i];
+          t.c0sh = c->pinv_sh[i];
+          t.perm = J.perm;
+          ntt_md.push_back(t);
+          if (accp[p]) K_.any_acc = true;
+        }
+        continue;
+      }
+      // ModDown fused with Rescale (l -> l-1).  With R = acc + (T - NTT(conv)) P^-1
+      // the rescaled limb i is R_i ql^-1 + NTT(SwitchModulus(INTT(R_top))) QlQlInv_i,
+      // which equals acc_i ql^-1 + T_i (P^-1 ql^-1) + NTT(z_i) with
+      // z_i = SwitchModulus(top) QlQlInv_i - conv_i (P^-1 ql^-1) and
+      // top = INTT(acc_top + T_top P^-1) - conv_top P^-1 (all maps are linear mod q_i).
+      const u32 tp = l - 1;
+      const u64 qt = c->primes[tp];
+      u64* rs = RS + (size_t)p * N;
+      axpy1.push_back(AxpyTask{c->limb(accp[p], tp), c->limb(Tp[p], tp), rs, c->pinv[tp], c->pinv_sh[tp], tp, 0});
+      intt_top.push_back(ntask(rs, tp));
+      const u64 npinv = (qt - c->pinv[tp]) % qt;
+      axpy2.push_back(AxpyTask{rs, c->limb(Yp, tp), rs, npinv, shoup_of(npinv, qt), tp, 0});
+      u64* dst = J.out0 + (size_t)p * (l - 1) * N;
+      for (u32 i = 0; i + 1 < l; ++i) {
+        const u64 qi = c->primes[i];
+        const size_t o = (size_t)l * L + i;
+        const u64 pq = mulm(c->pinv[i], c->qlinv[o], qi);
+        NttTask t = ntask(c->limb(dst, i), i);
+        t.src = rs;
+        t.src_pid = tp;
+        t.swdiff = qi - qt % qi;
+        t.src2 = c->limb(Yp, i);
         t.x = c->limb(Tp[p], i);
-        t.acc = accp[p] ? c->limb(accp[p], i) : nullptr;
-        t.out = c->limb(outp[p], i);
-        t.c0 = c->pinv[i];
-        t.c0sh = c->pinv_sh[i];
-        ntt_md.push_back(t);
-        if (accp[p]) K_.any_acc = true;
+        t.acc = c->limb(accp[p], i);
+        t.c0 = c->qlinv[o];
+        t.c0sh = c->qlinv_sh[o];
+        t.c2 = c->qlql[o];
+        t.c2sh = c->qlql_sh[o];
+        t.c3 = pq;
+        t.c3sh = shoup_of(pq, qi);
+        ntt_mdrs.push_back(t);
       }
     }
   }
@@ -720,6 +770,9 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   by_prime(ntt_ext);
   by_prime(intt_sp);
   by_prime(ntt_md);
+  by_prime(ntt_mdrs);
+  // key limbs of the same prime (shared by every job using the same key) next to each other
+  std::stable_sort(dot.begin(), dot.end(), [](const DotTask& a, const DotTask& b) { return a.pid < b.pid; });
   K_.intt_in = blob.add(intt_in);
   K_.n_intt_in = (u32)intt_in.size();
   for (u32 ns = 1; ns <= CONV_MAXS; ++ns) {
@@ -736,6 +789,14 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   K_.n_conv_down = (u32)conv_down.size();
   K_.ntt_md = blob.add(ntt_md);
   K_.n_ntt_md = (u32)ntt_md.size();
+  K_.axpy1 = blob.add(axpy1);
+  K_.n_axpy1 = (u32)axpy1.size();
+  K_.intt_top = blob.add(intt_top);
+  K_.n_intt_top = (u32)intt_top.size();
+  K_.axpy2 = blob.add(axpy2);
+  K_.n_axpy2 = (u32)axpy2.size();
+  K_.ntt_mdrs = blob.add(ntt_mdrs);
+  K_.n_ntt_mdrs = (u32)ntt_mdrs.size();
 }
 
 static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t st) {
@@ -748,8 +809,15 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
                            c->L + c->K, st);
   RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_sp), K_.n_intt_sp, st));
   RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_down), K_.n_conv_down, c->K, st));
-  RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_md), K_.n_ntt_md, PRO_NONE,
-                       K_.any_acc ? EPI_MODDOWN_ADD : EPI_MODDOWN, st));
+  if (K_.n_ntt_md)
+    RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_md), K_.n_ntt_md, PRO_NONE,
+                         K_.any_acc ? EPI_MODDOWN_ADD : EPI_MODDOWN, st));
+  if (K_.n_ntt_mdrs) {
+    fides_launch_axpy_batch(at<AxpyTask>(dev, K_.axpy1), K_.n_axpy1, c->prm, c->logN, st);
+    RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_top), K_.n_intt_top, st));
+    fides_launch_axpy_batch(at<AxpyTask>(dev, K_.axpy2), K_.n_axpy2, c->prm, c->logN, st);
+    RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_mdrs), K_.n_ntt_mdrs, PRO_MDRS, EPI_MDRS, st));
+  }
   return FIDES_OK;
 }
 
@@ -784,6 +852,7 @@ static void rs_build(fides_context* c, const std::vector<RsJob>& jobs, u64* scra
         NttTask u = ntask(c->limb(outp, i), i);
         u.src = RS + (size_t)p * N;
         u.src_pid = l - 1;
+        u.swdiff = c->primes[i] - c->primes[l - 1] % c->primes[i];
         u.x = c->limb(inp, i);
         const size_t o = (size_t)l * L + i;
         u.c0 = c->qlinv[o];
@@ -1009,17 +1078,16 @@ static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* 
     if (!a[i] || (!square && !b[i]) || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     if (rescale && lv[i] < 2) return fail(FIDES_ERR_ARG, "rescale needs l >= 2");
   }
-  // arena: [key-switch scratch | per job: c1d1 (l) | product [2][l] if rescaling | rescale top limbs (2)]
+  // arena: [key-switch scratch | per job: c1d1 (l) | tensor product [2][l] if rescaling]
   size_t ksw = 0, extra = 0;
   for (u32 i = 0; i < n; ++i) {
     ksw += ks_words(c, lv[i]);
-    extra += (size_t)N * (lv[i] + (rescale ? 2 * lv[i] + 2 : 0));
+    extra += (size_t)N * (lv[i] + (rescale ? 2 * lv[i] : 0));
   }
   RCHK(ensure_arena(c, ksw + extra, st));
   u64* w = c->arena + ksw;
   std::vector<TensorTask> tens;
   std::vector<KsJob> jobs;
-  std::vector<RsJob> rjobs;
   for (u32 i = 0; i < n; ++i) {
     const u32 l = lv[i];
     u64* D = w;
@@ -1043,15 +1111,18 @@ static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* 
       tt.square = square ? 1u : 0u;
       tens.push_back(tt);
     }
-    jobs.push_back(KsJob{D, l, rlk, prod, prod + S, prod, prod + S});
-    if (rescale) rjobs.push_back(RsJob{prod, l, out[i]});
+    KsJob J{D, l, rlk, prod, prod + S, prod, prod + S};
+    if (rescale) {
+      J.rescale = true;
+      J.out0 = out[i];
+      J.out1 = nullptr;
+    }
+    jobs.push_back(J);
   }
   Blob blob;
   const size_t o_tens = blob.add(tens);
   KsLists K_;
   ks_build(c, jobs, c->arena, blob, K_);
-  RsLists R;
-  if (rescale) rs_build(c, rjobs, w, blob, R);
   char* dev;
   int slot;
   RCHK(upload_blob(c, blob, st, &dev, &slot));
@@ -1059,7 +1130,6 @@ static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* 
     fides_launch_tensor_batch(at<TensorTask>(dev, o_tens) + o, std::min<u32>(65535, (u32)tens.size() - o), c->prm,
                               c->logN, st);
   RCHK(ks_launch(c, dev, K_, st));
-  if (rescale) RCHK(rs_launch(c, dev, R, st));
   RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
@@ -1130,37 +1200,30 @@ extern "C" int fides_automorph(fides_context* c, const uint64_t* in, uint32_t n,
 static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32* lv, const fides_key* const* keys,
                         const u32* kidx, u64* const* out, hipStream_t st) {
   const u32 N = c->N;
-  size_t ksw = 0, extra = 0;
+  size_t ksw = 0;
   for (u32 i = 0; i < n; ++i) {
     RCHK(check_key(c, keys[i], lv[i]));
     if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     if (in[i] == out[i]) return fail(FIDES_ERR_ARG, "rotate output must not alias its input");
     if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
     ksw += ks_words(c, lv[i]);
-    extra += (size_t)2 * lv[i] * N;
   }
-  RCHK(ensure_arena(c, ksw + extra, st));
-  u64* w = c->arena + ksw;
+  RCHK(ensure_arena(c, ksw, st));
   std::vector<KsJob> jobs;
-  std::vector<AutoTask> au;
   for (u32 i = 0; i < n; ++i) {
-    const u32 l = lv[i];
-    const size_t S = (size_t)l * N;
-    u64* tmp = w;
-    w += 2 * S;
-    jobs.push_back(KsJob{in[i] + S, l, keys[i], tmp, tmp + S, in[i], nullptr});
-    for (u32 t = 0; t < 2 * l; ++t) au.push_back(AutoTask{c->limb(tmp, t), c->limb(out[i], t), kidx[i], 0});
+    const size_t S = (size_t)lv[i] * N;
+    KsJob J{in[i] + S, lv[i], keys[i], out[i], out[i] + S, in[i], nullptr};
+    // the ModDown store applies the permutation (scatter with k^-1)
+    J.perm = inv_odd(kidx[i], 2 * N);
+    jobs.push_back(J);
   }
   Blob blob;
   KsLists K_;
   ks_build(c, jobs, c->arena, blob, K_);
-  const size_t o_au = blob.add(au);
   char* dev;
   int slot;
   RCHK(upload_blob(c, blob, st, &dev, &slot));
   RCHK(ks_launch(c, dev, K_, st));
-  for (u32 x = 0; x < au.size(); x += 65535)
-    fides_launch_automorph_batch(at<AutoTask>(dev, o_au) + x, std::min<u32>(65535, (u32)au.size() - x), c->logN, st);
   RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;

@@ -52,6 +52,16 @@ struct TensorTask {
   u32 square;
 };
 
+// One limb of out = a + b * c (mod q_pid) with a constant c.
+struct AxpyTask {
+  const u64* a;
+  const u64* b;
+  u64* out;
+  u64 c, csh;
+  u32 pid;
+  u32 pad;
+};
+
 // One limb of an evaluation-domain automorphism.
 struct AutoTask {
   const u64* in;
@@ -64,6 +74,7 @@ int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, const Pri
 void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, u32 kstride,
                             hipStream_t st);
 void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
+void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,

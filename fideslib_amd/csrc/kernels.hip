@@ -41,25 +41,37 @@ __global__ __launch_bounds__(256) void k_tensor_batch(const TensorTask* __restri
 // yields the exact residue.  Coefficient form in and out.  The NS sources
 // are loaded and pre-scaled once per thread and reused for every target of
 // the task's chunk.
+// Descriptors and prime constants are read-only for the whole launch and
+// uniform across a workgroup: reading them through the constant address
+// space lets them come in through scalar loads.
+#define CONSTANT_AS __attribute__((address_space(4)))
+
 template <int NS>
 __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__ tasks,
-                                                    const PrimeC* __restrict__ prm, u32 logN) {
-  const ConvTask tk = tasks[blockIdx.y];
-  const ConvDesc* __restrict__ d = tk.desc;
+                                                    const PrimeC* __restrict__ prm_, u32 logN) {
+  const CONSTANT_AS ConvTask* tp = (const CONSTANT_AS ConvTask*)(tasks + blockIdx.y);
+  const CONSTANT_AS ConvDesc* d = (const CONSTANT_AS ConvDesc*)tp->desc;
+  const CONSTANT_AS PrimeC* prm = (const CONSTANT_AS PrimeC*)prm_;
+  const u64* __restrict__ in = tp->in;
+  u64* __restrict__ out = tp->out;
+  const u32 t0 = tp->t0, t1 = tp->t1;
   const u32 k = blockIdx.x * 256u + threadIdx.x;
   u64 xs[NS];
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
     const u64 q = prm[d->src_pid[i]].q;
-    xs[i] = shoup(tk.in[((u64)i << logN) + k], d->hatinv[i], d->hatinv_sh[i], q);
+    xs[i] = shoup(in[((u64)i << logN) + k], d->hatinv[i], d->hatinv_sh[i], q);
   }
-  for (u32 t = tk.t0; t < tk.t1; ++t) {
-    const PrimeC P = prm[d->dst_pid[t]];
-    const u64* m = d->mat[t];
-    u64 hi = 0, lo = 0;
+  for (u32 t = t0; t < t1; ++t) {
+    PrimeC P;
+    P.q = prm[d->dst_pid[t]].q;
+    P.qneg_inv = prm[d->dst_pid[t]].qneg_inv;
+    Acc128 acc;
 #pragma unroll
-    for (int i = 0; i < NS; ++i) mac128(hi, lo, xs[i], m[i]);
-    tk.out[((u64)d->dst_off[t] << logN) + k] = mont_redc(hi, lo, P);
+    for (int i = 0; i < NS; ++i) acc_mac(acc, xs[i], d->mat[t][i]);
+    u64 hi, lo;
+    acc_value(acc, hi, lo);
+    out[((u64)d->dst_off[t] << logN) + k] = mont_redc(hi, lo, P);
   }
 }
 
@@ -75,14 +87,17 @@ __global__ __launch_bounds__(256) void k_dot_batch(const DotTask* __restrict__ t
   const u64 k = blockIdx.x * 256u + threadIdx.x;
   const PrimeC P = prm[t.pid];
   const u64 es = (u64)t.estride << logN, ks = (u64)kstride << logN;
-  u64 h0 = 0, l0 = 0, h1 = 0, l1 = 0;
+  Acc128 a0, a1;
   for (u32 j = 0; j < t.beta; ++j) {
     const u64 e = (j == t.jown) ? t.d[k] : t.e[j * es + k];
-    mac128(h0, l0, e, t.kb[j * ks + k]);
-    mac128(h1, l1, e, t.ka[j * ks + k]);
+    acc_mac_v(a0, e, t.kb[j * ks + k]);
+    acc_mac_v(a1, e, t.ka[j * ks + k]);
   }
-  t.o0[k] = mont_redc(h0, l0, P);
-  t.o1[k] = mont_redc(h1, l1, P);
+  u64 h, l;
+  acc_value(a0, h, l);
+  t.o0[k] = mont_redc(h, l, P);
+  acc_value(a1, h, l);
+  t.o1[k] = mont_redc(h, l, P);
 }
 
 // ---------------------------------------------------------------- key conversion
@@ -109,7 +124,21 @@ __global__ __launch_bounds__(256) void k_automorph_batch(const AutoTask* __restr
   t.out[i] = t.in[__brev(s) >> (32 - logN)];
 }
 
+// ---------------------------------------------------------------- axpy
+__global__ __launch_bounds__(256) void k_axpy_batch(const AxpyTask* __restrict__ tasks,
+                                                    const PrimeC* __restrict__ prm, u32 logN) {
+  const AxpyTask t = tasks[blockIdx.y];
+  const u32 k = blockIdx.x * 256u + threadIdx.x;
+  const u64 q = prm[t.pid].q;
+  t.out[k] = add_mod(t.a[k], shoup(t.b[k], t.c, t.csh, q), q);
+}
+
 // ---------------------------------------------------------------- launchers
+void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
+  if (ntasks == 0) return;
+  k_axpy_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
+}
+
 int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, const PrimeC* prm, u32 logN, hipStream_t st) {
   if (ntasks == 0) return 0;
   dim3 g((1u << logN) / 256, ntasks);

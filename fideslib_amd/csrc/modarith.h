@@ -35,6 +35,7 @@ struct PrimeC {
   u64 ninv_sh;   // Shoup companion of ninv
   u64 ninv_w;    // ipsi[1] * N^-1 mod q (last inverse stage, scaled)
   u64 ninv_w_sh;
+  u64 one_sh;    // floor(2^64 / q): Shoup companion of 1, reduces any x < 2^64
   u32 bits;
   u32 pad;
 };
@@ -98,12 +99,64 @@ __device__ __forceinline__ u64 barrett_mul(u64 a, u64 b, const PrimeC& P) {
   return csub(r, P.q);
 }
 
-// 128-bit accumulate: (hi, lo) += a * b.
+// 128-bit accumulate: (hi, lo) += a * b, with the low-word carry taken from
+// an explicit add-with-carry.
 __device__ __forceinline__ void mac128(u64& hi, u64& lo, u64 a, u64 b) {
-  u64 h, l;
-  mul64wide(a, b, h, l);
-  lo += l;
-  hi += h + (lo < l);
+  const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+  const u64 p00 = (u64)a0 * b0;
+  const u64 t = (u64)a0 * b1 + (p00 >> 32);  // < 2^64
+  const u64 u = (u64)a1 * b0 + (u32)t;       // < 2^64
+  const u64 h = (u64)a1 * b1 + (t >> 32) + (u >> 32);
+  const u64 l = (u >> 32 << 32) | (u32)p00;
+  unsigned long long c;
+  lo = __builtin_addcll(lo, l, 0ull, &c);
+  hi = hi + h + c;
+}
+
+// Column accumulator for sums of up to 16 products x * m with x, m < 2^62:
+//   T = L + 2^32 M + 2^64 (H + cL) + 2^96 cM
+// where L = sum x0 m0, M = sum (x0 m1 + x1 m0) and H = sum x1 m1 (32-bit
+// halves), each kept modulo 2^64 with the carries out of L and M counted
+// separately.  Every product is one v_mad_u64_u32 accumulating into its
+// column; the carry-out of the instruction feeds a 32-bit counter, so no
+// carry is propagated inside the loop.  (ALU-only inline assembly: the
+// compiler cannot express a consumed carry-out of v_mad_u64_u32.)
+struct Acc128 {
+  u64 L = 0, M = 0, H = 0;
+  u32 cL = 0, cM = 0;
+};
+
+__device__ __forceinline__ void acc_mac(Acc128& a, u64 x, u64 m) {
+  const u32 x0 = (u32)x, x1 = (u32)(x >> 32), m0 = (u32)m, m1 = (u32)(m >> 32);
+  u64 c0, c1, c2;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.L), "=s"(c0) : "v"(x0), "s"(m0));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cL), "=s"(c2) : "s"(c0));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.M), "=s"(c1) : "v"(x0), "s"(m1));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cM), "=s"(c2) : "s"(c1));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.M), "=s"(c1) : "v"(x1), "s"(m0));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cM), "=s"(c2) : "s"(c1));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.H), "=s"(c1) : "v"(x1), "s"(m1));
+}
+
+// Same with both operands per lane.
+__device__ __forceinline__ void acc_mac_v(Acc128& a, u64 x, u64 m) {
+  const u32 x0 = (u32)x, x1 = (u32)(x >> 32), m0 = (u32)m, m1 = (u32)(m >> 32);
+  u64 c0, c1, c2;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.L), "=s"(c0) : "v"(x0), "v"(m0));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cL), "=s"(c2) : "s"(c0));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.M), "=s"(c1) : "v"(x0), "v"(m1));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cM), "=s"(c2) : "s"(c1));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.M), "=s"(c1) : "v"(x1), "v"(m0));
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(a.cM), "=s"(c2) : "s"(c1));
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(a.H), "=s"(c1) : "v"(x1), "v"(m1));
+}
+
+// The 128-bit value (hi, lo) of a column accumulator.
+__device__ __forceinline__ void acc_value(const Acc128& a, u64& hi, u64& lo) {
+  const u64 ms = a.M << 32;
+  unsigned long long c;
+  lo = __builtin_addcll(a.L, ms, 0ull, &c);
+  hi = a.H + (u64)a.cL + (a.M >> 32) + ((u64)a.cM << 32) + c;
 }
 
 // Montgomery reduction of T = hi * 2^64 + lo: returns T * 2^-64 mod q, fully

@@ -35,9 +35,14 @@ struct NttTask {
   const u64* x;    // epilogue operand
   const u64* acc;  // epilogue accumulator (null: none)
   u64* out;        // epilogue destination (null: data)
+  const u64* src2; // PRO_MDRS: second prologue source (coefficient form, prime pid)
   u64 c0, c0sh, c1, c1sh;  // epilogue constants of this limb (with Shoup companions)
+  u64 c2, c2sh, c3, c3sh;  // further constants of the fused ModDown+Rescale modes
   u32 pid;         // prime of the transform
   u32 src_pid;
+  u32 perm;        // EPI_*: automorphism index applied to the store (0: none)
+  u32 pad;
+  u64 swdiff;      // SwitchModulus constant: q_pid - (q_src mod q_pid) when q_pid < q_src
 };
 
 // A set of limbs given inline as up to 8 segments of consecutive limbs
@@ -79,6 +84,9 @@ enum NttPro : int {
   // (OpenFHE NativeVector::SwitchModulus, reference src/CKKS/Rescale.cuh:47-68;
   //  fused as in src/CKKS/LimbPartitionMGPU.cu:339-373)
   PRO_SWITCH_MODULUS = 1,
+  // fused ModDown + Rescale: input = SwitchModulus(src, q_src_pid -> q_pid) * c2
+  //                                  - src2 * c3   (see EPI_MDRS)
+  PRO_MDRS = 2,
 };
 
 // Fused epilogue of the forward row pass (final store).
@@ -89,6 +97,8 @@ enum NttEpi : int {
   EPI_MODDOWN_ADD = 2,
   // out = x * q_l^-1 + ntt * QlQlInvModqlDivqlModq   Rescale, src/CKKS/NTTfusions.cuh:214-235
   EPI_RESCALE = 3,
+  // fused ModDown + Rescale: out = acc * c0 + x * c3 + ntt
+  EPI_MDRS = 4,
 };
 
 // Inline (non-list) description of a fused launch: the epilogue operand,
@@ -116,7 +126,11 @@ __device__ __forceinline__ NttTask resolve_task(const LimbSpan& s, const NttFuse
   t.x = nullptr;
   t.acc = nullptr;
   t.out = nullptr;
+  t.src2 = nullptr;
   t.c0 = t.c0sh = t.c1 = t.c1sh = 0;
+  t.c2 = t.c2sh = t.c3 = t.c3sh = 0;
+  t.perm = 0;
+  t.swdiff = 0;
   if (F.x.nseg) {
     u64* p;
     u32 q;

@@ -36,18 +36,16 @@ __device__ __forceinline__ void gs_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u6
 }
 
 // ------------------------------------------------------------- prologue/epilogue
-__device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, u64 nm) {
-  // OpenFHE NativeVector::SwitchModulus (centred lift), reference
-  // src/CKKS/Rescale.cuh:14-40 (quoted) and :47-68.
-  const u64 halfQ = om >> 1;
-  if (nm > om) {
-    if (a > halfQ) a += nm - om;
-    return a;
-  }
-  const u64 diff = nm - (om % nm);
-  if (a > halfQ) a += diff;
-  if (a >= nm) a %= nm;
-  return a;
+// OpenFHE NativeVector::SwitchModulus (centred lift) of a < om to modulus nm
+// (reference src/CKKS/Rescale.cuh:14-40 quoted, :47-68): values above om/2
+// are treated as negative.  diff = nm - (om mod nm) comes precomputed with
+// the task; the final reduction is a Shoup product by 1 (exact for any
+// input below 2^64) instead of a division.
+__device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, const PrimeC& NP, u64 diff) {
+  const u64 nm = NP.q;
+  if (nm > om) return a > (om >> 1) ? a + (nm - om) : a;
+  if (a > (om >> 1)) a += diff;
+  return shoup(a, 1, NP.one_sh, nm);
 }
 
 template <int EPI>
@@ -58,6 +56,10 @@ __device__ __forceinline__ u64 epilogue(u64 v, u32 idx, const NttTask& t, u64 q)
     const u64 a = shoup(t.x[idx], t.c0, t.c0sh, q);
     const u64 b = shoup(v, t.c1, t.c1sh, q);
     return add_mod(a, b, q);
+  } else if constexpr (EPI == EPI_MDRS) {
+    const u64 a = shoup(t.acc[idx], t.c0, t.c0sh, q);
+    const u64 b = shoup(t.x[idx], t.c3, t.c3sh, q);
+    return add_mod(add_mod(a, b, q), v, q);
   } else {
     u64 r = shoup(t.x[idx] + q - v, t.c0, t.c0sh, q);
     if constexpr (EPI == EPI_MODDOWN_ADD) {
@@ -85,8 +87,20 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
   u64 x[C::E];
   if constexpr (PRO == PRO_SWITCH_MODULUS) {
     const u64 om = T.prm[tk.src_pid].q;
+    const PrimeC NP = T.prm[pid];
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) x[k] = switch_modulus(tk.src[((r0 + C::TPL * k) << C::L2) | col], om, q);
+    for (int k = 0; k < C::E; ++k)
+      x[k] = switch_modulus(tk.src[((r0 + C::TPL * k) << C::L2) | col], om, NP, tk.swdiff);
+  } else if constexpr (PRO == PRO_MDRS) {
+    const u64 om = T.prm[tk.src_pid].q;
+    const PrimeC NP = T.prm[pid];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      const u32 e = ((r0 + C::TPL * k) << C::L2) | col;
+      const u64 a = shoup(switch_modulus(tk.src[e], om, NP, tk.swdiff), tk.c2, tk.c2sh, q);
+      const u64 b = shoup(tk.src2[e], tk.c3, tk.c3sh, q);
+      x[k] = sub_mod(a, b, q);
+    }
   } else {
     const u64* in = tk.src ? tk.src : data;
 #pragma unroll
@@ -181,7 +195,25 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
   }
   __syncthreads();
-  u64* dst = ((EPI != EPI_NONE && tk.out) ? tk.out : data) + ((u64)row << C::L2);
+  u64* base = (EPI != EPI_NONE && tk.out) ? tk.out : data;
+  if (EPI != EPI_NONE && tk.perm) {
+    // store through the automorphism X -> X^k as a scatter with k^-1:
+    // out[brev(((2 brev(j) + 1) k^-1 mod 2N) >> 1)] = v_j (reference
+    // automorph_slot, src/Rotation.cuh:14-26)
+    const u32 mask = (2u << LOGN) - 1;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      const u32 c = r0 + C::TPL * k;
+      const u32 j = (row << C::L2) | c;
+      u64 v = csub(csub(L[c], q2), q);
+      v = epilogue<EPI>(v, j, tk, q);
+      const u32 jr = __brev(j) >> (32 - LOGN);
+      const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
+      base[__brev(s) >> (32 - LOGN)] = v;
+    }
+    return;
+  }
+  u64* dst = base + ((u64)row << C::L2);
 #pragma unroll
   for (int k = 0; k < C::E; ++k) {
     const u32 c = r0 + C::TPL * k;
@@ -219,15 +251,14 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
   const u64* srow = src + ((u64)row << C::L2);
   u64* L = lds + ll * C::LDS_LINE;
   u64 x[C::E];
-#pragma unroll
-  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = srow[r0 + C::TPL * k];
-  __syncthreads();
   constexpr int EB = 1 << C::B;
 #pragma unroll
   for (int g = 0; g < C::G; ++g) {
     const u32 r1 = r0 * C::G + g;
+    // direct load of the thread's 2^B consecutive elements (no LDS staging;
+    // measured faster than a coalesced load through LDS)
 #pragma unroll
-    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[(r1 << C::B) + k];
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = srow[(r1 << C::B) + k];
 #pragma unroll
     for (int v = C::B - 1; v >= 0; --v) {
       const int u = C::A + v;
@@ -335,12 +366,15 @@ static void launch_fwd(const LimbSpan& s, const NttTables& T, int pro, int epi, 
   dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
   if (pro == PRO_SWITCH_MODULUS)
     k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else if (pro == PRO_MDRS)
+    k_ntt_fwd_col<LOGN, PRO_MDRS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
   else
     k_ntt_fwd_col<LOGN, PRO_NONE><<<g0, C0::THREADS, 0, st>>>(s, T, F);
   switch (epi) {
     case EPI_MODDOWN: k_ntt_fwd_row<LOGN, EPI_MODDOWN><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
     case EPI_MODDOWN_ADD: k_ntt_fwd_row<LOGN, EPI_MODDOWN_ADD><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
     case EPI_RESCALE: k_ntt_fwd_row<LOGN, EPI_RESCALE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    case EPI_MDRS: k_ntt_fwd_row<LOGN, EPI_MDRS><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
     default: k_ntt_fwd_row<LOGN, EPI_NONE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
   }
 }
