@@ -102,14 +102,36 @@ class Workload:
             ctx.rotate(self.r[i], l - 1, self.rot_keys[i], self.kidx[i], self.o[i], stream)
 
 
-def ntt_roofline(ctx, prm, ev, reps=20):
-    """Forward NTT (column + row pass) over 30 limbs, timed with HIP events."""
+ROOF_LIMBS = 960  # 32 limbs for each of the 30 Q primes: the size of the batched step's NTT launches
+
+
+def ntt_roofline(ctx, prm, ev, reps=10):
+    """Forward NTT (column + row pass) of ROOF_LIMBS limbs of N = 2^16,
+    grouped by prime as the batched executor issues them, timed with HIP
+    events on the stream it runs on.  Algorithmic bytes: one read and one
+    write of every limb, 2 N 8 bytes per limb.
+
+    A 256 MiB device-to-device copy runs first: its hardware FETCH/WRITE
+    counters calibrate the counter scale when this command is profiled.
+    The measured per-launch HBM traffic (from those counters) is read from
+    profiles/ntt_fwd_960x65536.json when present."""
     rng = np.random.default_rng(SEED + 1)
-    nl = 30
-    x = rand_limbs(rng, prm.q[:nl], prm.N)
-    buf = capi.DeviceBuffer.from_array(x)
-    pids = list(range(nl))
-    for _ in range(3):
+    cal_words = 32 << 20
+    ca, cb = capi.DeviceBuffer(cal_words), capi.DeviceBuffer(cal_words)
+    capi.check(capi.load().fides_copy_device(cb.ptr, ca.ptr, cal_words * 8, None), "calibration copy")
+    ctx.sync()
+    del ca, cb
+    nq = 30
+    x = rand_limbs(rng, prm.q[:nq], prm.N)
+    buf = capi.DeviceBuffer(ROOF_LIMBS * prm.N)
+    pids = [i for i in range(nq) for _ in range(ROOF_LIMBS // nq)]
+    # limb t holds residues of prime pids[t]
+    host = np.empty((ROOF_LIMBS, prm.N), np.uint64)
+    for t, p in enumerate(pids):
+        host[t] = x[p]
+    buf.upload(host)
+    del host
+    for _ in range(2):
         ctx.ntt(buf, pids)
     ctx.sync()
     e0, e1 = ev.event(), ev.event()
@@ -118,17 +140,23 @@ def ntt_roofline(ctx, prm, ev, reps=20):
         ctx.ntt(buf, pids)
     ev.record(e1)
     ms = ev.elapsed_ms(e0, e1) / reps
-    alg_bytes = 2 * prm.N * 8 * nl
+    alg_bytes = 2 * prm.N * 8 * ROOF_LIMBS
     achieved = alg_bytes / (ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "ntt_fwd_960x65536.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
     return {
-        "kernel": "ntt_fwd (k_ntt_fwd_col + k_ntt_fwd_row), 30 limbs x 2^16",
+        "kernel": "ntt_fwd = k_ntt_fwd_col + k_ntt_fwd_row, %d limbs x 2^16 (one fides_ntt call)" % ROOF_LIMBS,
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
         "launch_us": round(ms * 1e3, 2),
+        "us_per_limb": round(ms * 1e3 / ROOF_LIMBS, 4),
         "alg_bytes_per_launch": alg_bytes,
     }
 
@@ -144,7 +172,8 @@ def cpu_baseline(prm, budget_s=12.0):
     levels = [int(x) for x in rng.integers(6, prm.L + 1, 64)]
     done, t0 = 0, time.time()
     ops = []
-    for l in levels:
+    for i in range(100000):
+        l = levels[i % len(levels)]
         a = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
         b = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
         t = time.time()
@@ -160,12 +189,84 @@ def cpu_baseline(prm, budget_s=12.0):
         "unit": "key-switch ops/s (HMult+relin+Rescale)",
         "cores": cores,
         "kind": "port",
-        "sample": "%d HMult+relin+Rescale at N=2^16, limbs %s.. (levels U{5..29}), %.1f s; "
-                  "own CPU oracle (OpenMP), not OpenFHE" % (done, levels[:done][:4], el),
+        "sample": "%d HMult+relin+Rescale at N=2^16, limbs drawn U{6..30} (first %s), %.1f s of oracle time; "
+                  "own CPU oracle (OpenMP), not OpenFHE" % (done, levels[:4], el),
     }
 
 
+def summarize(trace_dir, fetch_dir=None, write_dir=None):
+    """Summarise a rocprofv3 run of this script: per-kernel time of the
+    batched step, the roofline launch's kernels, and (with the two --pmc
+    runs) the HBM bytes per roofline launch scaled by the calibration copy."""
+    import csv
+    import glob
+
+    def rows(d, pat):
+        f = glob.glob(os.path.join(d, "**", pat), recursive=True)
+        return list(csv.DictReader(open(f[0]))) if f else []
+
+    out = []
+    tr = rows(trace_dir, "*kernel_trace.csv")
+    roof_threads = 4096 * ROOF_LIMBS
+    if tr:
+        per = {}
+        for r in tr:
+            d = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            key = r["Kernel_Name"].split("(")[0]
+            per.setdefault(key, []).append(d)
+        tot = sum(sum(v) for v in per.values())
+        out.append("| kernel | calls | total ms | avg us | share |")
+        out.append("|---|---|---|---|---|")
+        for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+            out.append("| %s | %d | %.3f | %.2f | %.1f%% |" % (k, len(v), sum(v) / 1e6, sum(v) / len(v) / 1e3,
+                                                             100.0 * sum(v) / tot))
+        roof = {}
+        for r in tr:
+            if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == roof_threads and "ntt_fwd" in r["Kernel_Name"]:
+                roof.setdefault(r["Kernel_Name"].split("(")[0], []).append(
+                    int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        out.append("")
+        out.append("roofline launch (%d limbs): " % ROOF_LIMBS + ", ".join(
+            "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
+        out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in roof.values()))
+    res = {}
+    if fetch_dir and write_dir:
+        def counter(d, name):
+            vals = {}
+            cal = None
+            for r in rows(d, "*counter_collection.csv"):
+                if r["Counter_Name"] != name:
+                    continue
+                g = int(r["Grid_Size"])
+                v = float(r["Counter_Value"]) * 1024.0
+                if "copyBuffer" in r["Kernel_Name"] and (cal is None or g > cal[0]):
+                    cal = (g, v)
+                if g == roof_threads and "ntt_fwd" in r["Kernel_Name"]:
+                    vals.setdefault(r["Kernel_Name"].split("(")[0], []).append(v)
+            return cal, {k: sum(v) / len(v) for k, v in vals.items()}
+        cal_bytes = 256 << 20
+        (cf, fetch), (cw, write) = counter(fetch_dir, "FETCH_SIZE"), counter(write_dir, "WRITE_SIZE")
+        fs, ws = cal_bytes / cf[1], cal_bytes / cw[1]
+        tot = sum(fetch.values()) * fs + sum(write.values()) * ws
+        res = {"hbm_bytes_per_launch": int(tot), "fetch_scale": round(fs, 4), "write_scale": round(ws, 4),
+               "fetch_bytes": {k: int(v * fs) for k, v in fetch.items()},
+               "write_bytes": {k: int(v * ws) for k, v in write.items()},
+               "alg_bytes_per_launch": 2 * 65536 * 8 * ROOF_LIMBS}
+        out.append("")
+        out.append("HBM traffic per roofline launch: %.1f MB (fetch scale %.3f, write scale %.3f "
+                   "from a 256 MiB calibration copy); algorithmic %.1f MB" % (
+                       tot / 1e6, fs, ws, res["alg_bytes_per_launch"] / 1e6))
+    return "\n".join(out), res
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
+        a = sys.argv[2:]
+        text, res = summarize(a[0], a[1] if len(a) > 1 else None, a[2] if len(a) > 2 else None)
+        print(text)
+        if res:
+            print(json.dumps(res))
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)

@@ -197,8 +197,6 @@ struct fides_context {
   size_t arena_words = 0;
   TaskSlot ring[RING];
   int ring_next = 0;
-  NttTask* raw_tasks = nullptr;  // fides_ntt with more than 8 prime runs
-  u32 raw_tasks_cap = 0;
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -452,8 +450,6 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   H2D(c->modup, c->hmodup.data(), sizeof(ConvDesc) * c->hmodup.size());
   ALLOC(c->moddown, sizeof(ConvDesc) * c->hmoddown.size());
   H2D(c->moddown, c->hmoddown.data(), sizeof(ConvDesc) * c->hmoddown.size());
-  c->raw_tasks_cap = 4096;
-  ALLOC(c->raw_tasks, sizeof(NttTask) * c->raw_tasks_cap);
   for (auto& s : c->ring) {
     if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
       fides_context_destroy(c);
@@ -878,9 +874,11 @@ static int rs_launch(fides_context* c, char* dev, const RsLists& R, hipStream_t 
 // ======================================================================
 // raw limb operations
 // ======================================================================
-static int build_span(fides_context* c, u64* data, const uint32_t* pids, uint32_t n, hipStream_t st,
-                      LimbSpan& s) {
-  s = span0();
+// In-place NTT/INTT of n limbs with arbitrary prime ids: runs of
+// consecutive ids become inline segments; more than 8 runs go through a
+// task list uploaded via the task ring.
+static int raw_transform(fides_context* c, u64* data, const uint32_t* pids, uint32_t n, bool inverse,
+                         hipStream_t st) {
   const u32 NP = c->L + c->K;
   for (u32 t = 0; t < n; ++t)
     if (pids[t] >= NP) return fail(FIDES_ERR_ARG, "prime id out of range");
@@ -892,38 +890,40 @@ static int build_span(fides_context* c, u64* data, const uint32_t* pids, uint32_
       segs.push_back({t, 1});
   }
   if (segs.size() <= 8) {
+    LimbSpan s = span0();
     for (auto& sg : segs) span_add(s, c->limb(data, sg.first), pids[sg.first], sg.second);
+    const int rc = inverse ? fides_launch_intt(c->logN, s, c->T, st)
+                           : fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, fuse0(), st);
+    if (rc) return fail(FIDES_ERR_UNSUPPORTED, "ring dimension");
+    HIPCHK(hipGetLastError());
     return FIDES_OK;
   }
-  if (n > c->raw_tasks_cap) return fail(FIDES_ERR_ARG, "too many limbs in one call");
   std::vector<NttTask> tasks(n);
   for (u32 t = 0; t < n; ++t) tasks[t] = ntask(c->limb(data, t), pids[t]);
-  HIPCHK(hipMemcpyAsync(c->raw_tasks, tasks.data(), n * sizeof(NttTask), hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));  // the host vector goes out of scope
-  s.list = c->raw_tasks;
-  s.total = n;
+  Blob blob;
+  const size_t o = blob.add(tasks);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  if (inverse)
+    RCHK(launch_intt_list(c, at<NttTask>(dev, o), n, st));
+  else
+    RCHK(launch_ntt_list(c, at<NttTask>(dev, o), n, PRO_NONE, EPI_NONE, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
 
 extern "C" int fides_ntt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
   if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
-  LimbSpan s;
-  RCHK(build_span(c, d, pids, n, (hipStream_t)stream, s));
-  if (fides_launch_ntt(c->logN, s, c->T, PRO_NONE, EPI_NONE, fuse0(), (hipStream_t)stream))
-    return fail(FIDES_ERR_UNSUPPORTED, "ring dimension");
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
+  return raw_transform(c, d, pids, n, false, (hipStream_t)stream);
 }
 
 extern "C" int fides_intt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
   if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
-  LimbSpan s;
-  RCHK(build_span(c, d, pids, n, (hipStream_t)stream, s));
-  if (fides_launch_intt(c->logN, s, c->T, (hipStream_t)stream)) return fail(FIDES_ERR_UNSUPPORTED, "ring dimension");
-  HIPCHK(hipGetLastError());
-  return FIDES_OK;
+  return raw_transform(c, d, pids, n, true, (hipStream_t)stream);
 }
 
 extern "C" int fides_modup(fides_context* c, const uint64_t* d, uint32_t l, uint32_t j, uint64_t* out,
