@@ -70,6 +70,16 @@ SIGNATURES = {
                                  ctypes.POINTER(_vp), _vp]),
     "fides_rotate_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _u32p,
                                   ctypes.POINTER(_vp), _vp]),
+    "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
+                               ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_drop_limbs_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_add_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, _int,
+                               ctypes.POINTER(_vp), _vp]),
+    "fides_mul_const_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
+                                     ctypes.POINTER(_vp), _vp]),
+    "fides_add_const_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
+                                     ctypes.POINTER(_vp), _vp]),
 }
 
 _lib = None
@@ -235,6 +245,76 @@ class Context:
         kx = np.ascontiguousarray(kidx, np.uint32)
         check(load().fides_rotate_batch(self.h, n, I, lv.ctypes.data_as(_u32p), Kh, kx.ctypes.data_as(_u32p), O,
                                         stream), "fides_rotate_batch")
+
+    def rescale_batch(self, inp, levels, out, stream=None):
+        n = len(levels)
+        I = (_vp * n)(*[_ptr(x) for x in inp])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_rescale_batch(self.h, n, I, lv.ctypes.data_as(_u32p), O, stream), "fides_rescale_batch")
+
+    def add_batch(self, a, b, levels, out, subtract=False, stream=None):
+        n = len(levels)
+        A = (_vp * n)(*[_ptr(x) for x in a])
+        B = (_vp * n)(*[_ptr(x) for x in b])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_add_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), int(bool(subtract)), O, stream),
+              "fides_add_batch")
+
+    def lin_batch(self, a, b, levels, mul, add, sub, out, stream=None):
+        """out[i] = a[i] * mul[i] (+/-) b[i] + add[i]; b[i], mul[i], add[i] may be None.
+        mul/add are Python integers (any sign or size)."""
+        n = len(levels)
+        A = (_vp * n)(*[_ptr(x) for x in a])
+        B = (_vp * n)(*[None if x is None else _ptr(x) for x in b])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        keep = []
+
+        def res(vals):
+            arr = (_u64p * n)()
+            for i, (c, l) in enumerate(zip(vals, levels)):
+                if c is not None:
+                    r = np.ascontiguousarray([int(c) % self.q[j] for j in range(min(l, self.L))], np.uint64)
+                    keep.append(r)
+                    arr[i] = r.ctypes.data_as(_u64p)
+            return arr
+
+        M, C = res(mul), res(add)
+        S = np.ascontiguousarray([1 if x else 0 for x in sub], np.uint32)
+        check(load().fides_lin_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), M, C, S.ctypes.data_as(_u32p), O,
+                                     stream), "fides_lin_batch")
+
+    def drop_limbs_batch(self, inp, l_in, l_out, out, stream=None):
+        n = len(l_in)
+        I = (_vp * n)(*[_ptr(x) for x in inp])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        li = np.ascontiguousarray(l_in, np.uint32)
+        lo = np.ascontiguousarray(l_out, np.uint32)
+        check(load().fides_drop_limbs_batch(self.h, n, I, li.ctypes.data_as(_u32p), lo.ctypes.data_as(_u32p), O,
+                                            stream), "fides_drop_limbs_batch")
+
+    def _const_call(self, fn, a, levels, consts, out, stream, what):
+        n = len(levels)
+        A = (_vp * n)(*[_ptr(x) for x in a])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        res = [np.ascontiguousarray([int(c) % self.q[j] for j in range(min(l, self.L))], np.uint64)
+               for c, l in zip(consts, levels)]
+        R = (_u64p * n)(*[r.ctypes.data_as(_u64p) for r in res])
+        check(fn(self.h, n, A, lv.ctypes.data_as(_u32p), R, O, stream), what)
+
+    def mul_const_batch(self, a, levels, consts, out, stream=None):
+        """out[i] = a[i] * consts[i] for Python integers (any sign or size)."""
+        self._const_call(load().fides_mul_const_batch, a, levels, consts, out, stream, "fides_mul_const_batch")
+
+    def add_const_batch(self, a, levels, consts, out, stream=None):
+        """out[i] = a[i] + consts[i] (constant polynomial) for Python integers."""
+        self._const_call(load().fides_add_const_batch, a, levels, consts, out, stream, "fides_add_const_batch")
+
+    def copy(self, dst, src, nwords, stream=None):
+        check(load().fides_copy_device(_ptr(dst), _ptr(src), int(nwords) * 8, stream), "fides_copy_device")
 
     def sync(self, stream=None):
         check(load().fides_stream_sync(stream), "sync")

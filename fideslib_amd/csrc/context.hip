@@ -1242,3 +1242,167 @@ extern "C" int fides_rotate_batch(fides_context* c, uint32_t n, const uint64_t* 
   if (n == 0) return FIDES_OK;
   return rotate_batch(c, n, in, l, keys, k, out, (hipStream_t)stream);
 }
+
+// ======================================================================
+// element-wise ciphertext arithmetic and batched rescale
+// ======================================================================
+static int run_lin(fides_context* c, const std::vector<LinTask>& tasks, hipStream_t st) {
+  if (tasks.empty()) return FIDES_OK;
+  Blob blob;
+  const size_t o = blob.add(tasks);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  for (u32 x = 0; x < tasks.size(); x += 65535)
+    fides_launch_lin_batch(at<LinTask>(dev, o) + x, std::min<u32>(65535, (u32)tasks.size() - x), c->prm, c->logN, st);
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+static LinTask lin0(const u64* a, u64* out, u32 pid) {
+  LinTask t;
+  memset(&t, 0, sizeof(t));
+  t.a = a;
+  t.out = out;
+  t.pid = pid;
+  return t;
+}
+
+extern "C" int fides_add_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
+                               const uint32_t* l, int subtract, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!a[i] || !b[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 j = 0; j < l[i]; ++j) {
+        const size_t off = (size_t)p * l[i] + j;
+        LinTask t = lin0(c->limb(a[i], off), c->limb(out[i], off), j);
+        t.b = c->limb(b[i], off);
+        t.flags = LIN_B | (subtract ? LIN_SUB : 0);
+        tasks.push_back(t);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_mul_const_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint32_t* l,
+                                     const uint64_t* const* res, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!a || !l || !res || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!a[i] || !res[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    for (u32 j = 0; j < l[i]; ++j)
+      if (res[i][j] >= c->primes[j]) return fail(FIDES_ERR_ARG, "constant residue not reduced");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 j = 0; j < l[i]; ++j) {
+        const size_t off = (size_t)p * l[i] + j;
+        LinTask t = lin0(c->limb(a[i], off), c->limb(out[i], off), j);
+        t.m = res[i][j];
+        t.msh = shoup_of(res[i][j], c->primes[j]);
+        t.flags = LIN_MUL;
+        tasks.push_back(t);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_add_const_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint32_t* l,
+                                     const uint64_t* const* res, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!a || !l || !res || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!a[i] || !res[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    for (u32 j = 0; j < l[i]; ++j)
+      if (res[i][j] >= c->primes[j]) return fail(FIDES_ERR_ARG, "constant residue not reduced");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 j = 0; j < l[i]; ++j) {
+        const size_t off = (size_t)p * l[i] + j;
+        LinTask t = lin0(c->limb(a[i], off), c->limb(out[i], off), j);
+        // a constant polynomial evaluates to the same value at every point
+        t.add = p == 0 ? res[i][j] : 0;
+        tasks.push_back(t);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_lin_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
+                               const uint32_t* l, const uint64_t* const* mul_res, const uint64_t* const* add_res,
+                               const uint32_t* sub, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!a || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!a[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    const uint64_t* m = mul_res ? mul_res[i] : nullptr;
+    const uint64_t* k = add_res ? add_res[i] : nullptr;
+    const uint64_t* bb = b ? b[i] : nullptr;
+    for (u32 j = 0; j < l[i]; ++j)
+      if ((m && m[j] >= c->primes[j]) || (k && k[j] >= c->primes[j]))
+        return fail(FIDES_ERR_ARG, "constant residue not reduced");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 j = 0; j < l[i]; ++j) {
+        const size_t off = (size_t)p * l[i] + j;
+        LinTask t = lin0(c->limb(a[i], off), c->limb(out[i], off), j);
+        if (m) {
+          t.m = m[j];
+          t.msh = shoup_of(m[j], c->primes[j]);
+          t.flags |= LIN_MUL;
+        }
+        if (bb) {
+          t.b = c->limb(bb, off);
+          t.flags |= LIN_B | (sub && sub[i] ? LIN_SUB : 0);
+        }
+        t.add = (k && p == 0) ? k[j] : 0;
+        tasks.push_back(t);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_drop_limbs_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l_in,
+                                      const uint32_t* l_out, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!in || !l_in || !l_out || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l_in[i] == 0 || l_in[i] > c->L || l_out[i] == 0 || l_out[i] > l_in[i])
+      return fail(FIDES_ERR_ARG, "need 1 <= l_out <= l_in <= L");
+    if (in[i] == out[i] && l_out[i] != l_in[i]) return fail(FIDES_ERR_ARG, "limb drop must not run in place");
+    if (in[i] == out[i]) continue;
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 j = 0; j < l_out[i]; ++j)
+        tasks.push_back(lin0(c->limb(in[i], (size_t)p * l_in[i] + j), c->limb(out[i], (size_t)p * l_out[i] + j), j));
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_rescale_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
+                                   uint64_t* const* out, void* stream) {
+  if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (n == 0) return FIDES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<RsJob> jobs;
+  for (u32 i = 0; i < n; ++i) {
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] < 2 || l[i] > c->L) return fail(FIDES_ERR_ARG, "rescale needs 2 <= l <= L");
+    if (in[i] == out[i]) return fail(FIDES_ERR_ARG, "rescale output must not alias its input");
+    jobs.push_back(RsJob{in[i], l[i], out[i]});
+  }
+  RCHK(ensure_arena(c, 2 * (size_t)c->N * n, st));
+  Blob blob;
+  RsLists R;
+  rs_build(c, jobs, c->arena, blob, R);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(rs_launch(c, dev, R, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}

@@ -62,6 +62,18 @@ struct AxpyTask {
   u32 pad;
 };
 
+// One limb of out = (a * m or a) (+ or - b) + add  (mod q_pid).
+struct LinTask {
+  const u64* a;
+  const u64* b;   // used when flags & LIN_B
+  u64* out;
+  u64 m, msh;     // used when flags & LIN_MUL
+  u64 add;        // added constant residue (< q)
+  u32 pid;
+  u32 flags;
+};
+constexpr u32 LIN_B = 1, LIN_SUB = 2, LIN_MUL = 4;
+
 // One limb of an evaluation-domain automorphism.
 struct AutoTask {
   const u64* in;
@@ -75,6 +87,7 @@ void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm,
                             hipStream_t st);
 void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
+void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,

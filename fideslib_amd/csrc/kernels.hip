@@ -133,7 +133,27 @@ __global__ __launch_bounds__(256) void k_axpy_batch(const AxpyTask* __restrict__
   t.out[k] = add_mod(t.a[k], shoup(t.b[k], t.c, t.csh, q), q);
 }
 
+// ---------------------------------------------------------------- linear combination
+// Element-wise ciphertext arithmetic: add/sub, multiplication by an integer
+// constant, addition of a constant (reference AddSub.cu:10-123 add_/sub_/
+// scalar_ kernels; ElemenwiseBatchKernels.cu:1109-1360).
+__global__ __launch_bounds__(256) void k_lin_batch(const LinTask* __restrict__ tasks, const PrimeC* __restrict__ prm,
+                                                   u32 logN) {
+  const LinTask t = tasks[blockIdx.y];
+  const u32 k = blockIdx.x * 256u + threadIdx.x;
+  const u64 q = prm[t.pid].q;
+  u64 v = t.a[k];
+  if (t.flags & LIN_MUL) v = shoup(v, t.m, t.msh, q);
+  if (t.flags & LIN_B) v = (t.flags & LIN_SUB) ? sub_mod(v, t.b[k], q) : add_mod(v, t.b[k], q);
+  t.out[k] = add_mod(v, t.add, q);
+}
+
 // ---------------------------------------------------------------- launchers
+void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
+  if (ntasks == 0) return;
+  k_lin_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
+}
+
 void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
   if (ntasks == 0) return;
   k_axpy_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);

@@ -142,3 +142,9 @@ def config2() -> ParamSet:
 def config3(K: int | None = None) -> ParamSet:
     """N=2^16, 30 Q limbs (60 + 29x59 bits), dnum 3, scale 2^59."""
     return ckks_params(16, 60, 59, 29, dnum=3, K=K, name="cfg3_N2^16_L29_dnum3")
+
+
+def serve_small() -> ParamSet:
+    """N=2^12, one 60-bit + eleven 50-bit Q primes, dnum 3: deep enough for
+    the degree-15 Chebyshev request at small N (request-loop tests)."""
+    return ckks_params(12, 60, 50, 11, dnum=3, name="serve_N2^12_L12_dnum3")

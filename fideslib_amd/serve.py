@@ -1,0 +1,661 @@
+"""Continuous-batching request loop over the batched C ABI.
+
+Requests are small CKKS programs (a degree-15 Chebyshev series, a
+rotate-and-sum over 2^k slots, ...) compiled into a list of register ops.
+The loop admits requests as they arrive, while the HBM arena has room for
+their peak footprint, and runs them together: every step gathers the ops of
+all admitted requests whose inputs exist and issues ONE batched launch per op
+kind (all HMult+relin+Rescale of the step in one fides_hmult_batch, all
+rotations in one fides_rotate_batch, all element-wise ops in one
+fides_lin_batch, ...).  Requests that arrive while others are on the GPU join
+at the next step, so their work shares launches with requests already in
+flight.
+
+Keys stay resident for the life of the server.  Ciphertexts live in one
+device arena; each admitted request owns a block of it sized by a liveness
+analysis of its program (registers whose values are dead share storage).
+
+Everything runs on one HIP stream, so device order equals issue order.  The
+host keeps at most two steps in flight: it builds step s+1 while the device
+runs step s, and retires requests when the event after their last step has
+completed.
+
+Exact arithmetic makes each result a function of its own inputs only: the
+same request gives the same bits alone and in any mix.  The scheduler's
+ordering rules (below, `_compile`) are what keep that true when storage is
+reused.
+
+There is no serving loop in the reference; the operations it batches are the
+reference's CryptoContext operations (api/CryptoContext.hpp:116-186).
+"""
+from __future__ import annotations
+
+import bisect
+import ctypes
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# op kinds
+HMUL = "hmul"          # HMult + relin + Rescale (levels l -> l-1)
+HMUL_NR = "hmul_nr"    # HMult + relin, no rescale
+ROT = "rot"            # key switch + automorphism
+LIN = "lin"            # out = a*m (+/-) b + c
+RESCALE = "rescale"
+DROP = "drop"          # keep the first l_out limbs
+KIND_ORDER = (HMUL, HMUL_NR, ROT, LIN, RESCALE, DROP)
+KEYSWITCH_KINDS = (HMUL, HMUL_NR, ROT)
+
+
+class ServeError(RuntimeError):
+    pass
+
+
+# ----------------------------------------------------------------- programs
+@dataclass
+class Reg:
+    id: int
+    level: int            # number of Q limbs
+    scale: float          # nominal scaling factor of the encoded value
+
+    @property
+    def words(self) -> int:
+        return 2 * self.level
+
+
+@dataclass
+class Op:
+    kind: str
+    dst: Reg
+    srcs: tuple
+    level: int            # level of the inputs
+    arg: tuple = ()       # ROT: (autoIndex,), LIN: (mul, add, sub), DROP: (l_out,)
+
+
+class Program:
+    """A straight-line CKKS program over ciphertext registers.
+
+    Levels are tracked in Q limbs; binary ops align their operands by
+    dropping limbs of the higher one (as Ciphertext::add / mult do,
+    src/CKKS/Ciphertext.cpp:229-233, 484-485).  Scales are tracked
+    nominally (FIXEDMANUAL semantics: the rescaled scale is scale^2 / q_top).
+    """
+
+    def __init__(self, q, logN: int, scale_bits: int, name: str = ""):
+        self.q = [int(x) for x in q]
+        self.logN = logN
+        self.N = 1 << logN
+        self.delta = float(2 ** scale_bits)
+        self.name = name
+        self.regs: list[Reg] = []
+        self.ops: list[Op] = []
+        self.inputs: list[Reg] = []
+        self.output: Reg | None = None
+
+    # -- registers
+    def _new(self, level: int, scale: float) -> Reg:
+        if level < 1:
+            raise ServeError("program runs out of levels")
+        r = Reg(len(self.regs), level, scale)
+        self.regs.append(r)
+        return r
+
+    def input(self, level: int, scale: float | None = None) -> Reg:
+        if not 1 <= level <= len(self.q):
+            raise ServeError("input level out of range")
+        r = self._new(level, self.delta if scale is None else scale)
+        self.inputs.append(r)
+        return r
+
+    def _emit(self, kind, dst, srcs, level, arg=()):
+        self.ops.append(Op(kind, dst, tuple(srcs), level, tuple(arg)))
+        return dst
+
+    def drop(self, a: Reg, level: int) -> Reg:
+        if level == a.level:
+            return a
+        if not 1 <= level < a.level:
+            raise ServeError("drop needs 1 <= level < current level")
+        return self._emit(DROP, self._new(level, a.scale), [a], a.level, (level,))
+
+    def _align(self, a: Reg, b: Reg):
+        l = min(a.level, b.level)
+        return self.drop(a, l), self.drop(b, l)
+
+    # -- key-switching ops
+    def hmul(self, a: Reg, b: Reg, rescale: bool = True) -> Reg:
+        a, b = self._align(a, b)
+        l = a.level
+        if rescale:
+            if l < 2:
+                raise ServeError("HMult+Rescale needs 2 limbs")
+            out = self._new(l - 1, a.scale * b.scale / self.q[l - 1])
+            return self._emit(HMUL, out, [a, b], l)
+        return self._emit(HMUL_NR, self._new(l, a.scale * b.scale), [a, b], l)
+
+    def rotate(self, a: Reg, r: int) -> Reg:
+        k = pow(5, r % (self.N // 2), 2 * self.N)
+        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (k,))
+
+    def conjugate(self, a: Reg) -> Reg:
+        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (2 * self.N - 1,))
+
+    # -- element-wise ops
+    def lin(self, a: Reg, mul: int | None = None, b: Reg | None = None, sub: bool = False,
+            add: int | None = None, scale: float | None = None) -> Reg:
+        """out = a*mul (+/-) b + add for integers mul/add (exact)."""
+        if b is not None:
+            a, b = self._align(a, b)
+        out = self._new(a.level, a.scale if scale is None else scale)
+        srcs = [a] if b is None else [a, b]
+        return self._emit(LIN, out, srcs, a.level, (mul, add, bool(sub)))
+
+    def add(self, a: Reg, b: Reg) -> Reg:
+        return self.lin(a, b=b)
+
+    def sub(self, a: Reg, b: Reg) -> Reg:
+        return self.lin(a, b=b, sub=True)
+
+    def add_real(self, a: Reg, x: float) -> Reg:
+        return self.lin(a, add=int(round(x * a.scale)))
+
+    def mul_real(self, a: Reg, x: float) -> Reg:
+        """a * x for a real constant: integer multiply by round(x*delta), then Rescale."""
+        t = self.lin(a, mul=int(round(x * self.delta)), scale=a.scale * self.delta)
+        return self.rescale(t)
+
+    def rescale(self, a: Reg) -> Reg:
+        if a.level < 2:
+            raise ServeError("Rescale needs 2 limbs")
+        return self._emit(RESCALE, self._new(a.level - 1, a.scale / self.q[a.level - 1]), [a], a.level)
+
+    def sum_tree(self, regs: list) -> Reg:
+        regs = list(regs)
+        while len(regs) > 1:
+            nxt = [self.add(regs[i], regs[i + 1]) for i in range(0, len(regs) - 1, 2)]
+            if len(regs) % 2:
+                nxt.append(regs[-1])
+            regs = nxt
+        return regs[0]
+
+    def set_output(self, r: Reg) -> None:
+        self.output = r
+
+    # -- statistics
+    def count(self, kinds) -> int:
+        return sum(1 for o in self.ops if o.kind in kinds)
+
+
+# ----------------------------------------------------------------- request programs
+def chebyshev_coefficients(f, degree: int, lo: float, hi: float) -> np.ndarray:
+    """Chebyshev interpolant of f on [lo, hi] (coefficients of T_k((2x-lo-hi)/(hi-lo)))."""
+    k = np.arange(degree + 1)
+    y = np.cos(np.pi * (k + 0.5) / (degree + 1))
+    fx = np.array([f(0.5 * (hi - lo) * t + 0.5 * (hi + lo)) for t in y])
+    c = np.array([2.0 / (degree + 1) * np.sum(fx * np.cos(np.pi * j * (k + 0.5) / (degree + 1)))
+                  for j in range(degree + 1)])
+    c[0] /= 2
+    return c
+
+
+def chebyshev_depth(degree: int) -> int:
+    """Levels consumed by `chebyshev_program` (input map, T_k tree, final combination)."""
+    return 1 + max(1, math.ceil(math.log2(degree))) + 1
+
+
+def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
+    """Evaluate sum_k c_k T_k(y), y = (2x - lo - hi)/(hi - lo), on one ciphertext.
+
+    T_k by the product rule T_{m+n} = 2 T_m T_n - T_{|m-n|} with m the largest
+    power of two below m+n, so T_1..T_{2^j} are ready after j HMult levels;
+    each T costs one HMult+Rescale and one fused element-wise op
+    (2*prod - T_{|m-n|}, or 2*prod - 1).  The combination multiplies every
+    T_k by round(c_k * delta), adds them in a tree, adds c_0 and rescales once.
+    (The reference evaluates Chebyshev series with Paterson-Stockmeyer,
+    src/PolyApprox.cu; this is a shallower linear-depth-per-level variant.)
+    """
+    P = Program(prm.q, prm.logN, prm.scale_bits, "chebyshev%d" % (len(coeffs) - 1))
+    x = P.input(level)
+    deg = len(coeffs) - 1
+    # y = a*x + b with a = 2/(hi-lo), b = -(hi+lo)/(hi-lo)
+    a = 2.0 / (hi - lo)
+    b = -(hi + lo) / (hi - lo)
+    y = P.mul_real(x, a)
+    if b != 0.0:
+        y = P.add_real(y, b)
+    T = {1: y}
+    m = 1
+    while m < deg:
+        for k in range(m + 1, min(2 * m, deg) + 1):
+            n = k - m
+            prod = P.hmul(T[m], T[n])
+            if n == m:  # T_2m = 2 T_m^2 - 1
+                T[k] = P.lin(prod, mul=2, add=-int(round(prod.scale)))
+            else:       # T_{m+n} = 2 T_m T_n - T_{m-n}
+                T[k] = P.lin(prod, mul=2, b=T[m - n], sub=True)
+        m *= 2
+    low = min(T[k].level for k in T)
+    terms = []
+    for k in range(1, deg + 1):
+        if coeffs[k] == 0.0:
+            continue
+        t = P.drop(T[k], low)
+        terms.append(P.lin(t, mul=int(round(coeffs[k] * P.delta)), scale=t.scale * P.delta))
+    s = P.sum_tree(terms)
+    s = P.lin(s, add=int(round(coeffs[0] * s.scale)))
+    P.set_output(P.rescale(s))
+    return P
+
+
+def rotsum_program(prm, level: int, k: int) -> Program:
+    """Sum of 2^k consecutive slots into every slot: k rounds of x += rot(x, 2^i)
+    (the reference's Accumulate, src/CKKS/AccumulateBroadcast.cu)."""
+    P = Program(prm.q, prm.logN, prm.scale_bits, "rotsum%d" % k)
+    x = P.input(level)
+    for i in range(k):
+        x = P.add(x, P.rotate(x, 1 << i))
+    P.set_output(x)
+    return P
+
+
+def rotation_indices_rotsum(kmax: int) -> list:
+    return [1 << i for i in range(kmax)]
+
+
+# ----------------------------------------------------------------- compile: storage and ordering
+@dataclass
+class Compiled:
+    """Storage and ordering of one program.
+
+    slot_of[reg] -> slot; slot_off/slot_size in limbs (units of N words);
+    block = limbs the request owns.  deps[i] = ops that must be issued in an
+    EARLIER step than op i: the producers of its inputs, and every op that
+    wrote or read the previous value held in the slot op i writes (write
+    after read / write after write on reused storage).  Ops of one step are
+    issued grouped by kind, not in program order, so an op may never share a
+    step with an op it depends on.
+    """
+    slot_of: dict
+    slot_off: list
+    slot_size: list
+    block: int
+    deps: list
+    dependents: list
+
+
+def _compile(P: Program) -> Compiled:
+    if P.output is None:
+        raise ServeError("program has no output")
+    nops = len(P.ops)
+    last_use = {r.id: -1 for r in P.inputs}
+    for i, o in enumerate(P.ops):
+        last_use.setdefault(o.dst.id, i)      # a value nobody reads dies at its definition
+        for s_ in o.srcs:
+            last_use[s_.id] = max(last_use.get(s_.id, -1), i)
+    last_use[P.output.id] = nops              # kept until the request retires
+    dies_after = {}
+    for rid, u in last_use.items():
+        dies_after.setdefault(u, []).append(rid)
+
+    slot_size, slot_of, hist, free = [], {}, [], []
+
+    def alloc(reg):
+        need = reg.words
+        best = None
+        for idx, (sz, _) in enumerate(free):
+            if sz >= need and (best is None or sz < free[best][0]):
+                best = idx
+        if best is not None:
+            _, sl = free.pop(best)
+        else:
+            sl = len(slot_size)
+            slot_size.append(need)
+            hist.append([])
+        slot_of[reg.id] = sl
+        return sl
+
+    def release(u):
+        for rid in dies_after.get(u, ()):
+            sl = slot_of[rid]
+            free.append((slot_size[sl], sl))
+
+    for r in P.inputs:
+        alloc(r)
+    release(-1)
+    producer = {}
+    deps = []
+    for i, o in enumerate(P.ops):
+        d = set()
+        for s_ in o.srcs:
+            if s_.id in producer:
+                d.add(producer[s_.id])
+            elif s_.id not in slot_of:
+                raise ServeError("register %d read before it is written" % s_.id)
+        sl = alloc(o.dst)
+        d.update(hist[sl])
+        hist[sl] = [i]
+        for s_ in o.srcs:
+            hist[slot_of[s_.id]].append(i)
+        producer[o.dst.id] = i
+        d.discard(i)
+        deps.append(sorted(d))
+        release(i)
+    dependents = [[] for _ in range(nops)]
+    for i, d in enumerate(deps):
+        for j in d:
+            dependents[j].append(i)
+    off, acc = [], 0
+    for sz in slot_size:
+        off.append(acc)
+        acc += sz
+    return Compiled(slot_of, off, slot_size, acc, deps, dependents)
+
+
+# ----------------------------------------------------------------- arena
+class Arena:
+    """First-fit allocator over one device allocation (units: words)."""
+
+    def __init__(self, base_ptr: int, words: int):
+        self.base = int(base_ptr)
+        self.words = int(words)
+        self.free = [(0, self.words)]  # sorted (offset, size)
+        self.used = 0
+        self.peak = 0
+
+    def alloc(self, words: int):
+        for i, (o, s) in enumerate(self.free):
+            if s >= words:
+                if s == words:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (o + words, s - words)
+                self.used += words
+                self.peak = max(self.peak, self.used)
+                return o
+        return None
+
+    def release(self, off: int, words: int) -> None:
+        i = bisect.bisect_left(self.free, (off, 0))
+        self.free.insert(i, (off, words))
+        # coalesce with neighbours
+        if i + 1 < len(self.free) and off + words == self.free[i + 1][0]:
+            self.free[i] = (off, words + self.free[i + 1][1])
+            self.free.pop(i + 1)
+        if i > 0 and self.free[i - 1][0] + self.free[i - 1][1] == self.free[i][0]:
+            self.free[i - 1] = (self.free[i - 1][0], self.free[i - 1][1] + self.free[i][1])
+            self.free.pop(i)
+        self.used -= words
+
+
+# ----------------------------------------------------------------- backends
+class GpuBackend:
+    """Issues batched C ABI calls on the context's stream (the default stream)."""
+
+    def __init__(self, ctx, relin_key, rotation_keys: dict, arena_bytes: int):
+        from . import capi
+        self.capi = capi
+        self.ctx = ctx
+        self.N = ctx.N
+        self.rlk = relin_key
+        self.keys = dict(rotation_keys)          # autoIndex -> capi.Key
+        self.buf = capi.DeviceBuffer(arena_bytes // 8)
+        self.base = self.buf.ptr
+        self.words = self.buf.nwords
+        self.hip = _Hip()
+
+    def ptr(self, word_off: int) -> int:
+        return self.base + 8 * word_off
+
+    def key_for(self, k: int):
+        if k not in self.keys:
+            raise ServeError("no rotation key for automorphism index %d" % k)
+        return self.keys[k]
+
+    def hmul(self, a, b, levels, out, rescale):
+        self.ctx.hmult_batch(a, b, levels, self.rlk, out, rescale=rescale)
+
+    def rot(self, a, levels, kidx, out):
+        self.ctx.rotate_batch(a, levels, [self.key_for(k) for k in kidx], kidx, out)
+
+    def lin(self, a, b, levels, mul, add, sub, out):
+        self.ctx.lin_batch(a, b, levels, mul, add, sub, out)
+
+    def rescale(self, a, levels, out):
+        self.ctx.rescale_batch(a, levels, out)
+
+    def drop(self, a, l_in, l_out, out):
+        self.ctx.drop_limbs_batch(a, l_in, l_out, out)
+
+    def upload(self, ptr, host):
+        arr = np.ascontiguousarray(host, np.uint64)
+        self.capi.check(self.capi.load().fides_copy_to_device(ptr, arr.ctypes.data, arr.size * 8, None), "upload")
+
+    def copy(self, dst, src, words):
+        self.capi.check(self.capi.load().fides_copy_device(dst, src, words * 8, None), "copy")
+
+    def download(self, ptr, words) -> np.ndarray:
+        out = np.empty(words, np.uint64)
+        self.capi.check(self.capi.load().fides_copy_to_host(out.ctypes.data, ptr, words * 8, None), "download")
+        return out
+
+    def record(self):
+        return self.hip.record()
+
+    def wait(self, ev) -> None:
+        self.hip.wait(ev)
+
+
+class _Hip:
+    """HIP events on the default stream (plumbing for the step pipeline)."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL("libamdhip64.so")
+        self.lib.hipEventCreateWithFlags.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint]
+        self.lib.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.lib.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.pool = []
+
+    def record(self):
+        if self.pool:
+            e = self.pool.pop()
+        else:
+            e = ctypes.c_void_p()
+            if self.lib.hipEventCreateWithFlags(ctypes.byref(e), 2) != 0:  # hipEventDisableTiming
+                raise ServeError("hipEventCreate failed")
+        if self.lib.hipEventRecord(e, None) != 0:
+            raise ServeError("hipEventRecord failed")
+        return e
+
+    def wait(self, e) -> None:
+        if self.lib.hipEventSynchronize(e) != 0:
+            raise ServeError("hipEventSynchronize failed")
+        self.pool.append(e)
+
+
+# ----------------------------------------------------------------- requests and the loop
+@dataclass
+class Request:
+    rid: int
+    program: Program
+    arrival: float                      # seconds after the loop starts
+    inputs: list = field(default_factory=list)   # host arrays [2][l][N] or ("dev", ptr) sources
+    keep_result: bool = False
+    # filled by the server
+    admitted_at: float = -1.0
+    done_at: float = -1.0
+    result: np.ndarray | None = None
+    status: str = "pending"
+    compiled: Compiled | None = None
+
+    @property
+    def latency(self) -> float:
+        return self.done_at - self.arrival
+
+
+class _Active:
+    __slots__ = ("req", "comp", "off", "remaining", "ready", "left", "last_step", "N")
+
+    def __init__(self, req, comp, off, N):
+        self.req, self.comp, self.off, self.N = req, comp, off, N
+        self.remaining = [len(d) for d in comp.deps]
+        self.ready = [i for i, n in enumerate(self.remaining) if n == 0]
+        self.left = len(self.remaining)
+        self.last_step = -1
+
+    def reg_ptr_words(self, reg) -> int:
+        return (self.off + self.comp.slot_off[self.comp.slot_of[reg.id]]) * self.N
+
+
+class Server:
+    """Continuous-batching loop.  `backend` provides the batched ops, device
+    memory (`base`, `words`, `ptr`) and events."""
+
+    def __init__(self, backend, N: int, max_inflight_steps: int = 2):
+        self.be = backend
+        self.N = N
+        self.arena = Arena(0, backend.words // N)   # units: limbs of N words
+        self.max_inflight = max(1, max_inflight_steps)
+        self.stats = dict(steps=0, launches=0, ops=0, keyswitch_ops=0, rejected=0)
+
+    def footprint_bytes(self, program: Program) -> int:
+        return _compile(program).block * self.N * 8
+
+    # -- one scheduling step: every ready op of every active request, one launch per kind
+    def _step(self, active, step_no):
+        groups = {k: [] for k in KIND_ORDER}
+        for a in active:
+            if not a.ready:
+                continue
+            ops = a.req.program.ops
+            issued, a.ready = a.ready, []
+            for i in issued:
+                groups[ops[i].kind].append((a, ops[i]))
+            # dependents become ready for the NEXT step
+            for i in issued:
+                for j in a.comp.dependents[i]:
+                    a.remaining[j] -= 1
+                    if a.remaining[j] == 0:
+                        a.ready.append(j)
+            a.left -= len(issued)
+            a.last_step = step_no
+        be = self.be
+        nops = 0
+        for kind in KIND_ORDER:
+            g = groups[kind]
+            if not g:
+                continue
+            nops += len(g)
+            ptr = lambda a, r: be.ptr(a.reg_ptr_words(r))
+            s0 = [ptr(a, o.srcs[0]) for a, o in g]
+            dst = [ptr(a, o.dst) for a, o in g]
+            lv = [o.level for _, o in g]
+            if kind in (HMUL, HMUL_NR):
+                be.hmul(s0, [ptr(a, o.srcs[1]) for a, o in g], lv, dst, kind == HMUL)
+            elif kind == ROT:
+                be.rot(s0, lv, [o.arg[0] for _, o in g], dst)
+            elif kind == LIN:
+                be.lin(s0, [ptr(a, o.srcs[1]) if len(o.srcs) > 1 else None for a, o in g], lv,
+                       [o.arg[0] for _, o in g], [o.arg[1] for _, o in g], [o.arg[2] for _, o in g], dst)
+            elif kind == RESCALE:
+                be.rescale(s0, lv, dst)
+            elif kind == DROP:
+                be.drop(s0, lv, [o.arg[0] for _, o in g], dst)
+            self.stats["launches"] += 1
+            if kind in KEYSWITCH_KINDS:
+                self.stats["keyswitch_ops"] += len(g)
+        self.stats["ops"] += nops
+        return nops
+
+    def _admit(self, req: Request, now: float):
+        comp = req.compiled
+        off = self.arena.alloc(comp.block)
+        if off is None:
+            return None
+        a = _Active(req, comp, off, self.N)
+        for r, src in zip(req.program.inputs, req.inputs):
+            p = self.be.ptr(a.reg_ptr_words(r))
+            if isinstance(src, tuple) and src[0] == "dev":
+                self.be.copy(p, src[1], r.words * self.N)
+            else:
+                self.be.upload(p, src)
+        req.admitted_at = now
+        req.status = "running"
+        return a
+
+    def _retire(self, a: _Active, now: float):
+        req = a.req
+        if req.keep_result:
+            out = req.program.output
+            req.result = self.be.download(self.be.ptr(a.reg_ptr_words(out)), out.words * self.N).reshape(
+                2, out.level, self.N)
+        self.arena.release(a.off, a.comp.block)
+        req.done_at = now
+        req.status = "done"
+
+    def run(self, requests, realtime: bool = True, clock=time.perf_counter):
+        """Serve `requests` (arrival times relative to the start).  With
+        realtime=False every request is available at once (closed batch)."""
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        for r in reqs:
+            r.compiled = _compile(r.program)
+            if r.compiled.block > self.arena.words:
+                r.status = "rejected"
+                self.stats["rejected"] += 1
+        pending = deque(r for r in reqs if r.status != "rejected")
+        queue = deque()
+        active = []
+        inflight = deque()   # (event, step_no)
+        step_no = 0
+        t0 = clock()
+        now = lambda: clock() - t0
+
+        def drain_one():
+            ev, s = inflight.popleft()
+            self.be.wait(ev)
+            t = now()
+            for a in [x for x in active if x.left == 0 and x.last_step <= s]:
+                self._retire(a, t)
+                active.remove(a)
+
+        while pending or queue or active:
+            t = now()
+            while pending and (not realtime or pending[0].arrival <= t):
+                queue.append(pending.popleft())
+            while queue:
+                a = self._admit(queue[0], t)
+                if a is None:
+                    break
+                active.append(a)
+                queue.popleft()
+            if any(a.ready for a in active):
+                self._step(active, step_no)
+                inflight.append((self.be.record(), step_no))
+                step_no += 1
+                self.stats["steps"] += 1
+                while len(inflight) >= self.max_inflight:
+                    drain_one()
+            elif inflight:
+                drain_one()
+            elif pending and not queue:
+                wait = pending[0].arrival - now()
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+            elif queue and not active:
+                raise ServeError("request %d does not fit the arena" % queue[0].rid)
+        while inflight:
+            drain_one()
+        self.stats["wall_s"] = now()
+        return reqs
+
+
+def latency_summary(reqs) -> dict:
+    done = [r for r in reqs if r.status == "done"]
+    if not done:
+        return {}
+    lat = np.array([r.latency for r in done]) * 1e3
+    return dict(n=len(done), p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                mean_ms=float(lat.mean()), max_ms=float(lat.max()))

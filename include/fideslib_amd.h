@@ -169,6 +169,43 @@ int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_
                        const fides_key* const* keys, const uint32_t* auto_index, uint64_t* const* d_out,
                        void* stream);
 
+/* Rescale of n ciphertexts ([2][l[i]] -> [2][l[i]-1]) in one pass. */
+int fides_rescale_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
+                        uint64_t* const* d_out, void* stream);
+
+/* ---- element-wise ciphertext arithmetic (reference AddSub.cu:10-123 add_ /
+ * sub_ / scalar_, behind CryptoContext::EvalAdd / EvalSub / EvalMult(scalar),
+ * api/CryptoContext.hpp:118-160).  out[i] may alias a[i]. */
+
+/* out = a + b, or a - b when subtract != 0 ([2][l[i]] each). */
+int fides_add_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint64_t* const* d_b,
+                    const uint32_t* l, int subtract, uint64_t* const* d_out, void* stream);
+/* out = a * c for an integer constant c given by its residues: c_res[i] is a
+ * host array of l[i] values c mod q_j. */
+int fides_mul_const_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint32_t* l,
+                          const uint64_t* const* c_res, uint64_t* const* d_out, void* stream);
+/* out = a + c (the constant polynomial c added to component 0). */
+int fides_add_const_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint32_t* l,
+                          const uint64_t* const* c_res, uint64_t* const* d_out, void* stream);
+
+/* Fused form of the three above, one launch for n ciphertexts:
+ * out = a * m  (+ or -) b  + c, where b[i] may be NULL (no b term), mul_res
+ * or mul_res[i] NULL means m = 1, add_res or add_res[i] NULL means c = 0,
+ * and sub[i] != 0 (sub may be NULL) selects subtraction of b.  Residue
+ * arrays are host arrays of l[i] values.  Combines the reference's
+ * mult_scalar / add_scalar / add / sub element-wise kernels
+ * (ElemenwiseBatchKernels.cu:1281-1342). */
+int fides_lin_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint64_t* const* d_b,
+                    const uint32_t* l, const uint64_t* const* mul_res, const uint64_t* const* add_res,
+                    const uint32_t* sub, uint64_t* const* d_out, void* stream);
+
+/* Keep the first l_out[i] limbs of each component: [2][l_in] -> [2][l_out]
+ * (Ciphertext::dropToLevel, src/CKKS/Ciphertext.cpp:1143, RNSPoly::dropToLevel
+ * src/CKKS/RNSPoly.cpp:838; used for level alignment before add/mult,
+ * src/CKKS/Ciphertext.cpp:229-233).  in == out only when l_out == l_in. */
+int fides_drop_limbs_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l_in,
+                           const uint32_t* l_out, uint64_t* const* d_out, void* stream);
+
 /* Keyless evaluation-domain automorphism of nlimbs limbs (permutation only,
  * reference automorph_ src/Rotation.cu:10-28).  d_out must not alias. */
 int fides_automorph(fides_context* ctx, const uint64_t* d_in, uint32_t nlimbs, uint32_t auto_index,

@@ -364,3 +364,89 @@ def test_cfg3_batch_mixed_levels_bit_exact(cfg3):
         gctx.hmult(A[i], B[i], l, g_rlk, m)
         gctx.rescale(m, l, r)
         assert np.array_equal(r.download((2, l - 1, N)), OB[i].download((2, l - 1, N))), (i, l)
+
+
+# ------------------------------------------------------- element-wise batch ops, limb drop, batched rescale
+def test_cfg1_elementwise_batch_ops(cfg1):
+    prm, octx, gctx, keys = cfg1
+    say("cfg1 elementwise")
+    N = prm.N
+    rng = np.random.default_rng(55)
+    levels = [4, 1, 3, 2]
+    A = [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in levels]
+    B = [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in levels]
+    dA = [capi.DeviceBuffer.from_array(x) for x in A]
+    dB = [capi.DeviceBuffer.from_array(x) for x in B]
+    out = [capi.DeviceBuffer(x.size) for x in A]
+
+    def flat(x, l):
+        return x.reshape(2 * l, N)
+
+    def pids(l):
+        return list(range(l)) * 2
+
+    for sub in (False, True):
+        gctx.add_batch(dA, dB, levels, out, subtract=sub)
+        for i, l in enumerate(levels):
+            f = octx.sub_limbs if sub else octx.add_limbs
+            assert np.array_equal(out[i].download((2 * l, N)), f(flat(A[i], l), flat(B[i], l), pids(l))), (sub, i)
+    consts = [3, -7, (1 << 100) + 12345, -(1 << 70)]
+    gctx.mul_const_batch(dA, levels, consts, out)
+    for i, l in enumerate(levels):
+        cres = np.stack([np.full(N, consts[i] % prm.q[j], np.uint64) for j in range(l)] * 2)
+        assert np.array_equal(out[i].download((2 * l, N)), octx.mul_limbs(flat(A[i], l), cres, pids(l))), i
+    gctx.add_const_batch(dA, levels, consts, out)
+    for i, l in enumerate(levels):
+        cres = np.stack([np.full(N, consts[i] % prm.q[j], np.uint64) for j in range(l)] + [np.zeros(N, np.uint64)] * l)
+        assert np.array_equal(out[i].download((2 * l, N)), octx.add_limbs(flat(A[i], l), cres, pids(l))), i
+    # in place: out aliases a
+    gctx.add_batch(dA, dB, levels, dA)
+    for i, l in enumerate(levels):
+        assert np.array_equal(dA[i].download((2 * l, N)), octx.add_limbs(flat(A[i], l), flat(B[i], l), pids(l)))
+    # limb drop
+    lo = [2, 1, 1, 2]
+    dout = [capi.DeviceBuffer(2 * x * N) for x in lo]
+    gctx.drop_limbs_batch(dB, levels, lo, dout)
+    for i in range(len(levels)):
+        assert np.array_equal(dout[i].download((2, lo[i], N)), B[i][:, :lo[i]]), i
+    with pytest.raises(capi.FidesError):
+        gctx.drop_limbs_batch([dB[0]], [4], [2], [dB[0]])
+    with pytest.raises(capi.FidesError):
+        gctx.mul_const_batch([dA[0]], [5], [1], [out[0]])  # level above L
+    # batched rescale at mixed levels
+    lv = [4, 2, 3]
+    rin = [capi.DeviceBuffer.from_array(A[0]), capi.DeviceBuffer.from_array(B[3]), capi.DeviceBuffer.from_array(B[2])]
+    hin = [A[0], B[3], B[2]]
+    rout = [capi.DeviceBuffer(2 * (l - 1) * N) for l in lv]
+    gctx.rescale_batch(rin, lv, rout)
+    for i, l in enumerate(lv):
+        assert np.array_equal(rout[i].download((2, l - 1, N)), octx.rescale_ct(hin[i], l)), i
+
+
+def test_cfg1_lin_batch_fused(cfg1):
+    """out = a*m -/+ b + c with optional terms, against the oracle's limb ops."""
+    prm, octx, gctx, keys = cfg1
+    N = prm.N
+    rng = np.random.default_rng(56)
+    levels = [4, 3, 2, 4, 1]
+    A = [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in levels]
+    B = [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in levels]
+    dA = [capi.DeviceBuffer.from_array(x) for x in A]
+    dB = [capi.DeviceBuffer.from_array(x) for x in B]
+    out = [capi.DeviceBuffer(x.size) for x in A]
+    mul = [2, None, -(1 << 80), 5, 7]
+    add = [-(1 << 59), 3, None, None, 1]
+    useb = [True, True, False, True, False]
+    sub = [True, False, False, True, False]
+    gctx.lin_batch(dA, [dB[i] if useb[i] else None for i in range(5)], levels, mul, add, sub, out)
+    for i, l in enumerate(levels):
+        pid = list(range(l)) * 2
+        x = A[i].reshape(2 * l, N)
+        if mul[i] is not None:
+            x = octx.mul_limbs(x, np.stack([np.full(N, mul[i] % prm.q[j], np.uint64) for j in pid]), pid)
+        if useb[i]:
+            x = (octx.sub_limbs if sub[i] else octx.add_limbs)(x, B[i].reshape(2 * l, N), pid)
+        if add[i] is not None:
+            c = np.stack([np.full(N, add[i] % prm.q[j], np.uint64) for j in range(l)] + [np.zeros(N, np.uint64)] * l)
+            x = octx.add_limbs(x, c, pid)
+        assert np.array_equal(out[i].download((2 * l, N)), x), i
