@@ -15,7 +15,13 @@ with HIP events on the stream it runs on; the CPU baseline is the repo's
 C oracle (own CPU reference, not OpenFHE) on a bounded sample of the same
 operation.
 
+A serving leg follows: Poisson arrivals of Chebyshev-series and
+rotate-and-sum requests through the continuous-batching loop
+(fideslib_amd/serve.py), reported as p50/p99 latency beside ops/s in the
+"serve" object.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
+                       [--no-serve] [--serve-requests R] [--serve-rate LAMBDA] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -259,6 +265,81 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
     return "\n".join(out), res
 
 
+# ----------------------------------------------------------------- serving leg
+SERVE_KMAX = 15          # rotate-and-sum over 2^k slots, k ~ U{1..15}: rotation keys for 2^0 .. 2^14
+SERVE_ARENA_GIB = 64     # admission cap: ciphertext storage of admitted requests
+
+
+def serve_leg(prm, ctx, w, n_req, rate, seed):
+    """Poisson(rate) arrivals of a request mix served by the continuous-
+    batching loop; keys resident.  Mix: degree-15 Chebyshev series of the
+    sigmoid on slots in [-8, 8] (start limbs U{7..30}) and rotate-and-sum
+    over 2^k slots (k ~ U{1..15}, start limbs U{1..30}) in the ratio 2:1
+    (the 60:30 split of the serving workload; its bootstrap share is not
+    served).  Inputs are copied from resident ciphertexts at admission."""
+    from fideslib_amd import serve as S
+    N = prm.N
+    rng = np.random.default_rng(seed)
+    keys = {pow(5, 1 << i, 2 * N): w.rot_keys[i] for i in range(SERVE_KMAX)}
+    be = S.GpuBackend(ctx, w.rlk, keys, SERVE_ARENA_GIB << 30)
+    top = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])
+    pool = {l: capi.DeviceBuffer.from_array(np.ascontiguousarray(top[:, :l])) for l in range(1, prm.L + 1)}
+    coef = S.chebyshev_coefficients(lambda x: 1.0 / (1.0 + np.exp(-x)), 15, -8.0, 8.0)
+    dmin = S.chebyshev_depth(15) + 1
+
+    def make(n, arrivals):
+        out = []
+        for i in range(n):
+            if rng.random() < 2.0 / 3.0:
+                P = S.chebyshev_program(prm, int(rng.integers(dmin, prm.L + 1)), coef, -8.0, 8.0)
+            else:
+                P = S.rotsum_program(prm, int(rng.integers(1, prm.L + 1)), int(rng.integers(1, SERVE_KMAX + 1)))
+            out.append(S.Request(i, P, float(arrivals[i]), inputs=[("dev", pool[P.inputs[0].level].ptr)]))
+        return out
+
+    # warm-up, then capacity: every request present at t = 0
+    S.Server(be, N).run(make(16, np.zeros(16)), realtime=False)
+    ctx.sync()
+    cap_reqs = make(n_req, np.zeros(n_req))
+    srv = S.Server(be, N)
+    srv.run(cap_reqs, realtime=False)
+    cap = dict(req_per_s=round(n_req / srv.stats["wall_s"], 2),
+               keyswitch_ops_per_s=round(srv.stats["keyswitch_ops"] / srv.stats["wall_s"], 2),
+               steps=srv.stats["steps"], launches=srv.stats["launches"], ops=srv.stats["ops"],
+               arena_peak_gib=round(srv.arena.peak * N * 8 / 2 ** 30, 2))
+    print("bench: serve capacity %s" % cap, file=sys.stderr, flush=True)
+    # open loop at the stated rate
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    reqs = make(n_req, arr)
+    srv = S.Server(be, N)
+    srv.run(reqs, realtime=True)
+    lat = S.latency_summary(reqs)
+    kinds = {}
+    for name in ("chebyshev15", "rotsum"):
+        sub = [r for r in reqs if r.program.name.startswith(name)]
+        if sub:
+            kinds[name] = {k: round(v, 3) for k, v in S.latency_summary(sub).items()}
+    res = {
+        "workload": "Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 2:1 degree-15 Chebyshev (sigmoid on "
+                    "[-8,8], start limbs U{%d..30}) : rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
+                    "U{1..30}); keys resident; admission cap %d GiB" % (dmin, SERVE_ARENA_GIB),
+        "rate_req_per_s": rate,
+        "requests": n_req,
+        "p50_ms": round(lat["p50_ms"], 3),
+        "p99_ms": round(lat["p99_ms"], 3),
+        "mean_ms": round(lat["mean_ms"], 3),
+        "by_kind": kinds,
+        "achieved_req_per_s": round(n_req / srv.stats["wall_s"], 2),
+        "keyswitch_ops_per_s": round(srv.stats["keyswitch_ops"] / srv.stats["wall_s"], 2),
+        "ops_per_s": round(srv.stats["ops"] / srv.stats["wall_s"], 2),
+        "steps": srv.stats["steps"],
+        "launches": srv.stats["launches"],
+        "arena_peak_gib": round(srv.arena.peak * N * 8 / 2 ** 30, 2),
+        "capacity_closed_batch": cap,
+    }
+    return res
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         a = sys.argv[2:]
@@ -273,6 +354,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched", choices=["plain", "batched"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=150.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -335,6 +419,9 @@ def main():
         },
         "roofline": roof,
     }
+    if not args.no_serve:
+        print("bench: serving leg", file=sys.stderr, flush=True)
+        out["serve"] = serve_leg(prm, ctx, w, args.serve_requests, args.serve_rate, SEED + 5)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(prm)

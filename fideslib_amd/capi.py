@@ -70,6 +70,7 @@ SIGNATURES = {
                                  ctypes.POINTER(_vp), _vp]),
     "fides_rotate_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _u32p,
                                   ctypes.POINTER(_vp), _vp]),
+    "fides_rotate_hoisted": (_int, [_vp, _vp, _u32, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
                                ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
@@ -245,6 +246,15 @@ class Context:
         kx = np.ascontiguousarray(kidx, np.uint32)
         check(load().fides_rotate_batch(self.h, n, I, lv.ctypes.data_as(_u32p), Kh, kx.ctypes.data_as(_u32p), O,
                                         stream), "fides_rotate_batch")
+
+    def rotate_hoisted(self, inp, level, keys, kidx, out, stream=None):
+        """n rotations of one ciphertext sharing one ModUp; keys[i] may be None when kidx[i] == 1."""
+        n = len(kidx)
+        K = (_vp * n)(*[None if k is None else k.h for k in keys])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        ki = np.ascontiguousarray(kidx, np.uint32)
+        check(load().fides_rotate_hoisted(self.h, _ptr(inp), level, n, K, ki.ctypes.data_as(_u32p), O, stream),
+              "fides_rotate_hoisted")
 
     def rescale_batch(self, inp, levels, out, stream=None):
         n = len(levels)

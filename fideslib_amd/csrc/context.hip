@@ -624,13 +624,16 @@ struct KsJob {
   u32 perm = 0;     // store the result through the automorphism with this k^-1 (0: none)
   bool rescale = false;  // fuse Rescale into the ModDown: out0 = [2][l-1] result of
                          // Rescale(acc + KeySwitch(d)); acc0/acc1 required
+  int share = -1;        // index of an earlier job with the same d and l whose ModUp
+                         // (decomposed, extended digits) this job reuses (hoisting)
 };
 
 // Arena words needed by the key switch of one job.
-static size_t ks_words(const fides_context* c, u32 l) {
+static size_t ks_words(const fides_context* c, u32 l, bool shared_modup = false) {
   const size_t N = c->N;
   const u32 beta = num_digits(c, l);
-  return N * ((size_t)l + (size_t)beta * (l + c->K) + 2 * (size_t)(l + c->K) + 2 * (size_t)l + 2);
+  const size_t modup = shared_modup ? 0 : (size_t)l + (size_t)beta * (l + c->K);
+  return N * (modup + 2 * (size_t)(l + c->K) + 2 * (size_t)l + 2);
 }
 
 // Appends the task lists of the batched key switch (reference pipeline
@@ -658,21 +661,25 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   std::vector<DotTask> dot;
   std::vector<AxpyTask> axpy1, axpy2;
   u64* w = scratch;
-  for (const KsJob& J : jobs) {
+  std::vector<u64*> ext_of(jobs.size(), nullptr);
+  for (size_t jn = 0; jn < jobs.size(); ++jn) {
+    const KsJob& J = jobs[jn];
     const u32 l = J.l, beta = num_digits(c, l), ext_stride = l + K;
+    const bool shared = J.share >= 0;
     u64* Dc = w;
-    u64* EXT = Dc + (size_t)l * N;
-    u64* T0 = EXT + (size_t)beta * ext_stride * N;
+    u64* EXT = shared ? ext_of[J.share] : Dc + (size_t)l * N;
+    u64* T0 = shared ? w : EXT + (size_t)beta * ext_stride * N;
     u64* T1 = T0 + (size_t)(l + K) * N;
     u64* Y = T1 + (size_t)(l + K) * N;
     u64* RS = Y + (size_t)2 * l * N;
     w = RS + (size_t)2 * N;
-    for (u32 i = 0; i < l; ++i) {
+    ext_of[jn] = EXT;
+    for (u32 i = 0; i < l && !shared; ++i) {
       NttTask t = ntask(c->limb(Dc, i), i);
       t.src = c->limb(J.d, i);
       intt_in.push_back(t);
     }
-    for (u32 j = 0; j < beta; ++j) {
+    for (u32 j = 0; j < beta && !shared; ++j) {
       u32 st, en;
       digit_range(c, l, j, st, en);
       const u32 ns = en - st;
@@ -1217,6 +1224,55 @@ static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32
     J.perm = inv_odd(kidx[i], 2 * N);
     jobs.push_back(J);
   }
+  Blob blob;
+  KsLists K_;
+  ks_build(c, jobs, c->arena, blob, K_);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(ks_launch(c, dev, K_, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+// n rotations of one ciphertext sharing one ModUp (reference
+// Ciphertext::rotate_hoisted, src/CKKS/Ciphertext.cpp:940-1114: the digits are
+// raised once, then each index runs the inner product with its own rotation
+// key, ModDown, the c0 addition and the automorphism).  The result of each
+// index is bit-identical to fides_rotate with the same key; index 1 (the
+// identity) copies.
+extern "C" int fides_rotate_hoisted(fides_context* c, const uint64_t* in, uint32_t l, uint32_t n,
+                                    const fides_key* const* keys, const uint32_t* kidx, uint64_t* const* out,
+                                    void* stream) {
+  if (!c || !in || (n && (!keys || !kidx || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+  hipStream_t st = (hipStream_t)stream;
+  const u32 N = c->N;
+  const size_t S = (size_t)l * N;
+  for (u32 i = 0; i < n; ++i) {
+    if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
+    if (out[i] + 2 * S > in && in + 2 * S > out[i]) return fail(FIDES_ERR_ARG, "rotate output must not overlap its input");
+    for (u32 j = 0; j < i; ++j)
+      if (out[i] + 2 * S > out[j] && out[j] + 2 * S > out[i]) return fail(FIDES_ERR_ARG, "outputs overlap");
+    if (kidx[i] != 1) RCHK(check_key(c, keys[i], l));
+  }
+  std::vector<KsJob> jobs;
+  size_t ksw = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (kidx[i] == 1) {
+      HIPCHK(hipMemcpyAsync(out[i], in, 2 * S * sizeof(u64), hipMemcpyDeviceToDevice, st));
+      continue;
+    }
+    KsJob J{in + S, l, keys[i], out[i], out[i] + S, in, nullptr};
+    J.perm = inv_odd(kidx[i], 2 * N);
+    J.share = jobs.empty() ? -1 : 0;
+    ksw += ks_words(c, l, !jobs.empty());
+    jobs.push_back(J);
+  }
+  if (jobs.empty()) return FIDES_OK;
+  RCHK(ensure_arena(c, ksw, st));
   Blob blob;
   KsLists K_;
   ks_build(c, jobs, c->arena, blob, K_);

@@ -297,6 +297,40 @@ std::vector<Ciphertext> CryptoContextImpl::EvalMultMany(const std::vector<Cipher
   return out;
 }
 
+std::shared_ptr<void> CryptoContextImpl::EvalFastRotationPrecompute(const Ciphertext&) {
+  // The device path raises the digits inside EvalFastRotation (as the
+  // reference's GPU path does, api/CryptoContext.cpp:1451-1465).
+  return nullptr;
+}
+
+std::vector<Ciphertext> CryptoContextImpl::EvalFastRotation(const Ciphertext& a, const std::vector<int32_t>& indices,
+                                                            uint32_t, const std::shared_ptr<void>&) {
+  std::vector<uint64_t*> po;
+  std::vector<uint32_t> kv;
+  std::vector<const fides_key*> keys;
+  std::vector<Ciphertext> out;
+  for (int32_t idx : indices) {
+    const uint32_t k = AutoIndex(idx);
+    Ciphertext o = NewCiphertext(a->limbs);
+    o->NoiseLevel = a->NoiseLevel;
+    o->NoiseFactor = a->NoiseFactor;
+    o->slots = a->slots;
+    o->keyid = a->keyid;
+    po.push_back(o->d);
+    kv.push_back(k);
+    keys.push_back(k == 1 ? nullptr : RotationKey(k));
+    out.push_back(o);
+  }
+  chk(fides_rotate_hoisted(ctx, a->d, a->limbs, (uint32_t)indices.size(), keys.data(), kv.data(), po.data(), nullptr),
+      "EvalFastRotation");
+  return out;
+}
+
+Ciphertext CryptoContextImpl::EvalFastRotation(const Ciphertext& a, int32_t index, uint32_t m,
+                                               const std::shared_ptr<void>& precomp) {
+  return EvalFastRotation(a, std::vector<int32_t>{index}, m, precomp)[0];
+}
+
 std::vector<Ciphertext> CryptoContextImpl::EvalRotateMany(const std::vector<Ciphertext>& a,
                                                           const std::vector<int32_t>& idx) {
   if (a.size() != idx.size()) throw std::runtime_error("EvalRotateMany: counts differ");

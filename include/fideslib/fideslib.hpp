@@ -122,6 +122,12 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   Ciphertext EvalRotate(const Ciphertext& a, int32_t index);
   void EvalRotateInPlace(Ciphertext& a, int32_t index);
   Ciphertext EvalConjugate(const Ciphertext& a);
+  // Hoisted rotations (api/CryptoContext.hpp:169-174): one ModUp shared by
+  // all indices; each result equals EvalRotate with the same index.
+  std::shared_ptr<void> EvalFastRotationPrecompute(const Ciphertext& a);
+  Ciphertext EvalFastRotation(const Ciphertext& a, int32_t index, uint32_t m, const std::shared_ptr<void>& precomp);
+  std::vector<Ciphertext> EvalFastRotation(const Ciphertext& a, const std::vector<int32_t>& indices, uint32_t m,
+                                           const std::shared_ptr<void>& precomp);
   Ciphertext Rescale(const Ciphertext& a);
   void RescaleInPlace(Ciphertext& a);
   // Drop limbs above `limbs` (OpenFHE LevelReduce / FIDESlib dropToLevel).

@@ -169,6 +169,17 @@ int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_
                        const fides_key* const* keys, const uint32_t* auto_index, uint64_t* const* d_out,
                        void* stream);
 
+/* Hoisted rotations: n rotations of ONE ciphertext [2][l][N] sharing a single
+ * ModUp of c1 (reference Ciphertext::rotate_hoisted,
+ * src/CKKS/Ciphertext.cpp:940-1114, behind CryptoContext::EvalFastRotation /
+ * EvalFastRotationPrecompute, api/CryptoContext.hpp:169-174).  keys[i] is the
+ * rotation key of kidx[i] (5^r mod 2N, or 2N-1); kidx[i] == 1 copies the
+ * input (its key may be NULL).  Each out[i] ([2][l][N]) is bit-identical to
+ * fides_rotate(in, l, keys[i], kidx[i]).  Outputs must not overlap the input
+ * or each other. */
+int fides_rotate_hoisted(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint32_t n,
+                         const fides_key* const* keys, const uint32_t* kidx, uint64_t* const* d_out, void* stream);
+
 /* Rescale of n ciphertexts ([2][l[i]] -> [2][l[i]-1]) in one pass. */
 int fides_rescale_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
                         uint64_t* const* d_out, void* stream);

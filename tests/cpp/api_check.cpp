@@ -1,0 +1,134 @@
+// Drives the host C++ API (fideslib::CryptoContext) the way an application
+// does: Create -> AddEvalMultKey / AddRotationKey -> LoadCiphertext ->
+// EvalMult -> Rescale -> EvalRotate / EvalFastRotation / EvalMultMany ->
+// StoreCiphertext.  Inputs and outputs are raw little-endian uint64 files in
+// the directory given as argv[1] (written and checked by
+// tests/test_cpp_api_gpu.py against the CPU oracle).
+//
+//   params.bin : logN L K dnum l rot q[L] p[K]
+//   rlk.bin, rot.bin : [2][dnum][L+K][N] (a then b)
+//   c1.bin, c2.bin   : [2][l][N]
+// Outputs: mult.bin ([2][l-1][N], Rescale(EvalMult(c1, c2))),
+//          rot.bin  ([2][l-1][N], EvalRotate(mult, rot)),
+//          fast.bin ([3][2][l-1][N], EvalFastRotation(mult, {rot, 0, rot})),
+//          many.bin ([2][2][l-1][N], EvalMultMany({c1, c2}, {c2, c1}, rescale)),
+//          meta.txt (NoiseLevel / NoiseFactor / limbs of the results).
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <iomanip>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fideslib/fideslib.hpp"
+
+static std::vector<uint64_t> read_u64(const std::string& path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cannot open " + path);
+  const std::streamsize n = f.tellg();
+  f.seekg(0);
+  std::vector<uint64_t> v((size_t)n / 8);
+  f.read(reinterpret_cast<char*>(v.data()), n);
+  return v;
+}
+
+static void write_u64(const std::string& path, const std::vector<uint64_t>& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * 8));
+}
+
+static fideslib::RawKeySwitchKey key_of(const std::vector<uint64_t>& flat, uint32_t dnum, uint32_t LK, uint32_t N) {
+  fideslib::RawKeySwitchKey k;
+  k.r_key.resize(2);
+  size_t o = 0;
+  for (uint32_t c = 0; c < 2; ++c) {
+    k.r_key[c].resize(dnum);
+    for (uint32_t d = 0; d < dnum; ++d) {
+      k.r_key[c][d].resize(LK);
+      for (uint32_t t = 0; t < LK; ++t, o += N) k.r_key[c][d][t].assign(flat.begin() + o, flat.begin() + o + N);
+    }
+  }
+  return k;
+}
+
+static fideslib::RawCipherText ct_of(const std::vector<uint64_t>& flat, uint32_t l, uint32_t N, double scale) {
+  fideslib::RawCipherText c;
+  c.sub_0.resize(l);
+  c.sub_1.resize(l);
+  for (uint32_t i = 0; i < l; ++i) {
+    c.sub_0[i].assign(flat.begin() + (size_t)i * N, flat.begin() + (size_t)(i + 1) * N);
+    c.sub_1[i].assign(flat.begin() + (size_t)(l + i) * N, flat.begin() + (size_t)(l + i + 1) * N);
+  }
+  c.numRes = (int)l;
+  c.N = (int)N;
+  c.Noise = scale;
+  c.NoiseLevel = 1;
+  c.slots = (int)N / 2;
+  return c;
+}
+
+static void append(std::vector<uint64_t>& out, const fideslib::RawCipherText& r) {
+  for (const auto& limb : r.sub_0) out.insert(out.end(), limb.begin(), limb.end());
+  for (const auto& limb : r.sub_1) out.insert(out.end(), limb.begin(), limb.end());
+}
+
+int main(int argc, char** argv) {
+  if (argc != 2) {
+    std::fprintf(stderr, "usage: api_check DIR\n");
+    return 2;
+  }
+  const std::string dir = argv[1];
+  try {
+    const auto P = read_u64(dir + "/params.bin");
+    const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+    const uint32_t l = (uint32_t)P[4];
+    const int32_t rot = (int32_t)P[5];
+    const uint32_t N = 1u << logN;
+    fideslib::RawParams rp;
+    rp.logN = (int)logN;
+    rp.moduli.assign(P.begin() + 6, P.begin() + 6 + L);
+    rp.SPECIALmoduli.assign(P.begin() + 6 + L, P.begin() + 6 + L + K);
+    rp.dnum = (int)dnum;
+    rp.scalingTechnique = fideslib::FIXEDMANUAL;
+    auto cc = fideslib::CryptoContextImpl::Create(rp);
+    cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
+    cc->AddRotationKey(rot, key_of(read_u64(dir + "/rot.bin"), dnum, L + K, N));
+    const double scale = (double)rp.moduli[l - 1];
+    auto a = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), l, N, scale));
+    auto b = cc->LoadCiphertext(ct_of(read_u64(dir + "/c2.bin"), l, N, scale));
+    auto m = cc->Rescale(cc->EvalMult(a, b));
+    auto r = cc->EvalRotate(m, rot);
+    auto f = cc->EvalFastRotation(m, std::vector<int32_t>{rot, 0, rot}, 2 * N, cc->EvalFastRotationPrecompute(m));
+    auto many = cc->EvalMultMany({a, b}, {b, a}, true);
+    std::vector<uint64_t> out;
+    append(out, cc->StoreCiphertext(m));
+    write_u64(dir + "/mult.bin", out);
+    out.clear();
+    append(out, cc->StoreCiphertext(r));
+    write_u64(dir + "/rot.bin", out);
+    out.clear();
+    for (auto& x : f) append(out, cc->StoreCiphertext(x));
+    write_u64(dir + "/fast.bin", out);
+    out.clear();
+    for (auto& x : many) append(out, cc->StoreCiphertext(x));
+    write_u64(dir + "/many.bin", out);
+    std::ofstream meta(dir + "/meta.txt");
+    meta << std::setprecision(17);
+    meta << m->GetLimbs() << " " << m->NoiseLevel << " " << m->NoiseFactor << "\n";
+    meta << r->GetLimbs() << " " << r->NoiseLevel << " " << r->NoiseFactor << "\n";
+    // a missing rotation key is an error, not a crash
+    bool threw = false;
+    try {
+      cc->EvalRotate(m, rot + 1);
+    } catch (const std::runtime_error&) {
+      threw = true;
+    }
+    meta << (threw ? "missing-key-throws" : "missing-key-accepted") << "\n";
+    std::printf("api_check: ok\n");
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "api_check: %s\n", e.what());
+    return 1;
+  }
+  return 0;
+}

@@ -1,0 +1,62 @@
+"""The host C++ API (include/fideslib/fideslib.hpp) on the GPU: a C++
+program built against libfideslib_amd_host.so runs EvalMult, Rescale,
+EvalRotate, EvalFastRotation and EvalMultMany on ciphertexts and keys made
+by the CPU oracle; every limb it stores must equal the oracle's."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fideslib_amd import params
+from oracle import ckks_codec as C
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(HERE, "cpp", "api_check")
+
+
+def test_cpp_api_against_oracle(tmp_path):
+    assert os.path.exists(BIN), "tests/cpp/api_check is not built (run __graft_entry__.build())"
+    prm = params.config1()
+    N, l, rot = prm.N, prm.L, 3
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 17)
+    rng = np.random.default_rng(8)
+    z = [rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2) for _ in range(2)]
+    cts = [keys.encrypt(C.encode(v, 2.0 ** prm.scale_bits, N), l) for v in z]
+    rlk = keys.relin_key()
+    k = pow(5, rot, 2 * N)
+    rk = keys.rotation_key(k)
+    d = str(tmp_path)
+    hdr = [prm.logN, prm.L, prm.K, prm.dnum, l, rot] + list(prm.q) + list(prm.p)
+    np.asarray(hdr, np.uint64).tofile(os.path.join(d, "params.bin"))
+    rlk.astype(np.uint64).tofile(os.path.join(d, "rlk.bin"))
+    rk.astype(np.uint64).tofile(os.path.join(d, "rot.bin"))
+    cts[0].astype(np.uint64).tofile(os.path.join(d, "c1.bin"))
+    cts[1].astype(np.uint64).tofile(os.path.join(d, "c2.bin"))
+    print("cpp api: running", BIN, flush=True)
+    r = subprocess.run([BIN, d], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    shape = (2, l - 1, N)
+    mult = octx.rescale_ct(octx.hmult_relin(cts[0], cts[1], l, rlk), l)
+    rotd = octx.rotate(mult, l - 1, rk, k)
+    assert np.array_equal(np.fromfile(os.path.join(d, "mult.bin"), np.uint64).reshape(shape), mult)
+    assert np.array_equal(np.fromfile(os.path.join(d, "rot.bin"), np.uint64).reshape(shape), rotd)
+    fast = np.fromfile(os.path.join(d, "fast.bin"), np.uint64).reshape((3,) + shape)
+    assert np.array_equal(fast[0], rotd) and np.array_equal(fast[2], rotd)
+    assert np.array_equal(fast[1], mult)  # index 0: the input itself
+    many = np.fromfile(os.path.join(d, "many.bin"), np.uint64).reshape((2,) + shape)
+    assert np.array_equal(many[0], mult)
+    assert np.array_equal(many[1], octx.rescale_ct(octx.hmult_relin(cts[1], cts[0], l, rlk), l))
+    meta = open(os.path.join(d, "meta.txt")).read().split("\n")
+    limbs, nl, nf = meta[0].split()
+    assert int(limbs) == l - 1 and int(nl) == 1
+    # NoiseFactor: scale^2 / ModReduceFactor (the dropped prime)
+    assert abs(float(nf) - prm.q[l - 1] ** 2 / prm.q[l - 1]) / prm.q[l - 1] < 1e-12
+    assert meta[2] == "missing-key-throws"
+    # decrypts to the product rotated by 3 slots
+    dec = C.decode(keys.decrypt_coef(rotd, l - 1), 2.0 ** prm.scale_bits * 2.0 ** prm.scale_bits / prm.q[l - 1], N)
+    assert C.precision_bits(dec, np.roll(z[0] * z[1], -rot)) > 15

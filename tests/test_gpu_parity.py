@@ -450,3 +450,50 @@ def test_cfg1_lin_batch_fused(cfg1):
             c = np.stack([np.full(N, add[i] % prm.q[j], np.uint64) for j in range(l)] + [np.zeros(N, np.uint64)] * l)
             x = octx.add_limbs(x, c, pid)
         assert np.array_equal(out[i].download((2 * l, N)), x), i
+
+
+# ------------------------------------------------------- hoisted rotations (one ModUp, many keys)
+def test_cfg1_rotate_hoisted_equals_rotate(cfg1):
+    prm, octx, gctx, keys = cfg1
+    say("cfg1 hoisted")
+    N, l = prm.N, 3
+    _, (c1, _) = _enc_pair(prm, keys, 71, l)
+    ks = [5, 1, 2 * N - 1, pow(5, 7, 2 * N), pow(5, -3, 2 * N)]
+    hk = {k: keys.rotation_key(k) for k in ks if k != 1}
+    gk = {k: gctx.upload_key(v) for k, v in hk.items()}
+    inp = capi.DeviceBuffer.from_array(c1)
+    outs = [capi.DeviceBuffer(2 * l * N) for _ in ks]
+    gctx.rotate_hoisted(inp, l, [gk.get(k) for k in ks], ks, outs)
+    for k, o in zip(ks, outs):
+        got = o.download((2, l, N))
+        if k == 1:
+            assert np.array_equal(got, c1)
+            continue
+        assert np.array_equal(got, octx.rotate(c1, l, hk[k], k)), k
+        single = capi.DeviceBuffer(2 * l * N)
+        gctx.rotate(inp, l, gk[k], k, single)
+        assert np.array_equal(single.download((2, l, N)), got), k
+    assert np.array_equal(inp.download((2, l, N)), c1), "input must be unchanged"
+    with pytest.raises(capi.FidesError):
+        gctx.rotate_hoisted(inp, l, [gk[5]], [5], [inp])  # output overlaps input
+    with pytest.raises(capi.FidesError):
+        gctx.rotate_hoisted(inp, l, [gk[5], gk[5]], [5, 5], [outs[0], outs[0]])  # outputs overlap
+
+
+def test_cfg3_rotate_hoisted_equals_batch(cfg3):
+    prm, octx, gctx, keys = cfg3
+    say("cfg3 hoisted")
+    N, l = prm.N, 23
+    rng = np.random.default_rng(72)
+    c = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+    host_key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    ks = [pow(5, int(r), 2 * N) for r in rng.integers(1, 1 << 15, 6)]
+    gk = [gctx.upload_key(host_key) for _ in ks]
+    inp = capi.DeviceBuffer.from_array(c)
+    outs = [capi.DeviceBuffer(2 * l * N) for _ in ks]
+    gctx.rotate_hoisted(inp, l, gk, ks, outs)
+    ref = [capi.DeviceBuffer(2 * l * N) for _ in ks]
+    gctx.rotate_batch([inp] * len(ks), [l] * len(ks), gk, ks, ref)
+    for i in range(len(ks)):
+        assert np.array_equal(outs[i].download((2, l, N)), ref[i].download((2, l, N))), i
+    assert np.array_equal(outs[2].download((2, l, N)), octx.rotate(c, l, host_key, ks[2]))
