@@ -265,6 +265,33 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
     return "\n".join(out), res
 
 
+# ----------------------------------------------------------------- hoisted rotations
+def hoisted_leg(prm, ctx, w, ev, n_rot=16, reps=5):
+    """n_rot rotations of one 30-limb ciphertext: fides_rotate_hoisted (one
+    shared ModUp) against fides_rotate_batch of the same n_rot rotations."""
+    N, l = prm.N, prm.L
+    rng = np.random.default_rng(SEED + 7)
+    inp = capi.DeviceBuffer.from_array(np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)]))
+    outs = [capi.DeviceBuffer(2 * l * N) for _ in range(n_rot)]
+    ks = [pow(5, int(r), 2 * N) for r in rng.integers(1, 1 << 15, n_rot)]
+    keys = w.rot_keys[:n_rot]
+
+    def t(fn):
+        fn()
+        ctx.sync()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(reps):
+            fn()
+        ev.record(e1)
+        return ev.elapsed_ms(e0, e1) / reps
+
+    ms_h = t(lambda: ctx.rotate_hoisted(inp, l, keys, ks, outs))
+    ms_b = t(lambda: ctx.rotate_batch([inp] * n_rot, [l] * n_rot, keys, ks, outs))
+    return {"rotations": n_rot, "limbs": l, "hoisted_ms": round(ms_h, 3), "separate_batched_ms": round(ms_b, 3),
+            "speedup": round(ms_b / ms_h, 3)}
+
+
 # ----------------------------------------------------------------- serving leg
 SERVE_KMAX = 15          # rotate-and-sum over 2^k slots, k ~ U{1..15}: rotation keys for 2^0 .. 2^14
 SERVE_ARENA_GIB = 64     # admission cap: ciphertext storage of admitted requests
@@ -356,7 +383,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-rate", type=float, default=150.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -419,6 +446,8 @@ def main():
         },
         "roofline": roof,
     }
+    out["config"]["hoisted_rotations"] = hoisted_leg(prm, ctx, w, ev)
+    print("bench: hoisted %s" % out["config"]["hoisted_rotations"], file=sys.stderr, flush=True)
     if not args.no_serve:
         print("bench: serving leg", file=sys.stderr, flush=True)
         out["serve"] = serve_leg(prm, ctx, w, args.serve_requests, args.serve_rate, SEED + 5)

@@ -76,6 +76,52 @@ __device__ __forceinline__ u64 shoup_lazy(u64 x, u64 w, u64 wp, u64 q) {
   return mullo64_add(h, (u64)0 - q, x * w);
 }
 
+// The same product for the NTT butterflies, with ALU-only inline assembly
+// where the compiler's lowering costs extra instructions: the middle column
+// of x * wp is summed by one v_mad_u64_u32 whose carry-out is added back with
+// v_addc (instead of three register-pair moves), and the cross products of
+// h * (2^64 - q) are two v_mul_lo_u32 and one v_add3_u32 (the compiler
+// otherwise folds them into a v_mad_u64_u32 with two moves and a 64-bit add).
+// WS: w and wp are wave-uniform (scalar registers).  qn = 2^64 - q, uniform.
+template <bool WS>
+__device__ __forceinline__ u64 shoup_lazy_bf(u64 x, u64 w, u64 wp, u64 qn) {
+  const u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+  const u32 p0 = (u32)wp, p1 = (u32)(wp >> 32);
+  u64 B = (u64)x0 * p1 + (u64)__umulhi(x0, p0);  // < 2^64
+  u64 c;
+  if constexpr (WS)
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(B), "=s"(c) : "v"(x1), "s"(p0));
+  else
+    asm("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(B), "=s"(c) : "v"(x1), "v"(p0));
+  // floor(x wp / 2^64) = x1 p1 + floor(B / 2^32) + c 2^32 (fits 64 bits)
+  const u64 H = (u64)x1 * p1 + (B >> 32);
+  u32 h1 = (u32)(H >> 32);
+  u64 junk;
+  asm("v_addc_co_u32_e64 %0, %1, %0, 0, %2" : "+v"(h1), "=s"(junk) : "s"(c));
+  const u32 h0 = (u32)H;
+  const u64 P = (u64)h0 * (u32)qn + x * w;  // low word: x w + h0 qn0
+  u32 t1, t2, hi;
+  asm("v_mul_lo_u32 %0, %1, %2" : "=v"(t1) : "v"(h0), "s"((u32)(qn >> 32)));
+  asm("v_mul_lo_u32 %0, %1, %2" : "=v"(t2) : "v"(h1), "s"((u32)qn));
+  asm("v_add3_u32 %0, %1, %2, %3" : "=v"(hi) : "v"((u32)(P >> 32)), "v"(t1), "v"(t2));
+  return ((u64)hi << 32) | (u32)P;
+}
+
+// x - m if x >= m else x: subtract with borrow, then select on the borrow
+// (four instructions; m is kept in vector registers).
+__device__ __forceinline__ u64 csub_bf(u64 x, u64 m) {
+  const u32 x0 = (u32)x, x1 = (u32)(x >> 32);
+  u32 r0, r1;
+  asm("v_sub_co_u32_e32 %0, vcc, %2, %4\n\t"
+      "v_subb_co_u32_e32 %1, vcc, %3, %5, vcc\n\t"
+      "v_cndmask_b32_e32 %0, %0, %2, vcc\n\t"
+      "v_cndmask_b32_e32 %1, %1, %3, vcc"
+      : "=&v"(r0), "=&v"(r1)
+      : "v"(x0), "v"(x1), "v"((u32)m), "v"((u32)(m >> 32))
+      : "vcc");
+  return ((u64)r1 << 32) | r0;
+}
+
 __device__ __forceinline__ u64 csub(u64 x, u64 m) {
   const u64 t = x + (0 - m);  // one 64-bit add, no carry chain
   return x >= m ? t : x;

@@ -21,17 +21,20 @@ struct NttCfg {
 };
 
 // ---------------------------------------------------------------- butterflies
-// Forward Cooley-Tukey, lazy: X, Y in [0, 4q) -> [0, 4q).
+// Forward Cooley-Tukey, lazy: X, Y in [0, 4q) -> [0, 4q).  WS: twiddles are
+// wave-uniform.
+template <bool WS = false>
 __device__ __forceinline__ void ct_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
-  const u64 x = csub(X, q2);
-  const u64 t = shoup_lazy(Y, w, wp, q);
+  const u64 x = csub_bf(X, q2);
+  const u64 t = shoup_lazy_bf<WS>(Y, w, wp, (u64)0 - q);
   X = x + t;
   Y = x + q2 - t;
 }
 // Inverse Gentleman-Sande, lazy: X, Y in [0, 2q) -> [0, 2q).
+template <bool WS = false>
 __device__ __forceinline__ void gs_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
-  const u64 s = csub(X + Y, q2);
-  Y = shoup_lazy(X + q2 - Y, w, wp, q);
+  const u64 s = csub_bf(X + Y, q2);
+  Y = shoup_lazy_bf<WS>(X + q2 - Y, w, wp, (u64)0 - q);
   X = s;
 }
 
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << u) + (k >> (C::A - u));
-      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+      ct_bfly<true>(x[k], x[k + half], tw[idx], tws[idx], q, q2);
     }
   }
   u64* L = lds + ll * C::LDS_LINE;
@@ -342,7 +345,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << u) + (k >> (C::A - u));
-      gs_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+      gs_bfly<true>(x[k], x[k + half], tw[idx], tws[idx], q, q2);
     }
   }
   {  // stage 0 with N^-1: X' = (X+Y) N^-1, Y' = (X-Y) psi^-brev(1) N^-1
