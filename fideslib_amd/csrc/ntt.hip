@@ -17,7 +17,15 @@ struct NttCfg {
   static constexpr int LPW = LPW0 < NLINES ? LPW0 : NLINES;  // lines per workgroup
   static constexpr int THREADS = LPW * TPL;
   static constexpr int S0 = PASS == 0 ? 0 : L1;     // first global stage
-  static constexpr int LDS_LINE = M + 1;            // padded line stride (u64)
+  // LDS line layout.  Column pass: lines are columns, consecutive threads sit
+  // in consecutive lines, stride M + 1.  Row pass: consecutive threads share a
+  // line, so phase B (thread r reads 2^B consecutive elements) would put lanes
+  // 2^B u64 apart, an 8-way bank conflict at B = 4; one pad u64 after every
+  // 2^B elements (index i + (i >> B)) spreads them over all banks, and the
+  // stride M + M/2^B (16 mod 32 u64 at N = 2^16) keeps neighbouring lines on
+  // disjoint banks for both access patterns.
+  static constexpr int LDS_LINE = PASS == 0 ? M + 1 : M + (M >> B);
+  __device__ static constexpr u32 lx(u32 i) { return PASS == 0 ? i : i + (i >> B); }
 };
 
 // ---------------------------------------------------------------- butterflies
@@ -175,14 +183,14 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
   }
   u64* L = lds + ll * C::LDS_LINE;
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = x[k];
+  for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = x[k];
   __syncthreads();
   constexpr int EB = 1 << C::B;
 #pragma unroll
   for (int g = 0; g < C::G; ++g) {
     const u32 r1 = r0 * C::G + g;
 #pragma unroll
-    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[(r1 << C::B) + k];
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[C::lx((r1 << C::B) + k)];
 #pragma unroll
     for (int v = 0; v < C::B; ++v) {
       const int u = C::A + v;
@@ -195,7 +203,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
       }
     }
 #pragma unroll
-    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
+    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
   }
   __syncthreads();
   u64* base = (EPI != EPI_NONE && tk.out) ? tk.out : data;
@@ -208,7 +216,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) {
       const u32 c = r0 + C::TPL * k;
       const u32 j = (row << C::L2) | c;
-      u64 v = csub(csub(L[c], q2), q);
+      u64 v = csub(csub(L[C::lx(c)], q2), q);
       v = epilogue<EPI>(v, j, tk, q);
       const u32 jr = __brev(j) >> (32 - LOGN);
       const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
@@ -220,7 +228,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
 #pragma unroll
   for (int k = 0; k < C::E; ++k) {
     const u32 c = r0 + C::TPL * k;
-    u64 v = csub(csub(L[c], q2), q);
+    u64 v = csub(csub(L[C::lx(c)], q2), q);
     v = epilogue<EPI>(v, (row << C::L2) | c, tk, q);
     dst[c] = v;
   }
@@ -274,11 +282,11 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
       }
     }
 #pragma unroll
-    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
+    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) x[k] = L[r0 + C::TPL * k];
+  for (int k = 0; k < C::E; ++k) x[k] = L[C::lx(r0 + C::TPL * k)];
 #pragma unroll
   for (int u = C::A - 1; u >= 0; --u) {
     const int half = C::E >> (u + 1);
