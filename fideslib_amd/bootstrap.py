@@ -1,0 +1,327 @@
+"""CKKS bootstrapping built from the batched device operations.
+
+Algorithm (the reference's Bootstrap, src/CKKS/Bootstrap.cu:169-333, on
+precomputation it takes from OpenFHE; here every table is derived from the
+mathematics and computed on the host):
+
+  1. prescale   x <- x * 2^-c, Rescale, keep one limb (emulates a larger q_0
+                so the sine below is near-linear; undone by an exact integer
+                multiply by 2^c at the end, as the reference's correction factor)
+  2. ModRaise   one limb at q_0 -> L limbs: the ciphertext now encrypts
+                t = m + q_0 I(X) with a small integer polynomial I
+  3. CoeffToSlot  the inverse of the decoding map, so that slot k holds the
+                complex-packed coefficients t_k + i t_{k+n} (bit-reversed slot
+                order), scaled to y = t / (q_0 K')
+  4. split      Re = ct + conj(ct), Im = (ct - conj(ct)) * (-i)  (conj = the
+                automorphism 2N-1, -i = the monomial X^{3N/2})
+  5. EvalMod    cos(2 pi (K' y - 1/4) / 2^r) by a Chebyshev series, then r
+                double-angle steps c <- 2c^2 - 1, giving sin(2 pi t/q_0) ~
+                2 pi m / q_0 on Re and Im (reference approxModReduction,
+                src/CKKS/ApproxModEval.cu)
+  6. recombine  Re + i Im  (i = the monomial X^{N/2})
+  7. SlotToCoeff  the decoding map, scaled back by q_0 / (2 pi s)
+  8. x 2^c      exact integer multiply
+
+Decoding map: z_j = sum_k w_k zeta^(5^j k) for the complex-packed
+coefficients w.  It factors as U = B_logn ... B_1 BR (BR the bit reversal,
+B_s the butterfly stage of length 2^s with twiddles exp(2 pi i (5^j mod 2^(s+2))
+/ 2^(s+2))).  The bit reversal is dropped on both sides (EvalMod is
+slot-wise), and the stages are merged into `budget` groups (the reference's
+level budget {3, 3}); a group of b stages has at most 2^(b+1) - 1 non-zero
+diagonals, all multiples of one stride.  Each group is one level: a
+baby-step / giant-step evaluation (hoisted baby rotations, plaintext dot
+products of up to `g` terms, giant rotations, one Rescale), with plaintexts
+encoded at scale q_top so the ciphertext scale is unchanged.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import codec
+from .serve import Plaintext, Program, Reg, chebyshev_coefficients, chebyshev_eval
+
+
+# ----------------------------------------------------------------- linear algebra on diagonals
+def bitrev(n: int) -> np.ndarray:
+    b = n.bit_length() - 1
+    r = np.zeros(n, np.int64)
+    for i in range(n):
+        r[i] = int(format(i, "0%db" % b)[::-1], 2) if b else 0
+    return r
+
+
+def rot(v: np.ndarray, d: int) -> np.ndarray:
+    """Slot rotation: rot(v, d)[p] = v[(p + d) mod n] (a Rotate by d)."""
+    return np.roll(v, -d)
+
+
+def stage_diags(n: int, N: int, s: int, inverse: bool) -> dict:
+    """Butterfly stage of length 2^s as {offset mod n: diagonal}."""
+    ln = 1 << s
+    lenh = ln >> 1
+    lenq = ln << 2
+    M = 2 * N
+    j = np.arange(n) % ln
+    top = j < lenh
+    jj = np.where(top, j, j - lenh)
+    om = np.exp(2j * np.pi * (np.array([pow(5, int(x), M) for x in range(lenh)]) % lenq) / lenq)[jj]
+    d0 = np.zeros(n, complex)
+    dp = np.zeros(n, complex)
+    dm = np.zeros(n, complex)
+    if not inverse:
+        # top: y = x + w x[+lenh];  bottom: y = x[-lenh] - w x
+        d0[top], dp[top] = 1.0, om[top]
+        d0[~top], dm[~top] = -om[~top], 1.0
+    else:
+        # top: x = (y + y[+lenh]) / 2;  bottom: x = conj(w) (y[-lenh] - y) / 2
+        d0[top], dp[top] = 0.5, 0.5
+        d0[~top], dm[~top] = -0.5 * np.conj(om[~top]), 0.5 * np.conj(om[~top])
+    out = {}
+    for off, v in ((0, d0), (lenh % n, dp), ((-lenh) % n, dm)):
+        out[off] = out.get(off, 0) + v
+    return out
+
+
+def diag_mul(A: dict, B: dict, n: int) -> dict:
+    """Diagonals of the product A B (apply B first)."""
+    out = {}
+    for e, a in A.items():
+        for d, b in B.items():
+            k = (d + e) % n
+            out[k] = out.get(k, 0) + a * rot(b, e)
+    return {k: v for k, v in out.items() if np.max(np.abs(v)) > 1e-300}
+
+
+def diag_apply(D: dict, x: np.ndarray) -> np.ndarray:
+    y = np.zeros_like(x, dtype=complex)
+    for d, v in D.items():
+        y = y + v * rot(x, d)
+    return y
+
+
+def split_stages(logn: int, budget: int) -> list:
+    """Stages 1..logn into `budget` consecutive groups, larger groups first."""
+    budget = max(1, min(budget, logn))
+    base, extra = divmod(logn, budget)
+    sizes = [base + (1 if i < extra else 0) for i in range(budget)]
+    out, s = [], 1
+    for sz in sizes:
+        out.append(list(range(s, s + sz)))
+        s += sz
+    return out
+
+
+def transform_groups(n: int, N: int, budget: int, inverse: bool) -> list:
+    """Group matrices in application order.  Forward (SlotToCoeff without
+    the bit reversal): B_1..B_logn; inverse (CoeffToSlot): B_logn^-1..B_1^-1."""
+    logn = n.bit_length() - 1
+    groups = split_stages(logn, budget)
+    mats = []
+    for g in groups:
+        stages = g if not inverse else list(reversed(g))
+        M = None
+        for s in stages:
+            S = stage_diags(n, N, s, inverse)
+            M = S if M is None else diag_mul(S, M, n)
+        mats.append(M)
+    return mats if not inverse else list(reversed(mats))
+
+
+# ----------------------------------------------------------------- baby-step / giant-step plan
+@dataclass
+class GroupPlan:
+    stride: int
+    g: int                                   # baby steps
+    babies: list                             # b with rotation b*stride
+    giants: dict = field(default_factory=dict)   # k -> [(b, diagonal for rotation k*g*stride)]
+    n: int = 0
+
+    def rotations(self) -> set:
+        r = {(b * self.stride) % self.n for b in self.babies if b}
+        r |= {(k * self.g * self.stride) % self.n for k in self.giants if k}
+        return r
+
+
+def plan_group(D: dict, n: int, factor: complex = 1.0) -> GroupPlan:
+    offs = sorted(D)
+    stride = 0
+    for o in offs:
+        stride = math.gcd(stride, o)
+    stride = stride or n
+    Mn = n // stride
+    ms = sorted(((o // stride + Mn // 2) % Mn) - Mn // 2 for o in offs)   # signed multiples
+    span = ms[-1] - ms[0] + 1
+    g = 1 << max(0, math.ceil(math.log2(math.sqrt(span))))
+    g = min(g, 16)
+    P = GroupPlan(stride, g, [], {}, n)
+    babies = set()
+    for m in ms:
+        k, b = divmod(m, g)
+        diag = D[(m * stride) % n] * factor
+        # y = sum_k rot(sum_b rot(diag_m, -k g s) * rot(x, b s), k g s)
+        P.giants.setdefault(k, []).append((b, rot(diag, -k * g * stride)))
+        babies.add(b)
+    P.babies = sorted(babies)
+    return P
+
+
+def bsgs_apply(P: GroupPlan, x: np.ndarray) -> np.ndarray:
+    """The plan evaluated on plain vectors (for checking)."""
+    xb = {b: rot(x, b * P.stride) for b in P.babies}
+    y = np.zeros_like(x, dtype=complex)
+    for k, terms in P.giants.items():
+        inner = sum(v * xb[b] for b, v in terms)
+        y = y + rot(inner, k * P.g * P.stride)
+    return y
+
+
+# ----------------------------------------------------------------- the bootstrap plan
+@dataclass
+class BootConfig:
+    budget: tuple = (3, 3)      # (CoeffToSlot, SlotToCoeff) levels
+    degree: int = 55            # Chebyshev degree of the cosine
+    double_angle: int = 3       # r
+    K: float = 32.0             # |I| + |m/q_0| bound of the argument
+    correction: int = 10        # prescale 2^-c
+
+
+class BootstrapPlan:
+    """Tables and device plaintexts for bootstrapping full-slot ciphertexts
+    of one parameter set; `program()` builds the request program.
+
+    ctx: a capi.Context (plaintexts are uploaded and prepared on it), or None
+    to plan without a device (keep_host=True then keeps the plaintext
+    coefficients for host-side checking)."""
+
+    def __init__(self, prm, ctx=None, cfg: BootConfig = BootConfig(), keep_host: bool = False):
+        self.prm = prm
+        self.cfg = cfg
+        self.N = prm.N
+        self.n = prm.N // 2
+        self.q = [int(x) for x in prm.q]
+        self.L = len(self.q)
+        self.delta = float(2 ** prm.scale_bits)
+        self.ctx = ctx
+        self.keep_host = keep_host
+        n, N = self.n, self.N
+        cts = transform_groups(n, N, cfg.budget[0], inverse=True)
+        stc = transform_groups(n, N, cfg.budget[1], inverse=False)
+        q0 = float(self.q[0])
+        # CoeffToSlot output y = t / (q_0 K'), with the 1/2 of the Re/Im split folded in
+        self.f_cts = self.delta / (2.0 * q0 * cfg.K)
+        # SlotToCoeff input sin(2 pi t/q_0) ~ 2 pi m/q_0  ->  m / s1 with s1 ~ delta
+        self.f_stc = q0 / (2.0 * math.pi * self.delta)
+        self.cts_plans = [plan_group(D, n, self.f_cts if i == 0 else 1.0) for i, D in enumerate(cts)]
+        self.stc_plans = [plan_group(D, n, self.f_stc if i == 0 else 1.0) for i, D in enumerate(stc)]
+        self.cheb = chebyshev_coefficients(
+            lambda y: math.cos(2 * math.pi * (cfg.K * y - 0.25) / 2 ** cfg.double_angle), cfg.degree, -1.0, 1.0)
+        # levels
+        self.l_raise = self.L
+        self.depth = (cfg.budget[0] + cfg.budget[1] + max(1, math.ceil(math.log2(cfg.degree))) + 1
+                      + cfg.double_angle)
+        if self.l_raise - self.depth < 1:
+            raise ValueError("not enough levels for bootstrapping: need %d, have %d" % (self.depth + 1, self.L))
+        self._pt_cache = {}
+        self._pts = None
+
+    # -- rotation indices every program uses (slot rotations, mod n); conjugation separately
+    def rotations(self) -> list:
+        r = set()
+        for P in self.cts_plans + self.stc_plans:
+            r |= P.rotations()
+        r.discard(0)
+        return sorted(r)
+
+    def output_level(self) -> int:
+        return self.l_raise - self.depth
+
+    # -- plaintexts
+    def _encode(self, values, level: int, scale: float):
+        coeffs = codec.encode(values, scale, self.N)
+        rns = codec.to_rns(coeffs, self.q[:level])
+        return self._upload(rns, level, scale)
+
+    def _upload(self, rns, level, scale):
+        host = rns if (self.keep_host or self.ctx is None) else None
+        buf = None
+        if self.ctx is not None:
+            from . import capi
+            buf = capi.DeviceBuffer.from_array(rns)
+            self.ctx.plaintext_prepare(buf, level)
+        return Plaintext(buf if buf is not None else 0, level, scale, host)
+
+    def _monomial(self, power: int, level: int):
+        """X^power mod X^N + 1 as an exact plaintext (scale 1)."""
+        c = np.zeros(self.N, np.int64)
+        sign = 1
+        p = power % (2 * self.N)
+        if p >= self.N:
+            p -= self.N
+            sign = -1
+        c[p] = sign
+        return self._upload(codec.to_rns(c, self.q[:level]), level, 1.0)
+
+    def build_plaintexts(self) -> None:
+        """Encode every diagonal of every group at the level it is used."""
+        if self._pts is not None:
+            return
+        pts = {"cts": [], "stc": []}
+        l = self.l_raise
+        for P in self.cts_plans:
+            pts["cts"].append(self._group_pts(P, l))
+            l -= 1
+        l_split = l
+        self._minus_i = self._monomial(3 * self.N // 2, l_split)
+        l -= max(1, math.ceil(math.log2(self.cfg.degree))) + 1 + self.cfg.double_angle
+        self._plus_i = self._monomial(self.N // 2, l)
+        for P in self.stc_plans:
+            pts["stc"].append(self._group_pts(P, l))
+            l -= 1
+        self._pts = pts
+
+    def _group_pts(self, P: GroupPlan, level: int) -> dict:
+        scale = float(self.q[level - 1])      # Rescale after the products divides it out
+        return {k: [(b, self._encode(v, level, scale)) for b, v in terms] for k, terms in P.giants.items()}
+
+    # -- program pieces
+    def _linear(self, Pg: Program, x: Reg, plan: GroupPlan, pts: dict) -> Reg:
+        babies = {b: (x if b == 0 else Pg.rotate(x, b * plan.stride)) for b in plan.babies}
+        parts = []
+        for k in sorted(pts):
+            inner = Pg.ptdot([(babies[b], pt) for b, pt in pts[k]])
+            parts.append(inner if k == 0 else Pg.rotate(inner, k * plan.g * plan.stride))
+        return Pg.rescale(Pg.sum_tree(parts))
+
+    def _evalmod(self, Pg: Program, y: Reg) -> Reg:
+        c = chebyshev_eval(Pg, y, self.cheb)
+        for _ in range(self.cfg.double_angle):
+            sq = Pg.hmul(c, c)
+            c = Pg.lin(sq, mul=2, add=-int(round(sq.scale)))
+        return c
+
+    def program(self, level_in: int, scale_in: float | None = None) -> Program:
+        """Bootstrap of one full-slot ciphertext at `level_in` limbs (>= 2)."""
+        self.build_plaintexts()
+        cfg = self.cfg
+        Pg = Program(self.q, self.prm.logN, self.prm.scale_bits, "bootstrap")
+        x = Pg.input(level_in, scale_in)
+        x = Pg.mul_real(x, 2.0 ** -cfg.correction)
+        r = Pg.mod_raise(Pg.drop(x, 1), self.l_raise, self.delta)
+        for plan, pts in zip(self.cts_plans, self._pts["cts"]):
+            r = self._linear(Pg, r, plan, pts)
+        cj = Pg.conjugate(r)
+        re = Pg.add(r, cj)
+        im = Pg.ptdot([(Pg.sub(r, cj), self._minus_i)])
+        re = self._evalmod(Pg, re)
+        im = self._evalmod(Pg, im)
+        w = Pg.add(re, Pg.ptdot([(im, self._plus_i)]))
+        for plan, pts in zip(self.stc_plans, self._pts["stc"]):
+            w = self._linear(Pg, w, plan, pts)
+        out = Pg.lin(w, mul=1 << cfg.correction)
+        # value = the input message: the nominal scale of the result is the
+        # tracked one (the constants above are folded relative to delta)
+        Pg.set_output(out)
+        return Pg

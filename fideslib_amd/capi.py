@@ -71,6 +71,10 @@ SIGNATURES = {
     "fides_rotate_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _u32p,
                                   ctypes.POINTER(_vp), _vp]),
     "fides_rotate_hoisted": (_int, [_vp, _vp, _u32, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_plaintext_prepare": (_int, [_vp, _vp, _u32, _vp]),
+    "fides_ptdot_batch": (_int, [_vp, _u32, _u32p, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p,
+                                 ctypes.POINTER(_vp), _vp]),
+    "fides_mod_raise_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
                                ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
@@ -255,6 +259,29 @@ class Context:
         ki = np.ascontiguousarray(kidx, np.uint32)
         check(load().fides_rotate_hoisted(self.h, _ptr(inp), level, n, K, ki.ctypes.data_as(_u32p), O, stream),
               "fides_rotate_hoisted")
+
+    def plaintext_prepare(self, pt, level, stream=None):
+        check(load().fides_plaintext_prepare(self.h, _ptr(pt), level, stream), "fides_plaintext_prepare")
+
+    def ptdot_batch(self, terms, levels, out, stream=None):
+        """out[i] = sum_j ct_ij (*) pt_ij for terms[i] = [(ct, pt), ...] (prepared plaintexts)."""
+        n = len(levels)
+        cnt = np.ascontiguousarray([len(t) for t in terms], np.uint32)
+        flat = [x for t in terms for x in t]
+        C = (_vp * len(flat))(*[_ptr(c) for c, _ in flat])
+        P = (_vp * len(flat))(*[_ptr(p) for _, p in flat])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_ptdot_batch(self.h, n, cnt.ctypes.data_as(_u32p), C, P, lv.ctypes.data_as(_u32p), O,
+                                       stream), "fides_ptdot_batch")
+
+    def mod_raise_batch(self, inp, l_out, out, stream=None):
+        n = len(l_out)
+        I = (_vp * n)(*[_ptr(x) for x in inp])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lo = np.ascontiguousarray(l_out, np.uint32)
+        check(load().fides_mod_raise_batch(self.h, n, I, lo.ctypes.data_as(_u32p), O, stream),
+              "fides_mod_raise_batch")
 
     def rescale_batch(self, inp, levels, out, stream=None):
         n = len(levels)

@@ -1208,12 +1208,20 @@ static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32
                         const u32* kidx, u64* const* out, hipStream_t st) {
   const u32 N = c->N;
   size_t ksw = 0;
+  // Rotations of the same ciphertext share one ModUp (hoisting); the results
+  // are bit-identical to separate rotations.
+  std::vector<int> share(n, -1);
   for (u32 i = 0; i < n; ++i) {
     RCHK(check_key(c, keys[i], lv[i]));
     if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     if (in[i] == out[i]) return fail(FIDES_ERR_ARG, "rotate output must not alias its input");
     if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
-    ksw += ks_words(c, lv[i]);
+    for (u32 j = 0; j < i; ++j)
+      if (in[j] == in[i] && lv[j] == lv[i] && share[j] < 0) {
+        share[i] = (int)j;
+        break;
+      }
+    ksw += ks_words(c, lv[i], share[i] >= 0);
   }
   RCHK(ensure_arena(c, ksw, st));
   std::vector<KsJob> jobs;
@@ -1222,6 +1230,7 @@ static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32
     KsJob J{in[i] + S, lv[i], keys[i], out[i], out[i] + S, in[i], nullptr};
     // the ModDown store applies the permutation (scatter with k^-1)
     J.perm = inv_odd(kidx[i], 2 * N);
+    J.share = share[i];
     jobs.push_back(J);
   }
   Blob blob;
@@ -1458,6 +1467,107 @@ extern "C" int fides_rescale_batch(fides_context* c, uint32_t n, const uint64_t*
   int slot;
   RCHK(upload_blob(c, blob, st, &dev, &slot));
   RCHK(rs_launch(c, dev, R, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+// ======================================================================
+// plaintexts and modulus raising (bootstrap building blocks)
+// ======================================================================
+extern "C" int fides_plaintext_prepare(fides_context* c, uint64_t* pt, uint32_t l, void* stream) {
+  if (!c || !pt) return fail(FIDES_ERR_ARG, "null argument");
+  if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<uint32_t> pids(l);
+  for (u32 i = 0; i < l; ++i) pids[i] = i;
+  RCHK(raw_transform(c, pt, pids.data(), l, false, st));
+  fides_launch_to_mont(pt, c->prm, c->logN, l, l, st);
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+extern "C" int fides_ptdot_batch(fides_context* c, uint32_t n, const uint32_t* cnt, const uint64_t* const* ct,
+                                 const uint64_t* const* pt, const uint32_t* l, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!cnt || !ct || !pt || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<PtDotTask> tasks;
+  std::vector<PtTerm> terms;
+  size_t o = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (cnt[i] == 0 || cnt[i] > PTDOT_MAXT) return fail(FIDES_ERR_ARG, "term count must be 1..16");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    for (u32 j = 0; j < cnt[i]; ++j)
+      if (!ct[o + j] || !pt[o + j]) return fail(FIDES_ERR_ARG, "null term");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 t = 0; t < l[i]; ++t) {
+        PtDotTask tk;
+        tk.out = c->limb(out[i], (size_t)p * l[i] + t);
+        tk.pid = t;
+        tk.cnt = cnt[i];
+        tk.first = (u32)terms.size();
+        tk.pad = 0;
+        for (u32 j = 0; j < cnt[i]; ++j)
+          terms.push_back(PtTerm{c->limb(ct[o + j], (size_t)p * l[i] + t), c->limb(pt[o + j], t)});
+        tasks.push_back(tk);
+      }
+    o += cnt[i];
+  }
+  if (tasks.empty()) return FIDES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  Blob blob;
+  const size_t ot = blob.add(tasks);
+  const size_t om = blob.add(terms);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  for (u32 x = 0; x < tasks.size(); x += 65535)
+    fides_launch_ptdot_batch(at<PtDotTask>(dev, ot) + x, at<PtTerm>(dev, om), std::min<u32>(65535, (u32)tasks.size() - x),
+                             c->prm, c->logN, st);
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+extern "C" int fides_mod_raise_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l_out,
+                                     uint64_t* const* out, void* stream) {
+  if (!c || (n && (!in || !l_out || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (n == 0) return FIDES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const u32 N = c->N;
+  for (u32 i = 0; i < n; ++i) {
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l_out[i] == 0 || l_out[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    if (out[i] < in[i] + 2 * (size_t)N && in[i] < out[i] + 2 * (size_t)l_out[i] * N)
+      return fail(FIDES_ERR_ARG, "mod raise output must not overlap its input");
+  }
+  RCHK(ensure_arena(c, 2 * (size_t)N * n, st));
+  std::vector<NttTask> intt, ntt;
+  const u64 q0 = c->primes[0];
+  for (u32 i = 0; i < n; ++i) {
+    u64* tmp = c->arena + (size_t)2 * N * i;
+    for (u32 p = 0; p < 2; ++p) {
+      NttTask t = ntask(tmp + (size_t)p * N, 0);
+      t.src = c->limb(in[i], p);
+      intt.push_back(t);
+      for (u32 j = 0; j < l_out[i]; ++j) {
+        NttTask u = ntask(c->limb(out[i], (size_t)p * l_out[i] + j), j);
+        u.src = tmp + (size_t)p * N;
+        u.src_pid = 0;
+        u.swdiff = c->primes[j] - q0 % c->primes[j];
+        ntt.push_back(u);
+      }
+    }
+  }
+  std::stable_sort(ntt.begin(), ntt.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
+  Blob blob;
+  const size_t oi = blob.add(intt);
+  const size_t on = blob.add(ntt);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(launch_intt_list(c, at<NttTask>(dev, oi), (u32)intt.size(), st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, on), (u32)ntt.size(), PRO_SWITCH_MODULUS, EPI_NONE, st));
   RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;

@@ -62,6 +62,22 @@ struct AxpyTask {
   u32 pad;
 };
 
+// One output limb of sum_j x_j * m_j (mod q_pid), m_j in Montgomery form
+// (plaintext limbs prepared by fides_plaintext_prepare); terms[first ..
+// first + cnt) hold the operand limbs, cnt <= 16.
+struct PtTerm {
+  const u64* x;
+  const u64* m;
+};
+struct PtDotTask {
+  u64* out;
+  u32 pid;
+  u32 cnt;
+  u32 first;
+  u32 pad;
+};
+constexpr u32 PTDOT_MAXT = 16;
+
 // One limb of out = (a * m or a) (+ or - b) + add  (mod q_pid).
 struct LinTask {
   const u64* a;
@@ -87,6 +103,8 @@ void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm,
                             hipStream_t st);
 void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
+void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
+                              hipStream_t st);
 void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);

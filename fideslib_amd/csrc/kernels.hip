@@ -148,7 +148,34 @@ __global__ __launch_bounds__(256) void k_lin_batch(const LinTask* __restrict__ t
   t.out[k] = add_mod(v, t.add, q);
 }
 
+// ---------------------------------------------------------------- plaintext products
+// Sum of up to 16 element-wise ciphertext-limb x plaintext-limb products with
+// one Montgomery reduction (the reference's plaintext dot products of the
+// linear transforms, ElemenwiseBatchKernels.cu:1007-1107).  x < q and
+// m < q: the 128-bit sum stays below 16 q^2 < q 2^64.
+__global__ __launch_bounds__(256) void k_ptdot_batch(const PtDotTask* __restrict__ tasks,
+                                                     const PtTerm* __restrict__ terms,
+                                                     const PrimeC* __restrict__ prm, u32 logN) {
+  const PtDotTask t = tasks[blockIdx.y];
+  const u32 k = blockIdx.x * 256u + threadIdx.x;
+  const PrimeC P = prm[t.pid];
+  Acc128 a;
+  for (u32 j = 0; j < t.cnt; ++j) {
+    const PtTerm tm = terms[t.first + j];
+    acc_mac_v(a, tm.x[k], tm.m[k]);
+  }
+  u64 h, l;
+  acc_value(a, h, l);
+  t.out[k] = mont_redc(h, l, P);
+}
+
 // ---------------------------------------------------------------- launchers
+void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
+                              hipStream_t st) {
+  if (ntasks == 0) return;
+  k_ptdot_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, terms, prm, logN);
+}
+
 void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
   if (ntasks == 0) return;
   k_lin_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);

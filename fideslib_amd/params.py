@@ -148,3 +148,9 @@ def serve_small() -> ParamSet:
     """N=2^12, one 60-bit + eleven 50-bit Q primes, dnum 3: deep enough for
     the degree-15 Chebyshev request at small N (request-loop tests)."""
     return ckks_params(12, 60, 50, 11, dnum=3, name="serve_N2^12_L12_dnum3")
+
+
+def boot_small() -> ParamSet:
+    """N=2^12 with the headline prime sizes (60 + 21x59 bits, dnum 3): deep
+    enough for a full bootstrap at small N (bootstrap tests)."""
+    return ckks_params(12, 60, 59, 21, dnum=3, name="boot_N2^12_L22_dnum3")

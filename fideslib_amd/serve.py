@@ -46,7 +46,9 @@ ROT = "rot"            # key switch + automorphism
 LIN = "lin"            # out = a*m (+/-) b + c
 RESCALE = "rescale"
 DROP = "drop"          # keep the first l_out limbs
-KIND_ORDER = (HMUL, HMUL_NR, ROT, LIN, RESCALE, DROP)
+PTDOT = "ptdot"        # sum_j ct_j (*) pt_j with device plaintexts
+MODRAISE = "modraise"  # one limb at q_0 -> l_out limbs (centred lift)
+KIND_ORDER = (MODRAISE, HMUL, HMUL_NR, ROT, PTDOT, LIN, RESCALE, DROP)
 KEYSWITCH_KINDS = (HMUL, HMUL_NR, ROT)
 
 
@@ -64,6 +66,22 @@ class Reg:
     @property
     def words(self) -> int:
         return 2 * self.level
+
+
+class Plaintext:
+    """A device plaintext prepared for fides_ptdot_batch: evaluation form,
+    Montgomery representation, `level` limbs, encoded at `scale`.  `host`
+    optionally keeps the coefficient-form residues (for checking)."""
+
+    def __init__(self, buf, level: int, scale: float, host=None):
+        self.buf = buf
+        self.level = level
+        self.scale = scale
+        self.host = host
+
+    @property
+    def ptr(self) -> int:
+        return self.buf if isinstance(self.buf, int) else self.buf.ptr
 
 
 @dataclass
@@ -153,6 +171,27 @@ class Program:
         srcs = [a] if b is None else [a, b]
         return self._emit(LIN, out, srcs, a.level, (mul, add, bool(sub)))
 
+    def ptdot(self, terms) -> Reg:
+        """sum_j a_j (*) pt_j for (register, Plaintext) pairs at one level."""
+        if not 1 <= len(terms) <= 16:
+            raise ServeError("a plaintext dot product takes 1..16 terms")
+        l = min(a.level for a, _ in terms)
+        regs = [self.drop(a, l) for a, _ in terms]
+        pts = tuple(p for _, p in terms)
+        if any(p.level < l for p in pts):
+            raise ServeError("plaintext below the ciphertext level")
+        scale = regs[0].scale * pts[0].scale
+        return self._emit(PTDOT, self._new(l, scale), regs, l, (pts,))
+
+    def mod_raise(self, a: Reg, level: int, scale: float) -> Reg:
+        """One-limb ciphertext -> `level` limbs; `scale` is the nominal scale
+        declared for the raised ciphertext."""
+        if a.level != 1:
+            a = self.drop(a, 1)
+        if not 1 <= level <= len(self.q):
+            raise ServeError("raise level out of range")
+        return self._emit(MODRAISE, self._new(level, scale), [a], 1, (level,))
+
     def add(self, a: Reg, b: Reg) -> Reg:
         return self.lin(a, b=b)
 
@@ -166,6 +205,19 @@ class Program:
         """a * x for a real constant: integer multiply by round(x*delta), then Rescale."""
         t = self.lin(a, mul=int(round(x * self.delta)), scale=a.scale * self.delta)
         return self.rescale(t)
+
+    def match_scale(self, a: Reg, scale: float, level: int) -> Reg:
+        """Same value at `scale` (to 2^-60 relative) and `level` < a.level:
+        integer multiply by round(scale q / s_a) and Rescale.  In fixed
+        scaling every Rescale squares a scale's deviation from q, so
+        registers reached along different paths disagree; this aligns them
+        at the cost of one of the source's spare levels."""
+        if a.level <= level:
+            raise ServeError("match_scale needs a spare level")
+        q = self.q[a.level - 1]
+        m = int(round(scale * q / a.scale))
+        t = self.lin(a, mul=m, scale=a.scale * m)
+        return self.drop(self.rescale(t), level)
 
     def rescale(self, a: Reg) -> Reg:
         if a.level < 2:
@@ -219,13 +271,20 @@ def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
     """
     P = Program(prm.q, prm.logN, prm.scale_bits, "chebyshev%d" % (len(coeffs) - 1))
     x = P.input(level)
-    deg = len(coeffs) - 1
     # y = a*x + b with a = 2/(hi-lo), b = -(hi+lo)/(hi-lo)
     a = 2.0 / (hi - lo)
     b = -(hi + lo) / (hi - lo)
     y = P.mul_real(x, a)
     if b != 0.0:
         y = P.add_real(y, b)
+    P.set_output(chebyshev_eval(P, y, coeffs))
+    return P
+
+
+def chebyshev_eval(P: Program, y: Reg, coeffs) -> Reg:
+    """sum_k c_k T_k(y) for a register whose values lie in [-1, 1]; uses
+    ceil(log2 deg) + 1 levels (see chebyshev_program)."""
+    deg = len(coeffs) - 1
     T = {1: y}
     m = 1
     while m < deg:
@@ -234,20 +293,24 @@ def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
             prod = P.hmul(T[m], T[n])
             if n == m:  # T_2m = 2 T_m^2 - 1
                 T[k] = P.lin(prod, mul=2, add=-int(round(prod.scale)))
-            else:       # T_{m+n} = 2 T_m T_n - T_{m-n}
-                T[k] = P.lin(prod, mul=2, b=T[m - n], sub=True)
+            else:       # T_{m+n} = 2 T_m T_n - T_{m-n}, T_{m-n} brought to the product's scale
+                b = T[m - n]
+                if b.scale != prod.scale and b.level > prod.level:
+                    b = P.match_scale(b, prod.scale, prod.level)
+                T[k] = P.lin(prod, mul=2, b=b, sub=True)
         m *= 2
     low = min(T[k].level for k in T)
+    # every term at one scale: each T_k's own scale is folded into its constant
+    target = P.delta * T[deg].scale
     terms = []
     for k in range(1, deg + 1):
         if coeffs[k] == 0.0:
             continue
         t = P.drop(T[k], low)
-        terms.append(P.lin(t, mul=int(round(coeffs[k] * P.delta)), scale=t.scale * P.delta))
+        terms.append(P.lin(t, mul=int(round(coeffs[k] * target / t.scale)), scale=target))
     s = P.sum_tree(terms)
     s = P.lin(s, add=int(round(coeffs[0] * s.scale)))
-    P.set_output(P.rescale(s))
-    return P
+    return P.rescale(s)
 
 
 def rotsum_program(prm, level: int, k: int) -> Program:
@@ -429,6 +492,12 @@ class GpuBackend:
     def drop(self, a, l_in, l_out, out):
         self.ctx.drop_limbs_batch(a, l_in, l_out, out)
 
+    def ptdot(self, terms, levels, out):
+        self.ctx.ptdot_batch(terms, levels, out)
+
+    def modraise(self, a, l_out, out):
+        self.ctx.mod_raise_batch(a, l_out, out)
+
     def upload(self, ptr, host):
         arr = np.ascontiguousarray(host, np.uint64)
         self.capi.check(self.capi.load().fides_copy_to_device(ptr, arr.ctypes.data, arr.size * 8, None), "upload")
@@ -564,6 +633,10 @@ class Server:
                 be.rescale(s0, lv, dst)
             elif kind == DROP:
                 be.drop(s0, lv, [o.arg[0] for _, o in g], dst)
+            elif kind == PTDOT:
+                be.ptdot([[(ptr(a, r), p.ptr) for r, p in zip(o.srcs, o.arg[0])] for a, o in g], lv, dst)
+            elif kind == MODRAISE:
+                be.modraise(s0, [o.arg[0] for _, o in g], dst)
             self.stats["launches"] += 1
             if kind in KEYSWITCH_KINDS:
                 self.stats["keyswitch_ops"] += len(g)

@@ -164,7 +164,8 @@ int fides_hmult_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a
                       const uint32_t* l, const fides_key* rlk, int rescale, uint64_t* const* d_out, void* stream);
 
 /* n rotations/automorphisms, ciphertext i with its own key and index.
- * Identical to n calls of fides_rotate. */
+ * Identical to n calls of fides_rotate.  Entries that rotate the same input
+ * (same pointer and level) share one ModUp, as fides_rotate_hoisted does. */
 int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
                        const fides_key* const* keys, const uint32_t* auto_index, uint64_t* const* d_out,
                        void* stream);
@@ -216,6 +217,31 @@ int fides_lin_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, 
  * src/CKKS/Ciphertext.cpp:229-233).  in == out only when l_out == l_in. */
 int fides_drop_limbs_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l_in,
                            const uint32_t* l_out, uint64_t* const* d_out, void* stream);
+
+/* ---- plaintexts and modulus raising (bootstrap building blocks) ---- */
+
+/* In place: a plaintext [l][N] given as coefficient-form residues mod
+ * q_0..q_{l-1} becomes evaluation form in Montgomery representation, the
+ * form fides_ptdot_batch consumes (the reference keeps its linear-transform
+ * plaintexts on the device in evaluation form, src/CKKS/Plaintext.cu). */
+int fides_plaintext_prepare(fides_context* ctx, uint64_t* d_pt, uint32_t l, void* stream);
+
+/* Plaintext dot products: out[i] = sum_{j < cnt[i]} ct_ij (*) pt_ij, both
+ * ciphertext components multiplied element-wise by the same plaintext
+ * (reference dotProductLtBatchedPt, src/CKKS/ElemenwiseBatchKernels.cu:1007;
+ * a single term is EvalMult(ct, pt), api/CryptoContext.hpp:150).
+ * ct_terms / pt_terms list the terms of output 0, then output 1, ...
+ * (sum of cnt entries); ct [2][l[i]][N], pt prepared [l[i]][N];
+ * 1 <= cnt[i] <= 16.  out[i] may alias one of its own ct terms. */
+int fides_ptdot_batch(fides_context* ctx, uint32_t n, const uint32_t* cnt, const uint64_t* const* ct_terms,
+                      const uint64_t* const* pt_terms, const uint32_t* l, uint64_t* const* d_out, void* stream);
+
+/* ModRaise: a ciphertext at one limb ([2][1][N], modulus q_0) is lifted to
+ * l_out[i] limbs: each coefficient's centred representative mod q_0 is
+ * reduced mod q_0..q_{l_out-1} (reference ModRaise, src/CKKS/Bootstrap.cu:393;
+ * the lift is OpenFHE's SwitchModulus, src/CKKS/Rescale.cuh:113-133). */
+int fides_mod_raise_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l_out,
+                          uint64_t* const* d_out, void* stream);
 
 /* Keyless evaluation-domain automorphism of nlimbs limbs (permutation only,
  * reference automorph_ src/Rotation.cu:10-28).  d_out must not alias. */

@@ -484,6 +484,28 @@ OR_API void or_rescale_poly(const or_ctx* c, u64* a, uint32_t l) {
   free(t);
 }
 
+/* ModRaise of a one-limb ciphertext [2][1][N] (modulus q_0, evaluation
+ * form) to l_out limbs: INTT under q_0, SwitchModulus (centred lift) of each
+ * coefficient to q_i, NTT under q_i (reference ModRaise,
+ * src/CKKS/Bootstrap.cu:393-560, which raises c0/c1 the same way). */
+OR_API void or_mod_raise(const or_ctx* c, const u64* in, uint32_t l_out, u64* out) {
+  const uint32_t N = c->N;
+  const u64 q0 = c->primes[0];
+  for (uint32_t p = 0; p < 2; ++p) {
+    u64* t = (u64*)malloc(sizeof(u64) * N);
+    memcpy(t, in + (size_t)p * N, sizeof(u64) * N);
+    or_intt(t, c->logN, q0, c->ipsi_rev);
+#pragma omp parallel for schedule(static)
+    for (uint32_t i = 0; i < l_out; ++i) {
+      const u64 qi = c->primes[i];
+      u64* o = out + ((size_t)p * l_out + i) * N;
+      for (uint32_t k = 0; k < N; ++k) o[k] = or_switch_modulus(t[k], q0, qi);
+      or_ntt(o, c->logN, qi, c->psi_rev + (size_t)i * N);
+    }
+    free(t);
+  }
+}
+
 /* Ciphertext [2][l][N] -> [2][l-1][N] (packed), both components. */
 OR_API void or_rescale_ct(const or_ctx* c, const u64* in, uint32_t l, u64* out) {
   const size_t N = c->N;

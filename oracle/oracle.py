@@ -65,6 +65,7 @@ def lib() -> ctypes.CDLL:
             "or_rescale_table": (None, [vp, i, u64p]),
             "or_rescale_poly": (None, [vp, u64p, i]),
             "or_rescale_ct": (None, [vp, u64p, i, u64p]),
+            "or_mod_raise": (None, [vp, u64p, i, u64p]),
             "or_automorph": (None, [vp, u64p, u64p, i, i]),
             "or_hmult_relin": (None, [vp, u64p, u64p, i, u64p, u64p]),
             "or_rotate": (None, [vp, u64p, i, u64p, i, u64p]),
@@ -257,6 +258,12 @@ class Context:
         lib().or_keygen_switch(self._h, _p64(_u64(s_new)), _p64(_u64(s_old)), _p64(_u64(a)), _p64(_u64(e)),
                                _p64(key))
         return key
+
+    def mod_raise(self, ct: np.ndarray, l_out: int) -> np.ndarray:
+        """[2][1][N] at q_0 -> [2][l_out][N] (centred lift of each coefficient)."""
+        out = np.zeros((2, l_out, self.N), np.uint64)
+        lib().or_mod_raise(self._h, _p64(_u64(ct)), l_out, _p64(out))
+        return out
 
     def mul_limbs(self, x, y, pids):
         pid = np.ascontiguousarray(np.asarray(pids, np.uint32))

@@ -4,10 +4,16 @@ import numpy as np
 from fideslib_amd import serve as S
 
 
-def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys):
+def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys, keep=None):
     """The program, op by op, on the CPU oracle."""
     N = octx.N
     val = {r.id: np.asarray(x, np.uint64) for r, x in zip(P.inputs, inputs)}
+    pt_eval = {}
+
+    def plain(pt, l):
+        if id(pt) not in pt_eval:
+            pt_eval[id(pt)] = octx.ntt_limbs(np.ascontiguousarray(pt.host, np.uint64), list(range(pt.level)))
+        return pt_eval[id(pt)][:l]
     for o in P.ops:
         x = [val[s.id] for s in o.srcs]
         l = o.level
@@ -21,6 +27,16 @@ def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys):
             y = octx.rescale_ct(x[0], l)
         elif o.kind == S.DROP:
             y = np.ascontiguousarray(x[0][:, :o.arg[0]])
+        elif o.kind == S.MODRAISE:
+            y = octx.mod_raise(x[0], o.arg[0])
+        elif o.kind == S.PTDOT:
+            pid = list(range(l)) * 2
+            y = None
+            for ct, pt in zip(x, o.arg[0]):
+                m = plain(pt, l)
+                t = octx.mul_limbs(ct.reshape(2 * l, N), np.concatenate([m, m]), pid)
+                y = t if y is None else octx.add_limbs(y, t, pid)
+            y = y.reshape(2, l, N)
         elif o.kind == S.LIN:
             mul, add, sub = o.arg
             pid = list(range(l)) * 2
@@ -34,4 +50,6 @@ def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys):
                 y = octx.add_limbs(y, c, pid)
             y = y.reshape(2, l, N)
         val[o.dst.id] = y
+    if keep is not None:
+        keep.update(val)
     return val[P.output.id]

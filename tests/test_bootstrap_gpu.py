@@ -1,0 +1,107 @@
+"""Bootstrapping on the GPU through the request loop.
+
+* N = 2^12 (22 limbs, sparse secret h = 64): three bootstraps at different
+  input levels served together with a Chebyshev request; every result is
+  bit-exact against the oracle running the same program, bit-identical to
+  the same request served alone, and decrypts to the message (> 20 bits).
+* N = 2^16 (30 limbs, h = 192, the headline primes): two bootstraps served
+  together decrypt to their messages; the precision is printed.
+"""
+import time
+
+import numpy as np
+import pytest
+
+from fideslib_amd import bootstrap as B
+from fideslib_amd import capi, params
+from fideslib_amd import serve as S
+from oracle import ckks_codec as C
+from oracle import oracle as O
+from serve_oracle import run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def _keys(bp, keys, N):
+    rot = {}
+    for r in bp.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        rot[k] = keys.rotation_key(k)
+    return rot
+
+
+def test_bootstrap_small_equals_oracle_and_alone():
+    prm = params.boot_small()
+    N = prm.N
+    say("bootstrap N=2^12")
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 5, h=64)
+    bp = B.BootstrapPlan(prm, gctx, keep_host=True)
+    rlk = keys.relin_key()
+    rot = _keys(bp, keys, N)
+    be = S.GpuBackend(gctx, gctx.upload_key(rlk), {k: gctx.upload_key(v) for k, v in rot.items()}, 1 << 30)
+    rng = np.random.default_rng(11)
+    work = []
+    for i, lvl in enumerate((3, 6, prm.L)):
+        z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+        work.append((bp.program(lvl), keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), lvl), z))
+    coef = S.chebyshev_coefficients(lambda x: np.tanh(x / 4), 15, -8, 8)
+    xz = rng.uniform(-8, 8, N // 2)
+    work.append((S.chebyshev_program(prm, 12, coef, -8, 8),
+                 keys.encrypt(C.encode(xz, 2.0 ** prm.scale_bits, N), 12), None))
+    reqs = [S.Request(i, P, 0.0, inputs=[ct], keep_result=True) for i, (P, ct, _) in enumerate(work)]
+    srv = S.Server(be, N)
+    srv.run(reqs, realtime=False)
+    say("mix: %d steps, %d launches, %d ops" % (srv.stats["steps"], srv.stats["launches"], srv.stats["ops"]))
+    for r, (P, ct, z) in zip(reqs, work):
+        assert r.status == "done"
+        ref = run_oracle(octx, P, [ct], rlk, rot)
+        assert np.array_equal(r.result, ref), r.rid
+        if z is not None:
+            dec = C.decode(keys.decrypt_coef(r.result, P.output.level), P.output.scale, N)
+            bits = C.precision_bits(dec, z)
+            say("bootstrap request %d: %.1f bits" % (r.rid, bits))
+            assert bits > 20, bits
+    alone = S.Request(100, work[1][0], 0.0, inputs=[work[1][1]], keep_result=True)
+    S.Server(be, N).run([alone], realtime=False)
+    assert np.array_equal(alone.result, reqs[1].result)
+
+
+def test_bootstrap_cfg3_precision():
+    prm = params.config3()
+    N = prm.N
+    say("bootstrap N=2^16: keys")
+    t0 = time.time()
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 192, h=192)
+    bp = B.BootstrapPlan(prm, gctx)
+    gk = {}
+    for r in bp.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        gk[k] = gctx.upload_key(keys.rotation_key(k))
+    be = S.GpuBackend(gctx, gctx.upload_key(keys.relin_key()), gk, 8 << 30)
+    say("keys and plan %.1f s; plaintexts" % (time.time() - t0))
+    bp.build_plaintexts()
+    rng = np.random.default_rng(12)
+    work = []
+    for lvl in (3, 17):
+        z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+        work.append((bp.program(lvl), keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), lvl), z))
+    reqs = [S.Request(i, P, 0.0, inputs=[ct], keep_result=True) for i, (P, ct, _) in enumerate(work)]
+    say("serving 2 bootstraps")
+    srv = S.Server(be, N)
+    srv.run(reqs, realtime=False)
+    say("2 bootstraps: %.3f s wall, %d key switches" % (srv.stats["wall_s"], srv.stats["keyswitch_ops"]))
+    for r, (P, ct, z) in zip(reqs, work):
+        assert r.status == "done"
+        assert P.output.level == prm.L - bp.depth
+        dec = C.decode(keys.decrypt_coef(r.result, P.output.level), P.output.scale, N)
+        bits = C.precision_bits(dec, z)
+        say("N=2^16 bootstrap from level %d: %.1f bits of precision" % (P.inputs[0].level, bits))
+        assert bits > 15, bits

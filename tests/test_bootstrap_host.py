@@ -1,0 +1,112 @@
+"""Bootstrap tables and program, on the CPU.
+
+* The merged special-FFT groups equal the decoding map z_j = sum_k w_k
+  zeta^(5^j k) (checked against the definition) and its inverse.
+* Each baby-step/giant-step plan evaluates its group exactly.
+* The EvalMod polynomial (Chebyshev series of the shifted cosine plus
+  double-angle steps) equals sin(2 pi u) in FP64.
+* The whole bootstrap program, run op by op on the oracle with real keys
+  (N = 2^12, sparse secret), decrypts to the input message; a plausible
+  mistake (an argument bound K' below the actual |I|) is caught.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fideslib_amd import bootstrap as B
+from fideslib_amd import params
+from fideslib_amd import serve as S
+
+
+def decode_def(w, N):
+    n = N // 2
+    zeta = np.exp(1j * np.pi / N)
+    return np.array([np.sum(w * zeta ** ((pow(5, j, 2 * N) * np.arange(n)) % (2 * N))) for j in range(n)])
+
+
+@pytest.mark.parametrize("N,budget", [(64, 3), (256, 2), (4096, 3)])
+def test_groups_equal_decoding_map(N, budget):
+    n = N // 2
+    rng = np.random.default_rng(N)
+    w = rng.normal(size=n) + 1j * rng.normal(size=n)
+    z = decode_def(w, N)
+    br = B.bitrev(n)
+    x = w[br]
+    for D in B.transform_groups(n, N, budget, inverse=False):
+        assert len(D) <= 2 ** (math.ceil(math.log2(n) / budget) + 1) - 1
+        x = B.diag_apply(D, x)
+    assert np.abs(x - z).max() < 1e-9 * np.abs(z).max()
+    y = z
+    for D in B.transform_groups(n, N, budget, inverse=True):
+        y = B.diag_apply(D, y)
+    assert np.abs(y - w[br]).max() < 1e-9 * np.abs(w).max()
+
+
+def test_bsgs_plans_evaluate_groups():
+    N = 4096
+    n = N // 2
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    for inv in (False, True):
+        for D in B.transform_groups(n, N, 3, inv):
+            P = B.plan_group(D, n, factor=0.25)
+            assert max(len(t) for t in P.giants.values()) <= 16
+            assert np.abs(B.bsgs_apply(P, x) - 0.25 * B.diag_apply(D, x)).max() < 1e-12 * np.abs(x).max() * 64
+
+
+def test_evalmod_polynomial_fp64():
+    cfg = B.BootConfig()
+    bp = B.BootstrapPlan(params.boot_small(), None, cfg)
+    eps = np.linspace(-1e-3, 1e-3, 201)
+    worst = 0.0
+    for I in range(-20, 21):
+        u = I + eps
+        v = np.polynomial.chebyshev.chebval(u / cfg.K, bp.cheb)
+        for _ in range(cfg.double_angle):
+            v = 2 * v * v - 1
+        worst = max(worst, np.abs(v - np.sin(2 * np.pi * u)).max())
+    assert worst < 1e-11
+
+
+def test_plan_levels_and_rotations():
+    bp = B.BootstrapPlan(params.config3(), None)
+    assert bp.depth == 3 + 3 + 6 + 1 + 3
+    assert bp.output_level() == 30 - bp.depth
+    assert 0 < len(bp.rotations()) <= 48
+    with pytest.raises(ValueError):
+        B.BootstrapPlan(params.serve_small(), None)   # 12 limbs: too shallow
+
+
+def _oracle_bootstrap(cfg, seed=3):
+    from oracle import ckks_codec as C
+    from oracle import oracle as O
+    from serve_oracle import run_oracle
+    prm = params.boot_small()
+    N = prm.N
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 5, h=64)
+    bp = B.BootstrapPlan(prm, None, cfg, keep_host=True)
+    P = bp.program(3)
+    rot = {}
+    for r in bp.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        rot[k] = keys.rotation_key(k)
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    ct = keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), 3)
+    out = run_oracle(octx, P, [ct], keys.relin_key(), rot)
+    dec = C.decode(keys.decrypt_coef(out, P.output.level), P.output.scale, N)
+    return C.precision_bits(dec, z), P
+
+
+def test_bootstrap_on_the_oracle_decrypts():
+    bits, P = _oracle_bootstrap(B.BootConfig())
+    assert P.output.level == params.boot_small().L - B.BootstrapPlan(params.boot_small(), None).depth
+    assert bits > 20, bits
+
+
+def test_bootstrap_planted_mistake_is_caught():
+    # K' = 4 cannot cover |I| up to ~8 for this secret: the modular reduction fails
+    bits, _ = _oracle_bootstrap(B.BootConfig(K=4.0, degree=31))
+    assert bits < 5, bits
