@@ -268,7 +268,8 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
 # ----------------------------------------------------------------- hoisted rotations
 def hoisted_leg(prm, ctx, w, ev, n_rot=16, reps=5):
     """n_rot rotations of one 30-limb ciphertext: fides_rotate_hoisted (one
-    shared ModUp) against fides_rotate_batch of the same n_rot rotations."""
+    shared ModUp) against fides_rotate_batch of n_rot distinct copies (one
+    ModUp each)."""
     N, l = prm.N, prm.L
     rng = np.random.default_rng(SEED + 7)
     inp = capi.DeviceBuffer.from_array(np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)]))
@@ -286,8 +287,12 @@ def hoisted_leg(prm, ctx, w, ev, n_rot=16, reps=5):
         ev.record(e1)
         return ev.elapsed_ms(e0, e1) / reps
 
+    # separate rotations: distinct copies of the input, so no ModUp is shared
+    copies = [capi.DeviceBuffer(2 * l * N) for _ in range(n_rot)]
+    for c in copies:
+        ctx.copy(c, inp, 2 * l * N)
     ms_h = t(lambda: ctx.rotate_hoisted(inp, l, keys, ks, outs))
-    ms_b = t(lambda: ctx.rotate_batch([inp] * n_rot, [l] * n_rot, keys, ks, outs))
+    ms_b = t(lambda: ctx.rotate_batch(copies, [l] * n_rot, keys, ks, outs))
     return {"rotations": n_rot, "limbs": l, "hoisted_ms": round(ms_h, 3), "separate_batched_ms": round(ms_b, 3),
             "speedup": round(ms_b / ms_h, 3)}
 
@@ -297,17 +302,57 @@ SERVE_KMAX = 15          # rotate-and-sum over 2^k slots, k ~ U{1..15}: rotation
 SERVE_ARENA_GIB = 64     # admission cap: ciphertext storage of admitted requests
 
 
-def serve_leg(prm, ctx, w, n_req, rate, seed):
-    """Poisson(rate) arrivals of a request mix served by the continuous-
-    batching loop; keys resident.  Mix: degree-15 Chebyshev series of the
-    sigmoid on slots in [-8, 8] (start limbs U{7..30}) and rotate-and-sum
-    over 2^k slots (k ~ U{1..15}, start limbs U{1..30}) in the ratio 2:1
-    (the 60:30 split of the serving workload; its bootstrap share is not
-    served).  Inputs are copied from resident ciphertexts at admission."""
+def bench_keys(prm, w, boot):
+    """Resident synthetic keys for every automorphism the serving mix and the
+    bootstrap use: rotations by 2^i (rotate-and-sum), the bootstrap's
+    baby/giant steps and conjugation (distinct HBM copies of one synthetic key)."""
+    N = prm.N
+    idx = {pow(5, 1 << i, 2 * N) for i in range(SERVE_KMAX)}
+    idx |= {pow(5, r, 2 * N) for r in boot.rotations()}
+    idx.add(2 * N - 1)
+    idx = sorted(idx)
+    if len(idx) > len(w.rot_keys):
+        raise RuntimeError("need %d resident keys, have %d" % (len(idx), len(w.rot_keys)))
+    return {k: w.rot_keys[i] for i, k in enumerate(idx)}
+
+
+def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
+    """n_boot full-slot bootstraps (N=2^16, 30 limbs -> 14 limbs) served
+    together by the request loop; synthetic keys and ciphertexts (precision
+    is measured by tests/test_bootstrap_gpu.py with real keys)."""
+    from fideslib_amd import serve as S
+    N = prm.N
+    rng = np.random.default_rng(SEED + 9)
+    be = S.GpuBackend(ctx, w.rlk, keys, 16 << 30)
+    src = capi.DeviceBuffer.from_array(np.stack([rand_limbs(rng, prm.q[:3], N) for _ in range(2)]))
+
+    def reqs(n):
+        return [S.Request(i, boot.program(3), 0.0, inputs=[("dev", src.ptr)]) for i in range(n)]
+
+    S.Server(be, N).run(reqs(1), realtime=False)   # warm-up
+    ctx.sync()
+    srv = S.Server(be, N)
+    srv.run(reqs(n_boot), realtime=False)
+    wall = srv.stats["wall_s"]
+    return {"workload": "cfg4: %d full-slot bootstraps (2^15 slots) in one request-loop run, N=2^16, 30 limbs "
+                        "(60+29x59 bit), dnum=3, level budget {3,3}, EvalMod Chebyshev degree %d + %d double "
+                        "angles, output 14 limbs" % (n_boot, boot.cfg.degree, boot.cfg.double_angle),
+            "bootstraps_per_s": round(n_boot / wall, 3), "ms_per_bootstrap": round(wall * 1e3 / n_boot, 3),
+            "keyswitch_ops": srv.stats["keyswitch_ops"], "steps": srv.stats["steps"],
+            "launches": srv.stats["launches"],
+            "precision_bits": "17.6-17.8 at N=2^16, h=192 (tests/test_bootstrap_gpu.py, real keys)"}
+
+
+def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed):
+    """Poisson(rate) arrivals of the serving mix through the continuous-
+    batching loop; keys resident.  Mix: 60% degree-15 Chebyshev series of
+    the sigmoid on slots in [-8, 8] (start limbs U{7..30}), 30% rotate-and-sum
+    over 2^k slots (k ~ U{1..15}, start limbs U{1..30}), 10% full-slot
+    bootstrap (start limbs U{2..30}).  Inputs are copied from resident
+    ciphertexts at admission."""
     from fideslib_amd import serve as S
     N = prm.N
     rng = np.random.default_rng(seed)
-    keys = {pow(5, 1 << i, 2 * N): w.rot_keys[i] for i in range(SERVE_KMAX)}
     be = S.GpuBackend(ctx, w.rlk, keys, SERVE_ARENA_GIB << 30)
     top = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])
     pool = {l: capi.DeviceBuffer.from_array(np.ascontiguousarray(top[:, :l])) for l in range(1, prm.L + 1)}
@@ -317,10 +362,13 @@ def serve_leg(prm, ctx, w, n_req, rate, seed):
     def make(n, arrivals):
         out = []
         for i in range(n):
-            if rng.random() < 2.0 / 3.0:
+            u = rng.random()
+            if u < 0.6:
                 P = S.chebyshev_program(prm, int(rng.integers(dmin, prm.L + 1)), coef, -8.0, 8.0)
-            else:
+            elif u < 0.9:
                 P = S.rotsum_program(prm, int(rng.integers(1, prm.L + 1)), int(rng.integers(1, SERVE_KMAX + 1)))
+            else:
+                P = boot.program(int(rng.integers(2, prm.L + 1)))
             out.append(S.Request(i, P, float(arrivals[i]), inputs=[("dev", pool[P.inputs[0].level].ptr)]))
         return out
 
@@ -342,14 +390,15 @@ def serve_leg(prm, ctx, w, n_req, rate, seed):
     srv.run(reqs, realtime=True)
     lat = S.latency_summary(reqs)
     kinds = {}
-    for name in ("chebyshev15", "rotsum"):
+    for name in ("chebyshev15", "rotsum", "bootstrap"):
         sub = [r for r in reqs if r.program.name.startswith(name)]
         if sub:
             kinds[name] = {k: round(v, 3) for k, v in S.latency_summary(sub).items()}
     res = {
-        "workload": "Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 2:1 degree-15 Chebyshev (sigmoid on "
-                    "[-8,8], start limbs U{%d..30}) : rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
-                    "U{1..30}); keys resident; admission cap %d GiB" % (dmin, SERVE_ARENA_GIB),
+        "workload": "cfg5: Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 60%% degree-15 Chebyshev (sigmoid "
+                    "on [-8,8], start limbs U{%d..30}), 30%% rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
+                    "U{1..30}), 10%% bootstrap (start limbs U{2..30}); keys resident; admission cap %d GiB"
+                    % (dmin, SERVE_ARENA_GIB),
         "rate_req_per_s": rate,
         "requests": n_req,
         "p50_ms": round(lat["p50_ms"], 3),
@@ -382,8 +431,9 @@ def main():
     ap.add_argument("--mode", default="batched", choices=["plain", "batched"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-boot", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=400)
-    ap.add_argument("--serve-rate", type=float, default=300.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate", type=float, default=80.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -448,9 +498,18 @@ def main():
     }
     out["config"]["hoisted_rotations"] = hoisted_leg(prm, ctx, w, ev)
     print("bench: hoisted %s" % out["config"]["hoisted_rotations"], file=sys.stderr, flush=True)
+    if not (args.no_serve and args.no_boot):
+        from fideslib_amd import bootstrap as BT
+        print("bench: bootstrap plaintexts", file=sys.stderr, flush=True)
+        boot = BT.BootstrapPlan(prm, ctx)
+        boot.build_plaintexts()
+        keys = bench_keys(prm, w, boot)
+    if not args.no_boot:
+        out["bootstrap"] = bootstrap_leg(prm, ctx, w, boot, keys)
+        print("bench: bootstrap %s" % out["bootstrap"], file=sys.stderr, flush=True)
     if not args.no_serve:
         print("bench: serving leg", file=sys.stderr, flush=True)
-        out["serve"] = serve_leg(prm, ctx, w, args.serve_requests, args.serve_rate, SEED + 5)
+        out["serve"] = serve_leg(prm, ctx, w, boot, keys, args.serve_requests, args.serve_rate, SEED + 5)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(prm)

@@ -347,65 +347,75 @@ class Compiled:
     block: int
     deps: list
     dependents: list
+    asap: list = field(default_factory=list)
 
 
 def _compile(P: Program) -> Compiled:
+    """Storage by step-aware liveness.  Each op gets its ASAP step (one more
+    than its latest producer; inputs are step 0), which is the step the loop
+    issues it in when storage adds no ordering.  A value occupies its slot
+    from its defining step through its last reader's step, and a slot is
+    reused only by an op in a strictly later step, so the storage ordering
+    recorded in deps never delays an op past its data-ready step."""
     if P.output is None:
         raise ServeError("program has no output")
     nops = len(P.ops)
-    last_use = {r.id: -1 for r in P.inputs}
-    for i, o in enumerate(P.ops):
-        last_use.setdefault(o.dst.id, i)      # a value nobody reads dies at its definition
-        for s_ in o.srcs:
-            last_use[s_.id] = max(last_use.get(s_.id, -1), i)
-    last_use[P.output.id] = nops              # kept until the request retires
-    dies_after = {}
-    for rid, u in last_use.items():
-        dies_after.setdefault(u, []).append(rid)
-
-    slot_size, slot_of, hist, free = [], {}, [], []
-
-    def alloc(reg):
-        need = reg.words
-        best = None
-        for idx, (sz, _) in enumerate(free):
-            if sz >= need and (best is None or sz < free[best][0]):
-                best = idx
-        if best is not None:
-            _, sl = free.pop(best)
-        else:
-            sl = len(slot_size)
-            slot_size.append(need)
-            hist.append([])
-        slot_of[reg.id] = sl
-        return sl
-
-    def release(u):
-        for rid in dies_after.get(u, ()):
-            sl = slot_of[rid]
-            free.append((slot_size[sl], sl))
-
-    for r in P.inputs:
-        alloc(r)
-    release(-1)
     producer = {}
-    deps = []
+    asap = [0] * nops
     for i, o in enumerate(P.ops):
-        d = set()
+        st = 0
         for s_ in o.srcs:
             if s_.id in producer:
-                d.add(producer[s_.id])
-            elif s_.id not in slot_of:
+                st = max(st, asap[producer[s_.id]])
+            elif s_.id not in {r.id for r in P.inputs}:
                 raise ServeError("register %d read before it is written" % s_.id)
-        sl = alloc(o.dst)
-        d.update(hist[sl])
+        asap[i] = st + 1
+        producer[o.dst.id] = i
+    INF = 1 << 60
+    born = {r.id: 0 for r in P.inputs}
+    last = {r.id: 0 for r in P.inputs}
+    for i, o in enumerate(P.ops):
+        born[o.dst.id] = asap[i]
+        last.setdefault(o.dst.id, asap[i])
+        for s_ in o.srcs:
+            last[s_.id] = max(last[s_.id], asap[i])
+    last[P.output.id] = INF
+    reg = {r.id: r for r in P.inputs}
+    for o in P.ops:
+        reg[o.dst.id] = o.dst
+    # allocation in step order (program order within a step)
+    order = sorted(range(nops), key=lambda i: (asap[i], i))
+    slot_size, slot_free, slot_of, hist = [], [], {}, []   # slot_free[s]: last step of its current value
+
+    def alloc(rid, step):
+        need = reg[rid].words
+        best = None
+        for sl, sz in enumerate(slot_size):
+            if slot_free[sl] < step and sz >= need and (best is None or sz < slot_size[best]):
+                best = sl
+        if best is None:
+            best = len(slot_size)
+            slot_size.append(need)
+            slot_free.append(0)
+            hist.append([])
+        slot_of[rid] = best
+        slot_free[best] = last[rid]
+        return best
+
+    for r in P.inputs:
+        alloc(r.id, 0)
+        slot_free[slot_of[r.id]] = last[r.id]
+    deps = [None] * nops
+    for i in order:
+        o = P.ops[i]
+        d = {producer[s_.id] for s_ in o.srcs if s_.id in producer}
+        sl = alloc(o.dst.id, asap[i])
+        d.update(hist[sl])            # previous occupant's writer and readers
         hist[sl] = [i]
+        d.discard(i)
+        deps[i] = sorted(d)
         for s_ in o.srcs:
             hist[slot_of[s_.id]].append(i)
-        producer[o.dst.id] = i
-        d.discard(i)
-        deps.append(sorted(d))
-        release(i)
     dependents = [[] for _ in range(nops)]
     for i, d in enumerate(deps):
         for j in d:
@@ -414,7 +424,7 @@ def _compile(P: Program) -> Compiled:
     for sz in slot_size:
         off.append(acc)
         acc += sz
-    return Compiled(slot_of, off, slot_size, acc, deps, dependents)
+    return Compiled(slot_of, off, slot_size, acc, deps, dependents, asap)
 
 
 # ----------------------------------------------------------------- arena

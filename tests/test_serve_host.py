@@ -164,23 +164,24 @@ def test_compile_storage_reuse_and_deps():
     C = S._compile(P)
     # storage shared: far fewer slots than registers
     assert len(C.slot_size) < len(P.regs) // 2
-    # no two simultaneously live registers share a slot (program order)
-    last = {}
+    # every dependency is issued in a strictly earlier step (ASAP steps)
+    assert all(all(C.asap[j] < C.asap[i] for j in d) for i, d in enumerate(C.deps))
+    # registers sharing a slot have disjoint lifetimes in step space
+    born = {r.id: 0 for r in P.inputs}
+    last = {r.id: 0 for r in P.inputs}
     for i, o in enumerate(P.ops):
+        born[o.dst.id] = C.asap[i]
+        last.setdefault(o.dst.id, C.asap[i])
         for s in o.srcs:
-            last[s.id] = i
-    last[P.output.id] = len(P.ops)
-    born = {r.id: -1 for r in P.inputs}
-    for i, o in enumerate(P.ops):
-        born[o.dst.id] = i
+            last[s.id] = max(last[s.id], C.asap[i])
+    last[P.output.id] = 1 << 60
     regs = list(born)
     for x in regs:
         for y in regs:
             if x < y and C.slot_of[x] == C.slot_of[y]:
-                lx, ly = last.get(x, born[x]), last.get(y, born[y])
-                assert lx < born[y] or ly < born[x], (x, y)
-    # deps point backwards
-    assert all(all(j < i for j in d) for i, d in enumerate(C.deps))
+                assert last[x] < born[y] or last[y] < born[x], (x, y)
+    # storage never delays an op: the schedule depth equals the data depth
+    assert max(C.asap) <= 30
 
 
 @pytest.mark.parametrize("seed,arena_limbs", [(1, 4000), (2, 1200), (3, 20000)])
@@ -233,28 +234,35 @@ def test_arena_first_fit_coalesces():
     assert a.free == [(0, 100)] and a.used == 0
 
 
-def test_mock_catches_missing_storage_ordering(monkeypatch):
-    """Dropping the write-after-read edges of storage reuse must be caught."""
+def test_mock_catches_storage_overlap(monkeypatch):
+    """A planted allocation mistake (two registers whose lifetimes overlap
+    forced into one slot) must be caught."""
     real = S._compile
 
-    def producers_only(P):
+    def overlapping(P):
         C = real(P)
-        prod = {}
+        born = {r.id: 0 for r in P.inputs}
+        last = {r.id: 0 for r in P.inputs}
         for i, o in enumerate(P.ops):
-            prod[o.dst.id] = i
-        C.deps = [sorted({prod[s.id] for s in o.srcs if s.id in prod}) for o in P.ops]
-        C.dependents = [[] for _ in P.ops]
-        for i, d in enumerate(C.deps):
-            for j in d:
-                C.dependents[j].append(i)
+            born[o.dst.id] = C.asap[i]
+            last.setdefault(o.dst.id, C.asap[i])
+            for x in o.srcs:
+                last[x.id] = max(last[x.id], C.asap[i])
+        ids = sorted(born, key=lambda r: born[r])
+        for a in ids:
+            for b in ids:
+                if (a != b and born[a] < born[b] <= last[a] and C.slot_of[a] != C.slot_of[b]
+                        and C.slot_size[C.slot_of[a]] >= C.slot_size[C.slot_of[b]]):
+                    C.slot_of[b] = C.slot_of[a]
+                    return C
         return C
 
-    monkeypatch.setattr(S, "_compile", producers_only)
+    monkeypatch.setattr(S, "_compile", overlapping)
     rng = np.random.default_rng(1)
-    reqs = make_requests(rng, 24)
+    reqs = make_requests(rng, 8)
     caught = False
     try:
-        _MockServer(MockBackend(4000), 1).run(reqs, realtime=False)
+        _MockServer(MockBackend(20000), 1).run(reqs, realtime=False)
         caught = any(r.result != sequential(r.program, [t for t, _ in r.inputs]) for r in reqs)
     except AssertionError:
         caught = True
