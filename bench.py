@@ -109,9 +109,12 @@ class Workload:
 
 
 ROOF_LIMBS = 960  # 32 limbs for each of the 30 Q primes: the size of the batched step's NTT launches
+ROOF_REPS = 10
+ROOF_LAUNCHES = 2 + ROOF_REPS  # the roofline's warm-up and timed launches, the first of this size in a run
+ROOF_KERNELS = ("void k_ntt_fwd_col<16, 0>", "void k_ntt_fwd_row<16, 0>")
 
 
-def ntt_roofline(ctx, prm, ev, reps=10):
+def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
     """Forward NTT (column + row pass) of ROOF_LIMBS limbs of N = 2^16,
     grouped by prime as the batched executor issues them, timed with HIP
     events on the stream it runs on.  Algorithmic bytes: one read and one
@@ -227,10 +230,11 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
             out.append("| %s | %d | %.3f | %.2f | %.1f%% |" % (k, len(v), sum(v) / 1e6, sum(v) / len(v) / 1e3,
                                                              100.0 * sum(v) / tot))
         roof = {}
-        for r in tr:
-            if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == roof_threads and "ntt_fwd" in r["Kernel_Name"]:
-                roof.setdefault(r["Kernel_Name"].split("(")[0], []).append(
-                    int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        for r in sorted(tr, key=lambda r: int(r["Start_Timestamp"])):
+            k = r["Kernel_Name"].split("(")[0]
+            if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == roof_threads and k in ROOF_KERNELS:
+                if len(roof.get(k, [])) < ROOF_LAUNCHES:
+                    roof.setdefault(k, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
         out.append("")
         out.append("roofline launch (%d limbs): " % ROOF_LIMBS + ", ".join(
             "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
@@ -240,15 +244,16 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
         def counter(d, name):
             vals = {}
             cal = None
-            for r in rows(d, "*counter_collection.csv"):
+            for r in sorted(rows(d, "*counter_collection.csv"), key=lambda r: int(r["Start_Timestamp"])):
                 if r["Counter_Name"] != name:
                     continue
                 g = int(r["Grid_Size"])
                 v = float(r["Counter_Value"]) * 1024.0
                 if "copyBuffer" in r["Kernel_Name"] and (cal is None or g > cal[0]):
                     cal = (g, v)
-                if g == roof_threads and "ntt_fwd" in r["Kernel_Name"]:
-                    vals.setdefault(r["Kernel_Name"].split("(")[0], []).append(v)
+                k = r["Kernel_Name"].split("(")[0]
+                if g == roof_threads and k in ROOF_KERNELS and len(vals.get(k, [])) < ROOF_LAUNCHES:
+                    vals.setdefault(k, []).append(v)
             return cal, {k: sum(v) / len(v) for k, v in vals.items()}
         cal_bytes = 256 << 20
         (cf, fetch), (cw, write) = counter(fetch_dir, "FETCH_SIZE"), counter(write_dir, "WRITE_SIZE")
@@ -432,6 +437,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-boot", action="store_true")
+    ap.add_argument("--boot-only", action="store_true", help="only the bootstrap leg (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=80.0, help="Poisson arrival rate, requests/s")
     args = ap.parse_args()
@@ -445,6 +451,12 @@ def main():
     print("bench: building workload (64 ciphertexts, 65 keys)", file=sys.stderr, flush=True)
     w = Workload(prm, ctx, rng)
     ev = HipEvents()
+    if args.boot_only:
+        from fideslib_amd import bootstrap as BT
+        boot = BT.BootstrapPlan(prm, ctx)
+        boot.build_plaintexts()
+        print(json.dumps({"bootstrap": bootstrap_leg(prm, ctx, w, boot, bench_keys(prm, w, boot))}))
+        return
 
     def timed(step):
         for _ in range(args.warmup):

@@ -78,6 +78,8 @@ SIGNATURES = {
     "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
                                ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_lincomb_batch": (_int, [_vp, _u32, _u32p, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
+                                   ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_drop_limbs_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_add_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, _int,
                                ctypes.POINTER(_vp), _vp]),
@@ -322,6 +324,31 @@ class Context:
         S = np.ascontiguousarray([1 if x else 0 for x in sub], np.uint32)
         check(load().fides_lin_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), M, C, S.ctypes.data_as(_u32p), O,
                                      stream), "fides_lin_batch")
+
+    def lincomb_batch(self, terms, levels, add, out, stream=None):
+        """out[i] = sum_j m_ij * x_ij + add[i]; terms[i] = [(x, x_level, m), ...] with Python-int
+        constants; add[i] a Python int or None."""
+        n = len(levels)
+        cnt = np.ascontiguousarray([len(t) for t in terms], np.uint32)
+        flat = [(x, xl, m, l) for t, l in zip(terms, levels) for x, xl, m in t]
+        X = (_vp * len(flat))(*[_ptr(x) for x, _, _, _ in flat])
+        XL = np.ascontiguousarray([xl for _, xl, _, _ in flat], np.uint32)
+        keep = []
+        M = (_u64p * len(flat))()
+        for k, (_, _, m, l) in enumerate(flat):
+            r = np.ascontiguousarray([int(m) % self.q[j] for j in range(min(l, self.L))], np.uint64)
+            keep.append(r)
+            M[k] = r.ctypes.data_as(_u64p)
+        A = (_u64p * n)()
+        for i, (a, l) in enumerate(zip(add, levels)):
+            if a is not None:
+                r = np.ascontiguousarray([int(a) % self.q[j] for j in range(min(l, self.L))], np.uint64)
+                keep.append(r)
+                A[i] = r.ctypes.data_as(_u64p)
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_lincomb_batch(self.h, n, cnt.ctypes.data_as(_u32p), X, XL.ctypes.data_as(_u32p), M, A,
+                                         lv.ctypes.data_as(_u32p), O, stream), "fides_lincomb_batch")
 
     def drop_limbs_batch(self, inp, l_in, l_out, out, stream=None):
         n = len(l_in)

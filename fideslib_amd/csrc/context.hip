@@ -1572,3 +1572,65 @@ extern "C" int fides_mod_raise_batch(fides_context* c, uint32_t n, const uint64_
   HIPCHK(hipGetLastError());
   return FIDES_OK;
 }
+
+// ======================================================================
+// linear combination of ciphertexts with integer constants
+// ======================================================================
+// R = 2^64 mod q times m (Montgomery form of m), on the host.
+static u64 to_mont_host(u64 m, u64 q) {
+  const unsigned __int128 r = ((unsigned __int128)1 << 64) % q;
+  return (u64)(((unsigned __int128)m * r) % q);
+}
+
+extern "C" int fides_lincomb_batch(fides_context* c, uint32_t n, const uint32_t* cnt, const uint64_t* const* x,
+                                   const uint32_t* x_l, const uint64_t* const* m_res,
+                                   const uint64_t* const* add_res, const uint32_t* l, uint64_t* const* out,
+                                   void* stream) {
+  if (!c || (n && (!cnt || !x || !x_l || !m_res || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  std::vector<LcTask> tasks;
+  std::vector<LcTerm> terms;
+  size_t o = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (cnt[i] == 0 || cnt[i] > LINCOMB_MAXT) return fail(FIDES_ERR_ARG, "term count must be 1..64");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    const uint64_t* a = add_res ? add_res[i] : nullptr;
+    for (u32 j = 0; j < cnt[i]; ++j) {
+      if (!x[o + j] || !m_res[o + j]) return fail(FIDES_ERR_ARG, "null term");
+      if (x_l[o + j] < l[i] || x_l[o + j] > c->L) return fail(FIDES_ERR_ARG, "term below the output level");
+      for (u32 t = 0; t < l[i]; ++t)
+        if (m_res[o + j][t] >= c->primes[t]) return fail(FIDES_ERR_ARG, "constant residue not reduced");
+    }
+    for (u32 t = 0; t < l[i]; ++t)
+      if (a && a[t] >= c->primes[t]) return fail(FIDES_ERR_ARG, "constant residue not reduced");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 t = 0; t < l[i]; ++t) {
+        LcTask tk;
+        tk.out = c->limb(out[i], (size_t)p * l[i] + t);
+        tk.add = (a && p == 0) ? a[t] : 0;
+        tk.pid = t;
+        tk.cnt = cnt[i];
+        tk.first = (u32)terms.size();
+        tk.pad = 0;
+        for (u32 j = 0; j < cnt[i]; ++j)
+          terms.push_back(
+              LcTerm{c->limb(x[o + j], (size_t)p * x_l[o + j] + t), to_mont_host(m_res[o + j][t], c->primes[t])});
+        tasks.push_back(tk);
+      }
+    o += cnt[i];
+  }
+  if (tasks.empty()) return FIDES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  Blob blob;
+  const size_t ot = blob.add(tasks);
+  const size_t om = blob.add(terms);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  for (u32 x0 = 0; x0 < tasks.size(); x0 += 65535)
+    fides_launch_lincomb_batch(at<LcTask>(dev, ot) + x0, at<LcTerm>(dev, om),
+                               std::min<u32>(65535, (u32)tasks.size() - x0), c->prm, c->logN, st);
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}

@@ -78,6 +78,22 @@ struct PtDotTask {
 };
 constexpr u32 PTDOT_MAXT = 16;
 
+// One output limb of sum_j m_j x_j + add (mod q_pid) with integer constants
+// m_j < q given in Montgomery form (m_j 2^64 mod q); cnt <= 64.
+struct LcTerm {
+  const u64* x;
+  u64 m;
+};
+struct LcTask {
+  u64* out;
+  u64 add;
+  u32 pid;
+  u32 cnt;
+  u32 first;
+  u32 pad;
+};
+constexpr u32 LINCOMB_MAXT = 64;
+
 // One limb of out = (a * m or a) (+ or - b) + add  (mod q_pid).
 struct LinTask {
   const u64* a;
@@ -105,6 +121,8 @@ void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC
 void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
                               hipStream_t st);
+void fides_launch_lincomb_batch(const LcTask* tasks, const LcTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
+                                hipStream_t st);
 void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);

@@ -169,7 +169,38 @@ __global__ __launch_bounds__(256) void k_ptdot_batch(const PtDotTask* __restrict
   t.out[k] = mont_redc(h, l, P);
 }
 
+// ---------------------------------------------------------------- linear combination
+// sum_j m_j x_j + add for up to 64 ciphertext limbs and integer constants
+// (the Chebyshev-series combination; the reference evaluates it with
+// scalar multiplications and additions, src/CKKS/ApproxModEval.cu).  The
+// constants are uniform per task and arrive through scalar loads; one
+// column accumulation and one Montgomery reduction per coefficient
+// (x, m < q < 2^60, 64 terms: the sum stays below 4 q 2^64).
+__global__ __launch_bounds__(256) void k_lincomb_batch(const LcTask* __restrict__ tasks_,
+                                                       const LcTerm* __restrict__ terms_,
+                                                       const PrimeC* __restrict__ prm_, u32 logN) {
+  const CONSTANT_AS LcTask* t = (const CONSTANT_AS LcTask*)(tasks_ + blockIdx.y);
+  const CONSTANT_AS LcTerm* terms = (const CONSTANT_AS LcTerm*)terms_;
+  const CONSTANT_AS PrimeC* prm = (const CONSTANT_AS PrimeC*)prm_;
+  const u32 k = blockIdx.x * 256u + threadIdx.x;
+  PrimeC P;
+  P.q = prm[t->pid].q;
+  P.qneg_inv = prm[t->pid].qneg_inv;
+  const u32 first = t->first, cnt = t->cnt;
+  Acc128 acc;
+  for (u32 j = 0; j < cnt; ++j) acc_mac(acc, terms[first + j].x[k], terms[first + j].m);
+  u64 hi, lo;
+  acc_value(acc, hi, lo);
+  t->out[k] = add_mod(mont_redc(hi, lo, P), t->add, P.q);
+}
+
 // ---------------------------------------------------------------- launchers
+void fides_launch_lincomb_batch(const LcTask* tasks, const LcTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
+                                hipStream_t st) {
+  if (ntasks == 0) return;
+  k_lincomb_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, terms, prm, logN);
+}
+
 void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
                               hipStream_t st) {
   if (ntasks == 0) return;

@@ -48,7 +48,8 @@ RESCALE = "rescale"
 DROP = "drop"          # keep the first l_out limbs
 PTDOT = "ptdot"        # sum_j ct_j (*) pt_j with device plaintexts
 MODRAISE = "modraise"  # one limb at q_0 -> l_out limbs (centred lift)
-KIND_ORDER = (MODRAISE, HMUL, HMUL_NR, ROT, PTDOT, LIN, RESCALE, DROP)
+LINCOMB = "lincomb"    # sum_j m_j x_j + c with integer constants (terms at any level >= the output's)
+KIND_ORDER = (MODRAISE, HMUL, HMUL_NR, ROT, PTDOT, LINCOMB, LIN, RESCALE, DROP)
 KEYSWITCH_KINDS = (HMUL, HMUL_NR, ROT)
 
 
@@ -183,6 +184,15 @@ class Program:
         scale = regs[0].scale * pts[0].scale
         return self._emit(PTDOT, self._new(l, scale), regs, l, (pts,))
 
+    def lincomb(self, terms, add: int | None, scale: float) -> Reg:
+        """sum_j m_j a_j + add for (register, integer) pairs; the result is at
+        the lowest term level with nominal `scale` (terms read in place)."""
+        if not 1 <= len(terms) <= 64:
+            raise ServeError("a linear combination takes 1..64 terms")
+        l = min(a.level for a, _ in terms)
+        regs = [a for a, _ in terms]
+        return self._emit(LINCOMB, self._new(l, scale), regs, l, (tuple(int(m) for _, m in terms), add))
+
     def mod_raise(self, a: Reg, level: int, scale: float) -> Reg:
         """One-limb ciphertext -> `level` limbs; `scale` is the nominal scale
         declared for the raised ciphertext."""
@@ -299,17 +309,11 @@ def chebyshev_eval(P: Program, y: Reg, coeffs) -> Reg:
                     b = P.match_scale(b, prod.scale, prod.level)
                 T[k] = P.lin(prod, mul=2, b=b, sub=True)
         m *= 2
-    low = min(T[k].level for k in T)
-    # every term at one scale: each T_k's own scale is folded into its constant
+    # one fused combination; every term lands on one scale because each
+    # T_k's own scale is folded into its integer constant
     target = P.delta * T[deg].scale
-    terms = []
-    for k in range(1, deg + 1):
-        if coeffs[k] == 0.0:
-            continue
-        t = P.drop(T[k], low)
-        terms.append(P.lin(t, mul=int(round(coeffs[k] * target / t.scale)), scale=target))
-    s = P.sum_tree(terms)
-    s = P.lin(s, add=int(round(coeffs[0] * s.scale)))
+    terms = [(T[k], int(round(coeffs[k] * target / T[k].scale))) for k in range(1, deg + 1) if coeffs[k] != 0.0]
+    s = P.lincomb(terms, add=int(round(coeffs[0] * target)), scale=target)
     return P.rescale(s)
 
 
@@ -505,6 +509,9 @@ class GpuBackend:
     def ptdot(self, terms, levels, out):
         self.ctx.ptdot_batch(terms, levels, out)
 
+    def lincomb(self, terms, levels, add, out):
+        self.ctx.lincomb_batch(terms, levels, add, out)
+
     def modraise(self, a, l_out, out):
         self.ctx.mod_raise_batch(a, l_out, out)
 
@@ -643,6 +650,9 @@ class Server:
                 be.rescale(s0, lv, dst)
             elif kind == DROP:
                 be.drop(s0, lv, [o.arg[0] for _, o in g], dst)
+            elif kind == LINCOMB:
+                be.lincomb([[(ptr(a, r), r.level, m) for r, m in zip(o.srcs, o.arg[0])] for a, o in g], lv,
+                           [o.arg[1] for _, o in g], dst)
             elif kind == PTDOT:
                 be.ptdot([[(ptr(a, r), p.ptr) for r, p in zip(o.srcs, o.arg[0])] for a, o in g], lv, dst)
             elif kind == MODRAISE:

@@ -211,6 +211,20 @@ int fides_lin_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, 
                     const uint32_t* l, const uint64_t* const* mul_res, const uint64_t* const* add_res,
                     const uint32_t* sub, uint64_t* const* d_out, void* stream);
 
+/* Linear combination with integer constants:
+ * out[i] = sum_{j < cnt[i]} m_ij * x_ij + c_i  (c_i added to component 0),
+ * 1 <= cnt[i] <= 64.  Terms are listed output by output (sum of cnt
+ * entries): x_terms[k] a ciphertext stored with x_l[k] limbs per component
+ * (x_l[k] >= l[i]: only its first l[i] limbs are read, so a term at a higher
+ * level needs no limb drop), m_res[k] a host array of l[i] residues of the
+ * constant, add_res[i] a host array of l[i] residues or NULL (add_res may be
+ * NULL).  out[i] is [2][l[i]][N] and must not alias a term.  Fuses the
+ * scalar multiplications and additions of a Chebyshev-series combination
+ * (reference src/CKKS/ApproxModEval.cu, EvalChebyshevSeries). */
+int fides_lincomb_batch(fides_context* ctx, uint32_t n, const uint32_t* cnt, const uint64_t* const* x_terms,
+                        const uint32_t* x_l, const uint64_t* const* m_res, const uint64_t* const* add_res,
+                        const uint32_t* l, uint64_t* const* d_out, void* stream);
+
 /* Keep the first l_out[i] limbs of each component: [2][l_in] -> [2][l_out]
  * (Ciphertext::dropToLevel, src/CKKS/Ciphertext.cpp:1143, RNSPoly::dropToLevel
  * src/CKKS/RNSPoly.cpp:838; used for level alignment before add/mult,

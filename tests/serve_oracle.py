@@ -27,6 +27,18 @@ def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys, keep=None):
             y = octx.rescale_ct(x[0], l)
         elif o.kind == S.DROP:
             y = np.ascontiguousarray(x[0][:, :o.arg[0]])
+        elif o.kind == S.LINCOMB:
+            ms, add = o.arg
+            pid = list(range(l)) * 2
+            y = None
+            for xj, m in zip(x, ms):
+                c = np.stack([np.full(N, m % octx.q[j], np.uint64) for j in range(l)] * 2)
+                t = octx.mul_limbs(np.ascontiguousarray(xj[:, :l]).reshape(2 * l, N), c, pid)
+                y = t if y is None else octx.add_limbs(y, t, pid)
+            if add is not None:
+                c = np.stack([np.full(N, add % octx.q[j], np.uint64) for j in range(l)] + [np.zeros(N, np.uint64)] * l)
+                y = octx.add_limbs(y, c, pid)
+            y = y.reshape(2, l, N)
         elif o.kind == S.MODRAISE:
             y = octx.mod_raise(x[0], o.arg[0])
         elif o.kind == S.PTDOT:

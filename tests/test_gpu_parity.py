@@ -497,3 +497,30 @@ def test_cfg3_rotate_hoisted_equals_batch(cfg3):
     for i in range(len(ks)):
         assert np.array_equal(outs[i].download((2, l, N)), ref[i].download((2, l, N))), i
     assert np.array_equal(outs[2].download((2, l, N)), octx.rotate(c, l, host_key, ks[2]))
+
+
+def test_cfg1_lincomb_mixed_levels(cfg1):
+    """out = sum_j m_j x_j + c with terms stored at levels >= the output's."""
+    prm, octx, gctx, keys = cfg1
+    N = prm.N
+    rng = np.random.default_rng(57)
+    lv_terms = [[4, 3, 2], [4] * 20, [2]]
+    lv_out = [2, 4, 1]
+    xs = [[np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in t] for t in lv_terms]
+    dx = [[capi.DeviceBuffer.from_array(x) for x in t] for t in xs]
+    ms = [[int(rng.integers(-(1 << 62), 1 << 62)) * (1 << int(rng.integers(0, 40))) for _ in t] for t in lv_terms]
+    adds = [5, None, -(1 << 70)]
+    out = [capi.DeviceBuffer(2 * l * N) for l in lv_out]
+    gctx.lincomb_batch([[(dx[i][j], lv_terms[i][j], ms[i][j]) for j in range(len(lv_terms[i]))] for i in range(3)],
+                       lv_out, adds, out)
+    for i, l in enumerate(lv_out):
+        pid = list(range(l)) * 2
+        y = None
+        for x, m in zip(xs[i], ms[i]):
+            c = np.stack([np.full(N, m % prm.q[j], np.uint64) for j in range(l)] * 2)
+            t = octx.mul_limbs(np.ascontiguousarray(x[:, :l]).reshape(2 * l, N), c, pid)
+            y = t if y is None else octx.add_limbs(y, t, pid)
+        if adds[i] is not None:
+            c = np.stack([np.full(N, adds[i] % prm.q[j], np.uint64) for j in range(l)] + [np.zeros(N, np.uint64)] * l)
+            y = octx.add_limbs(y, c, pid)
+        assert np.array_equal(out[i].download((2 * l, N)), y), i

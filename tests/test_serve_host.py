@@ -72,6 +72,34 @@ class MockBackend:
                      lambda v: [hash(("lin", mul[i], add[i], sub[i], v[2 * i], v[2 * i + 1] if b[i] is not None
                                       else None, lv[i])) for i in range(len(a))])
 
+    def lincomb(self, terms, lv, add, out):
+        reads = [(p, xl) for t in terms for p, xl, _ in t]
+        per = [len(t) for t in terms]
+
+        def fn(v):
+            res, o = [], 0
+            for i, t in enumerate(terms):
+                res.append(hash(("lincomb", tuple(m for _, _, m in t), add[i], tuple(v[o:o + per[i]]), lv[i])))
+                o += per[i]
+            return res
+        self._launch_var("lincomb", reads, per, list(zip(out, lv)), fn)
+
+    def _launch_var(self, name, reads, per, writes, fn):
+        wset = [set(range(p, p + 2 * l)) for p, l in writes]
+        rs, o = [], 0
+        for c in per:
+            rs.append([set(range(p, p + 2 * l)) for p, l in reads[o:o + c]])
+            o += c
+        for i in range(len(writes)):
+            for j in range(len(writes)):
+                if i != j:
+                    assert not (wset[i] & wset[j]), "%s: two tasks write the same storage" % name
+                    assert not any(r & wset[i] for r in rs[j]), "%s: task reads what another writes" % name
+        vals = [self._read(p, l) for p, l in reads]
+        for (p, l), t in zip(writes, fn(vals)):
+            self._write(p, l, t)
+        self.calls.append((name, len(writes)))
+
     def rescale(self, a, lv, out):
         self._launch("rescale", list(zip(a, lv)), [(o, l - 1) for o, l in zip(out, lv)],
                      lambda v: [hash(("rescale", v[i], lv[i])) for i in range(len(a))])
@@ -111,6 +139,8 @@ def sequential(P: S.Program, input_tokens):
             val[o.dst.id] = hash(("rescale", v[0], o.level))
         elif o.kind == S.DROP:
             val[o.dst.id] = hash(("drop", v[0], o.level, o.arg[0]))
+        elif o.kind == S.LINCOMB:
+            val[o.dst.id] = hash(("lincomb", o.arg[0], o.arg[1], tuple(v), o.level))
     return val[P.output.id]
 
 
