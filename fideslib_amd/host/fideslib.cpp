@@ -6,11 +6,13 @@
 //   FIXEDAUTO: operands at scale degree 2 are rescaled before a mult and
 //   levels are aligned by dropping limbs (:1483-1518 adjustForAddOrSub /
 //   adjustForMult, FIXED branch).
-// FLEXIBLEAUTO(EXT) needs OpenFHE's per-level real scaling factors and the
-// scalar-multiplication correction of adjustScaleAndLevel (:1376-1455); it is
-// rejected with an error instead of being approximated.
+// FLEXIBLEAUTO(EXT): OpenFHE's per-level real scaling factors come in
+// RawParams (or follow OpenFHE's rule by default) and operands are aligned by
+// the scalar-multiply-and-rescale correction of adjustScaleAndLevel
+// (:1376-1455).
 #include "fideslib/fideslib.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -34,8 +36,6 @@ CiphertextImpl::~CiphertextImpl() {
 
 CryptoContext CryptoContextImpl::Create(const RawParams& P) {
   if (P.moduli.empty() || P.SPECIALmoduli.empty()) throw std::runtime_error("RawParams: empty moduli");
-  if (P.scalingTechnique == FLEXIBLEAUTO || P.scalingTechnique == FLEXIBLEAUTOEXT)
-    throw std::runtime_error("FLEXIBLEAUTO scaling needs OpenFHE's real scaling factors; not supported");
   std::shared_ptr<CryptoContextImpl> c(new CryptoContextImpl());
   c->logN = (uint32_t)P.logN;
   c->N = 1u << P.logN;
@@ -58,6 +58,20 @@ CryptoContext CryptoContextImpl::Create(const RawParams& P) {
   if (c->modReduceFactor.size() < c->L) {
     c->modReduceFactor.resize(c->L);
     for (uint32_t i = 0; i < c->L; ++i) c->modReduceFactor[i] = (double)c->q[i];
+  }
+  c->sfReal = P.ScalingFactorReal;
+  c->sfBig = P.ScalingFactorRealBig;
+  if (c->sfReal.size() < c->L || c->sfBig.size() < c->L) {
+    c->sfReal.assign(c->L, 0.0);
+    c->sfBig.assign(c->L, 0.0);
+    if (c->Flexible()) {
+      c->sfReal[c->L - 1] = (double)c->q[c->L - 1];
+      for (uint32_t l = c->L - 1; l >= 1; --l) c->sfReal[l - 1] = c->sfReal[l] * c->sfReal[l] / (double)c->q[l];
+    } else {
+      const double p = std::ldexp(1.0, (int)std::lround(std::log2((double)c->q[c->L > 1 ? 1 : 0])));
+      for (uint32_t l = 0; l < c->L; ++l) c->sfReal[l] = p;
+    }
+    for (uint32_t l = 0; l < c->L; ++l) c->sfBig[l] = c->sfReal[l] * c->sfReal[l];
   }
   return c;
 }
@@ -191,9 +205,249 @@ Ciphertext CryptoContextImpl::DropToLimbs(const Ciphertext& a, uint32_t limbs) {
   return o;
 }
 
+// ---------------------------------------------------------------- scalars
+// round(operand * scale) as residues mod q_0..q_{limbs-1}: rounded by adding
+// 1/2 and truncating to a 128-bit integer, with magnitudes beyond 2^125
+// divided by 2^k first and multiplied back in RNS (reference
+// ContextData::ElemForEvalMult, src/CKKS/Context.cu:366-463).
+static uint64_t mulmod_host(uint64_t a, uint64_t b, uint64_t q) {
+  return (uint64_t)(((unsigned __int128)a * b) % q);
+}
+static uint64_t pow2mod(int k, uint64_t q) {
+  uint64_t r = 1 % q, b = 2 % q;
+  for (; k > 0; k >>= 1, b = mulmod_host(b, b, q))
+    if (k & 1) r = mulmod_host(r, b, q);
+  return r;
+}
+
+std::vector<uint64_t> CryptoContextImpl::ScalarResidues(uint32_t limbs, double operand, double scale) const {
+  int logApprox = 0;
+  const double mag = std::fabs(operand * scale);
+  if (mag > 0) {
+    const int logSF = (int)std::ceil(std::log2(mag));
+    logApprox = logSF - std::min(logSF, 125);
+  }
+  const __int128 large = (__int128)(operand / std::ldexp(1.0, logApprox) * scale + 0.5);
+  std::vector<uint64_t> r(limbs);
+  for (uint32_t i = 0; i < limbs; ++i) {
+    __int128 x = large % (__int128)q[i];
+    if (x < 0) x += q[i];
+    r[i] = (uint64_t)x;
+    if (logApprox > 0) r[i] = mulmod_host(r[i], pow2mod(logApprox, q[i]), q[i]);
+  }
+  return r;
+}
+
+// The constant of an addition: round(|c| * sf) (64-bit, the reference's
+// ElemForEvalAddOrSub, src/CKKS/Context.cu:468-560), times round(sf) for
+// every scale degree above one, negated for c < 0 (Ciphertext::addScalar,
+// src/CKKS/Ciphertext.cpp:780-798).
+std::vector<uint64_t> CryptoContextImpl::AddScalarResidues(uint32_t limbs, double c, int noiseDeg) const {
+  const uint32_t level = limbs - 1;
+  const bool extTop = technique == FLEXIBLEAUTOEXT && level == L - 1;
+  const double sf = extTop ? sfBig.at(level) : sfReal.at(level);
+  int logApprox = 0;
+  const double mag = std::fabs(c) * sf;
+  if (mag > 0) {
+    const int logSF = (int)std::ceil(std::log2(mag));
+    logApprox = logSF - std::min(logSF, 60);
+  }
+  const uint64_t base = (uint64_t)(std::fabs(c) * sf / std::ldexp(1.0, logApprox) + 0.5);
+  const uint64_t isf = (uint64_t)(sf + 0.5);
+  std::vector<uint64_t> r(limbs);
+  for (uint32_t i = 0; i < limbs; ++i) {
+    uint64_t x = base % q[i];
+    if (logApprox > 0) x = mulmod_host(x, pow2mod(logApprox, q[i]), q[i]);
+    if (!extTop)
+      for (int d = 1; d < noiseDeg; ++d) x = mulmod_host(x, isf % q[i], q[i]);
+    if (c < 0 && x) x = q[i] - x;
+    r[i] = x;
+  }
+  return r;
+}
+
+Ciphertext CryptoContextImpl::MulConst(const Ciphertext& a, const std::vector<uint64_t>& res) {
+  Ciphertext o = NewCiphertext(a->limbs);
+  const uint64_t* in = a->d;
+  uint64_t* out = o->d;
+  const uint64_t* rp = res.data();
+  const uint32_t l = a->limbs;
+  chk(fides_mul_const_batch(ctx, 1, &in, &l, &rp, &out, nullptr), "scalar multiplication");
+  o->NoiseLevel = a->NoiseLevel;
+  o->NoiseFactor = a->NoiseFactor;
+  o->slots = a->slots;
+  o->keyid = a->keyid;
+  return o;
+}
+
+// Ciphertext::multScalarNoPrecheck (src/CKKS/Ciphertext.cpp:747-762): the
+// constant is encoded at the level's real scaling factor; scale degree + 1.
+Ciphertext CryptoContextImpl::MultScalarNoPrecheck(const Ciphertext& a, double c) {
+  const double sf = sfReal.at(a->limbs - 1);
+  Ciphertext o = MulConst(a, ScalarResidues(a->limbs, c, sf));
+  o->NoiseLevel = a->NoiseLevel + 1;
+  o->NoiseFactor = a->NoiseFactor * sf;
+  return o;
+}
+
+Ciphertext CryptoContextImpl::MultScalar(const Ciphertext& a0, double c) {
+  Ciphertext a = a0;
+  if (technique != FIXEDMANUAL && a->NoiseLevel == 2) a = Rescale(a);
+  if (a->NoiseLevel != 1) throw std::runtime_error("EvalMult(scalar): operand at scale degree > 1 (FIXEDMANUAL)");
+  return MultScalarNoPrecheck(a, c);
+}
+
+// ---------------------------------------------------------------- level / scale alignment
+// Brings `a` (at a higher level) to `level` and to the given scale degree and
+// scaling factor, as Ciphertext::adjustScaleAndLevel
+// (src/CKKS/Ciphertext.cpp:1376-1455): a scalar multiplication by the ratio
+// of the scaling factors merged with the Rescale that drops the level.
+// Returns false when `a` is the lower of the two (the caller swaps roles).
+bool CryptoContextImpl::AdjustScaleAndLevel(Ciphertext& a, int scaleDegree, uint32_t level, double scalingFactor) {
+  const uint32_t l1 = a->limbs - 1, l2 = level;
+  const int d1 = a->NoiseLevel, d2 = scaleDegree;
+  if (l1 > l2) {
+    if (d1 == 2) {
+      if (d2 == 2) {
+        const double scf1 = a->NoiseFactor, scf2 = scalingFactor, scf = sfReal.at(l1);
+        const double q1 = modReduceFactor.at(l2 + 1);
+        a = MultScalarNoPrecheck(a, scf2 * q1 / scf1 / scf);
+        if (a->limbs - 1 > l2 + 1) a = DropToLimbs(a, l2 + 2);
+        a->NoiseFactor = sfBig.at(l2) * modReduceFactor.at(l2 + 1);
+        a = Rescale(a);
+        a->NoiseFactor = scalingFactor;
+      } else if (l1 - 1 == l2) {
+        a = Rescale(a);
+      } else {
+        const double scf1 = a->NoiseFactor, scf2 = sfBig.at(l2 + 1), scf = sfReal.at(l1);
+        const double q1 = modReduceFactor.at(l1);
+        a = MultScalarNoPrecheck(a, scf2 / scf1 * q1 / scf);
+        a->NoiseFactor = sfBig.at(a->limbs - 2) * modReduceFactor.at(l1);
+        a = Rescale(a);
+        if (a->limbs - 2 > l2) a = DropToLimbs(a, l2 + 2);
+        a->NoiseFactor = sfBig.at(a->limbs - 1);
+        a = Rescale(a);
+        a->NoiseFactor = scalingFactor;
+      }
+    } else {
+      if (d2 == 2) {
+        const double scf1 = a->NoiseFactor, scf2 = scalingFactor, scf = sfReal.at(l1);
+        a = MultScalarNoPrecheck(a, scf2 / scf1 / scf);
+        a = DropToLimbs(a, l2 + 1);
+        a->NoiseFactor = scf2;
+      } else {
+        const double scf1 = a->NoiseFactor, scf2 = sfBig.at(l2 + 1), scf = sfReal.at(l1);
+        a = MultScalarNoPrecheck(a, scf2 / scf1 / scf);
+        if (l1 - 1 > l2) a = DropToLimbs(a, l2 + 2);
+        a = Rescale(a);
+        a->NoiseFactor = scalingFactor;
+      }
+    }
+    return true;
+  }
+  if (l1 < l2) return false;
+  if (d1 < d2) {
+    a = MultScalarNoPrecheck(a, 1.0);
+  } else if (d2 < d1) {
+    return false;
+  }
+  return true;
+}
+
+// Ciphertext::adjustForAddOrSub (src/CKKS/Ciphertext.cpp:1457-1490).
+bool CryptoContextImpl::AdjustForAddOrSub(Ciphertext& a, const Ciphertext& b) {
+  if (Flexible()) return AdjustScaleAndLevel(a, b->NoiseLevel, b->limbs - 1, b->NoiseFactor);
+  const int la = (int)a->limbs - 1 - a->NoiseLevel, lb = (int)b->limbs - 1 - b->NoiseLevel;
+  if (la > lb) {
+    if (b->NoiseLevel == 1 && a->NoiseLevel == 2)
+      a = Rescale(a);
+    else if (b->NoiseLevel == 2 && a->NoiseLevel == 1)
+      a = MultScalarNoPrecheck(a, 1.0);
+    return true;
+  }
+  if (b->NoiseLevel == 1 && a->NoiseLevel == 2) {
+    a = Rescale(a);
+    return true;
+  }
+  return !(a->NoiseLevel == 1 && b->NoiseLevel == 2);
+}
+
+// Ciphertext::adjustForMult (src/CKKS/Ciphertext.cpp:1492-1508).
+bool CryptoContextImpl::AdjustForMult(Ciphertext& a, const Ciphertext& b) {
+  const bool ok = AdjustForAddOrSub(a, b);
+  if (a->NoiseLevel == 2) a = Rescale(a);
+  return ok && b->NoiseLevel != 2;
+}
+
+Ciphertext CryptoContextImpl::AddSub(const Ciphertext& a0, const Ciphertext& b0, bool sub) {
+  Ciphertext a = a0, b = b0;
+  if (technique != FIXEDMANUAL) {
+    if (!AdjustForAddOrSub(a, b)) {
+      if (!AdjustForAddOrSub(b, a)) throw std::runtime_error("EvalAdd/EvalSub: cannot align operands");
+    }
+  } else if (a->NoiseLevel != b->NoiseLevel) {
+    throw std::runtime_error("EvalAdd/EvalSub: operands at different scale degrees (FIXEDMANUAL)");
+  }
+  const uint32_t l = std::min(a->limbs, b->limbs);
+  if (a->limbs > l) a = DropToLimbs(a, l);
+  if (b->limbs > l) b = DropToLimbs(b, l);
+  Ciphertext o = NewCiphertext(l);
+  const uint64_t* pa = a->d;
+  const uint64_t* pb = b->d;
+  uint64_t* po = o->d;
+  chk(fides_add_batch(ctx, 1, &pa, &pb, &l, sub ? 1 : 0, &po, nullptr), sub ? "EvalSub" : "EvalAdd");
+  o->NoiseLevel = std::max(a->NoiseLevel, b->NoiseLevel);
+  o->NoiseFactor = a->NoiseFactor;
+  o->slots = a->slots;
+  o->keyid = a->keyid;
+  return o;
+}
+
+Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, const Ciphertext& b) { return AddSub(a, b, false); }
+Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, const Ciphertext& b) { return AddSub(a, b, true); }
+
+Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double c) {
+  Ciphertext o = NewCiphertext(a->limbs);
+  const std::vector<uint64_t> res = AddScalarResidues(a->limbs, c, a->NoiseLevel);
+  const uint64_t* in = a->d;
+  uint64_t* out = o->d;
+  const uint64_t* rp = res.data();
+  const uint32_t l = a->limbs;
+  chk(fides_add_const_batch(ctx, 1, &in, &l, &rp, &out, nullptr), "EvalAdd(scalar)");
+  o->NoiseLevel = a->NoiseLevel;
+  o->NoiseFactor = a->NoiseFactor;
+  o->slots = a->slots;
+  o->keyid = a->keyid;
+  return o;
+}
+
+Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, double c) { return EvalAdd(a, -c); }
+
+Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a, double c) {
+  Ciphertext o = MultScalar(a, c);
+  if (technique == FIXEDAUTO) o = Rescale(o);
+  return o;
+}
+
+Ciphertext CryptoContextImpl::EvalNegate(const Ciphertext& a) {
+  std::vector<uint64_t> res(a->limbs);
+  for (uint32_t i = 0; i < a->limbs; ++i) res[i] = q[i] - 1;
+  return MulConst(a, res);
+}
+
 void CryptoContextImpl::PrepareMult(Ciphertext& a, Ciphertext& b) {
   if (technique == FIXEDMANUAL) {
     if (a->limbs != b->limbs) throw std::runtime_error("EvalMult: operands at different levels (FIXEDMANUAL)");
+    return;
+  }
+  if (Flexible()) {
+    // Ciphertext::mult's alignment (src/CKKS/Ciphertext.cpp:458-477)
+    if (!AdjustForMult(a, b)) {
+      if (!AdjustForMult(b, a)) throw std::runtime_error("EvalMult: cannot align operands");
+    }
+    if (a->NoiseLevel == 2) a = Rescale(a);
+    if (b->NoiseLevel == 2) b = Rescale(b);
+    if (a->limbs != b->limbs) throw std::runtime_error("EvalMult: alignment left different levels");
     return;
   }
   // FIXEDAUTO: rescale degree-2 operands, then align limb counts

@@ -41,6 +41,13 @@ struct RawParams {
   int dnum = 1;
   ScalingTechnique scalingTechnique = FIXEDMANUAL;
   std::vector<double> ModReduceFactor;          // per level (optional; default = the dropped prime)
+  // OpenFHE's scaling factors per level (level = limbs - 1), as FIDESlib's
+  // GetRawParams copies them (RawCiphertext.cu:260-272).  Optional: the
+  // defaults follow OpenFHE's rule -- FLEXIBLEAUTO(EXT): sf[L-1] = q[L-1],
+  // sf[l-1] = sf[l]^2 / q[l], Big[l] = sf[l]^2; fixed techniques: 2^p with p
+  // the bit length of q[1] rounded (the scaling modulus size).
+  std::vector<double> ScalingFactorReal;
+  std::vector<double> ScalingFactorRealBig;
 };
 
 // Ciphertext in RNS evaluation form (RawCipherText, RawCiphertext.cuh:19-35).
@@ -133,6 +140,22 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   // Drop limbs above `limbs` (OpenFHE LevelReduce / FIDESlib dropToLevel).
   Ciphertext DropToLimbs(const Ciphertext& a, uint32_t limbs);
 
+  // ---- element-wise operations (api/CryptoContext.hpp:114-160), with the
+  // level/scale alignment of the scaling technique (Ciphertext::add/sub,
+  // adjustForAddOrSub, src/CKKS/Ciphertext.cpp:180-240, 1462-1500)
+  Ciphertext EvalAdd(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalSub(const Ciphertext& a, const Ciphertext& b);
+  Ciphertext EvalAdd(const Ciphertext& a, double c);
+  Ciphertext EvalSub(const Ciphertext& a, double c);
+  Ciphertext EvalMult(const Ciphertext& a, double c);
+  Ciphertext EvalNegate(const Ciphertext& a);
+  void EvalAddInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalAdd(a, b); }
+  void EvalSubInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalSub(a, b); }
+  void EvalMultInPlace(Ciphertext& a, double c) { a = EvalMult(a, c); }
+
+  double ScalingFactorReal(uint32_t level) const { return sfReal.at(level); }
+  double ScalingFactorRealBig(uint32_t level) const { return sfBig.at(level); }
+
   // ---- cross-ciphertext batching (one launch per pipeline stage)
   std::vector<Ciphertext> EvalMultMany(const std::vector<Ciphertext>& a, const std::vector<Ciphertext>& b,
                                        bool rescale);
@@ -147,14 +170,25 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   Ciphertext NewCiphertext(uint32_t limbs);
   uint32_t AutoIndex(int32_t index) const;
   const fides_key* RotationKey(uint32_t autoIndex) const;
-  void PrepareMult(Ciphertext& a, Ciphertext& b);  // FIXEDAUTO level/scale alignment
+  void PrepareMult(Ciphertext& a, Ciphertext& b);  // FIXEDAUTO / FLEXIBLEAUTO level/scale alignment
   void MultMetadata(CiphertextImpl& out, const CiphertextImpl& a, const CiphertextImpl& b) const;
+  bool Flexible() const { return technique == FLEXIBLEAUTO || technique == FLEXIBLEAUTOEXT; }
+  std::vector<uint64_t> ScalarResidues(uint32_t limbs, double operand, double scale) const;
+  std::vector<uint64_t> AddScalarResidues(uint32_t limbs, double operand, int noiseDeg) const;
+  Ciphertext MulConst(const Ciphertext& a, const std::vector<uint64_t>& res);
+  Ciphertext MultScalarNoPrecheck(const Ciphertext& a, double c);
+  Ciphertext MultScalar(const Ciphertext& a, double c);
+  bool AdjustScaleAndLevel(Ciphertext& a, int scaleDegree, uint32_t level, double scalingFactor);
+  bool AdjustForAddOrSub(Ciphertext& a, const Ciphertext& b);
+  bool AdjustForMult(Ciphertext& a, const Ciphertext& b);
+  Ciphertext AddSub(const Ciphertext& a, const Ciphertext& b, bool sub);
 
   fides_context* ctx = nullptr;
   uint32_t logN = 0, N = 0, L = 0, K = 0, dnum = 0;
   ScalingTechnique technique = FIXEDMANUAL;
   std::vector<uint64_t> q;
   std::vector<double> modReduceFactor;
+  std::vector<double> sfReal, sfBig;  // by level = limbs - 1
   fides_key* evalMultKey = nullptr;
   std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
 };

@@ -73,12 +73,69 @@ static void append(std::vector<uint64_t>& out, const fideslib::RawCipherText& r)
   for (const auto& limb : r.sub_1) out.insert(out.end(), limb.begin(), limb.end());
 }
 
+static void store(const std::string& path, fideslib::CryptoContext& cc, const fideslib::Ciphertext& c,
+                  std::ofstream& meta) {
+  std::vector<uint64_t> out;
+  append(out, cc->StoreCiphertext(c));
+  write_u64(path, out);
+  meta << c->GetLimbs() << " " << c->NoiseLevel << " " << c->NoiseFactor << "\n";
+}
+
+// FLEXIBLEAUTO: operands at different levels and scale degrees are aligned by
+// the API (scalar-multiply-and-rescale), as FIDESlib does on OpenFHE's tables.
+// Inputs c1 (L limbs), c2 (L-1 limbs), c3 (L limbs) at the default scaling
+// factors of their levels.  Outputs m = c1*c2, a = m + c3, s = 0.5 a,
+// t = s + 0.25, u = t - c2, v = -u, w = 3 c3, each with its metadata line.
+static int flex(const std::string& dir) {
+  const auto P = read_u64(dir + "/params.bin");
+  const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+  const uint32_t N = 1u << logN;
+  fideslib::RawParams rp;
+  rp.logN = (int)logN;
+  rp.moduli.assign(P.begin() + 6, P.begin() + 6 + L);
+  rp.SPECIALmoduli.assign(P.begin() + 6 + L, P.begin() + 6 + L + K);
+  rp.dnum = (int)dnum;
+  rp.scalingTechnique = fideslib::FLEXIBLEAUTO;
+  auto cc = fideslib::CryptoContextImpl::Create(rp);
+  cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
+  auto c1 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), L, N, cc->ScalingFactorReal(L - 1)));
+  auto c2 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c2.bin"), L - 1, N, cc->ScalingFactorReal(L - 2)));
+  auto c3 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c3.bin"), L, N, cc->ScalingFactorReal(L - 1)));
+  std::ofstream meta(dir + "/meta.txt");
+  meta << std::setprecision(17);
+  for (uint32_t l = 0; l < L; ++l) meta << cc->ScalingFactorReal(l) << (l + 1 < L ? " " : "\n");
+  auto m = cc->EvalMult(c1, c2);
+  auto a = cc->EvalAdd(m, c3);
+  auto s = cc->EvalMult(a, 0.5);
+  auto t = cc->EvalAdd(s, 0.25);
+  auto u = cc->EvalSub(t, c2);
+  auto v = cc->EvalNegate(u);
+  auto w = cc->EvalMult(c3, 3.0);
+  store(dir + "/m.bin", cc, m, meta);
+  store(dir + "/a.bin", cc, a, meta);
+  store(dir + "/s.bin", cc, s, meta);
+  store(dir + "/t.bin", cc, t, meta);
+  store(dir + "/u.bin", cc, u, meta);
+  store(dir + "/v.bin", cc, v, meta);
+  store(dir + "/w.bin", cc, w, meta);
+  std::printf("api_check flex: ok\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
-  if (argc != 2) {
-    std::fprintf(stderr, "usage: api_check DIR\n");
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: api_check DIR [flex]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  if (argc > 2 && std::string(argv[2]) == "flex") {
+    try {
+      return flex(dir);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "api_check: %s\n", e.what());
+      return 1;
+    }
+  }
   try {
     const auto P = read_u64(dir + "/params.bin");
     const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];

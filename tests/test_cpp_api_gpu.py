@@ -60,3 +60,49 @@ def test_cpp_api_against_oracle(tmp_path):
     # decrypts to the product rotated by 3 slots
     dec = C.decode(keys.decrypt_coef(rotd, l - 1), 2.0 ** prm.scale_bits * 2.0 ** prm.scale_bits / prm.q[l - 1], N)
     assert C.precision_bits(dec, np.roll(z[0] * z[1], -rot)) > 15
+
+
+def test_cpp_api_flexibleauto(tmp_path):
+    """FLEXIBLEAUTO through the C++ API: operands at different levels and scale
+    degrees are aligned automatically; every result decrypts to its FP64 value
+    at the NoiseFactor the API reports."""
+    prm = params.config1()
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 19)
+    # default FLEXIBLEAUTO scaling factors: sf[L-1] = q[L-1], sf[l-1] = sf[l]^2 / q[l]
+    sf = [0.0] * L
+    sf[L - 1] = float(prm.q[L - 1])
+    for l in range(L - 1, 0, -1):
+        sf[l - 1] = sf[l] * sf[l] / float(prm.q[l])
+    rng = np.random.default_rng(9)
+    z = [rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2) for _ in range(3)]
+    c1 = keys.encrypt(C.encode(z[0], sf[L - 1], N), L)
+    c2 = keys.encrypt(C.encode(z[1], sf[L - 2], N), L - 1)
+    c3 = keys.encrypt(C.encode(z[2], sf[L - 1], N), L)
+    d = str(tmp_path)
+    hdr = [prm.logN, prm.L, prm.K, prm.dnum, L, 1] + list(prm.q) + list(prm.p)
+    np.asarray(hdr, np.uint64).tofile(os.path.join(d, "params.bin"))
+    keys.relin_key().astype(np.uint64).tofile(os.path.join(d, "rlk.bin"))
+    for name, ct in (("c1", c1), ("c2", c2), ("c3", c3)):
+        ct.astype(np.uint64).tofile(os.path.join(d, name + ".bin"))
+    r = subprocess.run([BIN, d, "flex"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    meta = open(os.path.join(d, "meta.txt")).read().strip().split("\n")
+    got_sf = [float(x) for x in meta[0].split()]
+    assert np.allclose(got_sf, sf, rtol=1e-15)
+    m = z[0] * z[1]
+    a = m + z[2]
+    s = 0.5 * a
+    t = s + 0.25
+    u = t - z[1]
+    v = -u
+    w = 3 * z[2]
+    for i, (name, want) in enumerate((("m", m), ("a", a), ("s", s), ("t", t), ("u", u), ("v", v), ("w", w))):
+        limbs, nl, nf = meta[1 + i].split()
+        limbs, nf = int(limbs), float(nf)
+        ct = np.fromfile(os.path.join(d, name + ".bin"), np.uint64).reshape(2, limbs, N)
+        dec = C.decode(keys.decrypt_coef(ct, limbs), nf, N)
+        bits = C.precision_bits(dec, want)
+        print("flex %s: limbs %d, degree %s, %.1f bits" % (name, limbs, nl, bits), flush=True)
+        assert bits > 14, (name, bits)
