@@ -75,6 +75,7 @@ SIGNATURES = {
     "fides_ptdot_batch": (_int, [_vp, _u32, _u32p, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p,
                                  ctypes.POINTER(_vp), _vp]),
     "fides_mod_raise_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_key_download": (_int, [_vp, _u64p]),
     "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
                                ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),

@@ -1036,6 +1036,22 @@ extern "C" int fides_key_upload(fides_context* c, const uint64_t* h, fides_key**
   return FIDES_OK;
 }
 
+extern "C" int fides_key_download(const fides_key* k, uint64_t* h) {
+  if (!k || !h) return fail(FIDES_ERR_ARG, "null argument");
+  const fides_context* c = k->ctx;
+  const size_t LK = c->L + c->K, nl = 2 * (size_t)c->dnum * LK, N = c->N;
+  HIPCHK(hipMemcpy(h, k->d, nl * N * 8, hipMemcpyDeviceToHost));
+  // out of Montgomery form: x * 2^-64 mod q
+  for (size_t limb = 0; limb < nl; ++limb) {
+    const u64 q = c->primes[limb % LK];
+    const u64 r = (u64)(((unsigned __int128)1 << 64) % q);
+    const u64 rinv = powm(r, q - 2, q);
+    u64* x = h + limb * N;
+    for (size_t i = 0; i < N; ++i) x[i] = mulm(x[i], rinv, q);
+  }
+  return FIDES_OK;
+}
+
 extern "C" void fides_key_destroy(fides_key* k) {
   if (!k) return;
   (void)hipFree(k->d);

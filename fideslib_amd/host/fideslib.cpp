@@ -54,6 +54,7 @@ CryptoContext CryptoContextImpl::Create(const RawParams& P) {
   fides_params fp{c->logN, c->L, c->K, c->dnum, P.moduli.data(), P.SPECIALmoduli.data(),
                   have_roots ? roots.data() : nullptr};
   chk(fides_context_create(&fp, &c->ctx), "context creation");
+  c->raw = P;
   c->modReduceFactor = P.ModReduceFactor;
   if (c->modReduceFactor.size() < c->L) {
     c->modReduceFactor.resize(c->L);
@@ -98,6 +99,42 @@ static fides_key* upload(fides_context* ctx, const RawKeySwitchKey& k, uint32_t 
   fides_key* out = nullptr;
   chk(fides_key_upload(ctx, flat.data(), &out), "key upload");
   return out;
+}
+
+static RawKeySwitchKey download(const fides_key* key, uint32_t dnum, uint32_t LK, uint32_t N) {
+  std::vector<uint64_t> flat((size_t)2 * dnum * LK * N);
+  chk(fides_key_download(key, flat.data()), "key download");
+  RawKeySwitchKey k;
+  k.r_key.resize(2);
+  for (size_t c = 0; c < 2; ++c) {
+    k.r_key[c].resize(dnum);
+    for (size_t j = 0; j < dnum; ++j) {
+      k.r_key[c][j].resize(LK);
+      for (size_t i = 0; i < LK; ++i) {
+        const auto b = flat.begin() + ((c * dnum + j) * LK + i) * N;
+        k.r_key[c][j][i].assign(b, b + N);
+      }
+    }
+  }
+  return k;
+}
+
+RawKeySwitchKey CryptoContextImpl::GetEvalMultKey() const {
+  if (!evalMultKey) throw std::runtime_error("no relinearization key loaded");
+  return download(evalMultKey, dnum, L + K, N);
+}
+
+std::map<uint32_t, RawKeySwitchKey> CryptoContextImpl::GetRotationKeys() const {
+  std::map<uint32_t, RawKeySwitchKey> out;
+  for (const auto& kv : rotationKeys) out[kv.first] = download(kv.second, dnum, L + K, N);
+  return out;
+}
+
+void CryptoContextImpl::AddAutomorphismKey(uint32_t a, const RawKeySwitchKey& k) {
+  if ((a & 1) == 0 || a >= 2 * N) throw std::runtime_error("automorphism index must be odd and < 2N");
+  auto it = rotationKeys.find(a);
+  if (it != rotationKeys.end()) fides_key_destroy(it->second);
+  rotationKeys[a] = upload(ctx, k, dnum, L + K, N);
 }
 
 void CryptoContextImpl::AddEvalMultKey(const RawKeySwitchKey& k) {

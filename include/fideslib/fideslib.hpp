@@ -165,6 +165,12 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
 
   fides_context* Handle() const { return ctx; }
 
+  // ---- serialization support (fideslib/serialize.hpp)
+  const RawParams& GetRawParams() const { return raw; }
+  RawKeySwitchKey GetEvalMultKey() const;                       // downloads the resident key
+  std::map<uint32_t, RawKeySwitchKey> GetRotationKeys() const;  // by automorphism index
+  void AddAutomorphismKey(uint32_t autoIndex, const RawKeySwitchKey& key);
+
  private:
   CryptoContextImpl() = default;
   Ciphertext NewCiphertext(uint32_t limbs);
@@ -184,6 +190,7 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   Ciphertext AddSub(const Ciphertext& a, const Ciphertext& b, bool sub);
 
   fides_context* ctx = nullptr;
+  RawParams raw;
   uint32_t logN = 0, N = 0, L = 0, K = 0, dnum = 0;
   ScalingTechnique technique = FIXEDMANUAL;
   std::vector<uint64_t> q;

@@ -114,6 +114,10 @@ int fides_moddown(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint64_t
  * KeySwitchingKey::Initialize (src/CKKS/KeySwitchingKey.cu:20-35) fed by
  * GetEvalKeySwitchKey / GetRotationKeySwitchKey (RawCiphertext.cu:645-760). */
 int fides_key_upload(fides_context* ctx, const uint64_t* h_key, fides_key** out);
+/* Copies a key back to the host in its upload layout and form ([2][dnum][L+K][N],
+ * standard residues): the inverse of fides_key_upload, for serialization
+ * (reference CryptoContext::SerializeEvalMultKey, api/CryptoContext.hpp:90-95). */
+int fides_key_download(const fides_key* key, uint64_t* h_key);
 void fides_key_destroy(fides_key* key);
 
 /* Hybrid key switch of one evaluation-form polynomial d (l limbs):

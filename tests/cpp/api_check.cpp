@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "fideslib/fideslib.hpp"
+#include "fideslib/serialize.hpp"
 
 static std::vector<uint64_t> read_u64(const std::string& path) {
   std::ifstream f(path, std::ios::binary | std::ios::ate);
@@ -122,12 +123,111 @@ static int flex(const std::string& dir) {
   return 0;
 }
 
+static bool same(const fideslib::RawCipherText& a, const fideslib::RawCipherText& b) {
+  return a.sub_0 == b.sub_0 && a.sub_1 == b.sub_1 && a.Noise == b.Noise && a.NoiseLevel == b.NoiseLevel &&
+         a.slots == b.slots && a.keyid == b.keyid;
+}
+
+static fideslib::RawParams params_of(const std::vector<uint64_t>& P) {
+  const uint32_t L = (uint32_t)P[1], K = (uint32_t)P[2];
+  fideslib::RawParams rp;
+  rp.logN = (int)P[0];
+  rp.moduli.assign(P.begin() + 6, P.begin() + 6 + L);
+  rp.SPECIALmoduli.assign(P.begin() + 6 + L, P.begin() + 6 + L + K);
+  rp.dnum = (int)P[3];
+  rp.scalingTechnique = fideslib::FIXEDMANUAL;
+  rp.ModReduceFactor = {1.5, 2.5};
+  rp.ScalingFactorReal = {3.25};
+  return rp;
+}
+
+// Host structs through the binary format (no device needed).
+static int serial_host(const std::string& dir) {
+  using namespace fideslib;
+  const auto P = read_u64(dir + "/params.bin");
+  const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+  const uint32_t l = (uint32_t)P[4], N = 1u << logN;
+  RawParams rp = params_of(P), rp2;
+  RawCipherText c = ct_of(read_u64(dir + "/c1.bin"), l, N, 1234.5), c2;
+  c.keyid = "key-1";
+  RawKeySwitchKey k = key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N), k2;
+  k.keyid = "relin";
+  bool ok = Serial::SerializeToFile(dir + "/p.fds", rp) && Serial::DeserializeFromFile(dir + "/p.fds", rp2);
+  ok = ok && rp2.logN == rp.logN && rp2.moduli == rp.moduli && rp2.SPECIALmoduli == rp.SPECIALmoduli &&
+       rp2.dnum == rp.dnum && rp2.ModReduceFactor == rp.ModReduceFactor && rp2.ScalingFactorReal == rp.ScalingFactorReal;
+  ok = ok && Serial::SerializeToFile(dir + "/c.fds", c) && Serial::DeserializeFromFile(dir + "/c.fds", c2) && same(c, c2);
+  ok = ok && Serial::SerializeToFile(dir + "/k.fds", k) && Serial::DeserializeFromFile(dir + "/k.fds", k2) &&
+       k2.r_key == k.r_key && k2.keyid == k.keyid;
+  // a file of another type, a truncated file and JSON are refused
+  RawKeySwitchKey wrong;
+  ok = ok && !Serial::DeserializeFromFile(dir + "/c.fds", wrong);
+  {
+    std::ifstream f(dir + "/c.fds", std::ios::binary);
+    std::string all((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    std::ofstream g(dir + "/t.fds", std::ios::binary);
+    g.write(all.data(), (std::streamsize)(all.size() / 2));
+  }
+  RawCipherText c3;
+  ok = ok && !Serial::DeserializeFromFile(dir + "/t.fds", c3);
+  ok = ok && !Serial::SerializeToFile(dir + "/j.fds", c, JSON);
+  std::printf("api_check serial-host: %s\n", ok ? "ok" : "FAILED");
+  return ok ? 0 : 1;
+}
+
+// A whole context (parameters + keys) and ciphertexts through files: the
+// deserialized context computes the same bits.
+static int serial(const std::string& dir) {
+  using namespace fideslib;
+  const auto P = read_u64(dir + "/params.bin");
+  const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+  const uint32_t l = (uint32_t)P[4];
+  const int32_t rot = (int32_t)P[5];
+  const uint32_t N = 1u << logN;
+  RawParams rp = params_of(P);
+  rp.ModReduceFactor.clear();
+  rp.ScalingFactorReal.clear();
+  auto cc = CryptoContextImpl::Create(rp);
+  cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
+  cc->AddRotationKey(rot, key_of(read_u64(dir + "/rot.bin"), dnum, L + K, N));
+  const double scale = (double)rp.moduli[l - 1];
+  auto a = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), l, N, scale));
+  auto b = cc->LoadCiphertext(ct_of(read_u64(dir + "/c2.bin"), l, N, scale));
+  auto r1 = cc->StoreCiphertext(cc->EvalRotate(cc->Rescale(cc->EvalMult(a, b)), rot));
+  bool ok = Serial::SerializeToFile(dir + "/ctx.fds", cc) &&
+            Serial::SerializeToFile(dir + "/a.fds", cc->StoreCiphertext(a)) &&
+            Serial::SerializeToFile(dir + "/b.fds", cc->StoreCiphertext(b));
+  CryptoContext cc2;
+  RawCipherText ra, rb;
+  ok = ok && Serial::DeserializeFromFile(dir + "/ctx.fds", cc2) && Serial::DeserializeFromFile(dir + "/a.fds", ra) &&
+       Serial::DeserializeFromFile(dir + "/b.fds", rb);
+  if (!ok) {
+    std::printf("api_check serial: FAILED (io)\n");
+    return 1;
+  }
+  auto a2 = cc2->LoadCiphertext(ra), b2 = cc2->LoadCiphertext(rb);
+  auto r2 = cc2->StoreCiphertext(cc2->EvalRotate(cc2->Rescale(cc2->EvalMult(a2, b2)), rot));
+  ok = same(r1, r2) && cc2->HasRotationKey(rot) && cc2->GetEvalMultKey().r_key == cc->GetEvalMultKey().r_key;
+  std::vector<uint64_t> out;
+  append(out, r2);
+  write_u64(dir + "/serial_result.bin", out);
+  std::printf("api_check serial: %s\n", ok ? "ok" : "FAILED");
+  return ok ? 0 : 1;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: api_check DIR [flex]\n");
+    std::fprintf(stderr, "usage: api_check DIR [flex|serial|serial-host]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  if (argc > 2 && (std::string(argv[2]) == "serial-host" || std::string(argv[2]) == "serial")) {
+    try {
+      return std::string(argv[2]) == "serial" ? serial(dir) : serial_host(dir);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "api_check: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc > 2 && std::string(argv[2]) == "flex") {
     try {
       return flex(dir);
