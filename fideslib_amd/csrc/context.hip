@@ -174,9 +174,11 @@ struct TaskSlot {
   char* host = nullptr;  // pinned
   char* dev = nullptr;
   size_t cap = 0;
-  hipEvent_t done = nullptr;
+  hipEvent_t done = nullptr;    // the operation that read this slot has finished
+  hipEvent_t copied = nullptr;  // the slot's upload has landed on the device
   bool used = false;
 };
+constexpr size_t SLOT_BYTES = 8u << 20;  // initial capacity of every task slot
 
 struct fides_context {
   u32 logN, N, L, K, dnum, alpha;
@@ -197,6 +199,7 @@ struct fides_context {
   size_t arena_words = 0;
   TaskSlot ring[RING];
   int ring_next = 0;
+  hipStream_t copy_stream = nullptr;  // task-list uploads, overlapped with the previous operation's kernels
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -273,7 +276,9 @@ extern "C" void fides_context_destroy(fides_context* c) {
     if (s.host) (void)hipHostFree(s.host);
     if (s.dev) (void)hipFree(s.dev);
     if (s.done) (void)hipEventDestroy(s.done);
+    if (s.copied) (void)hipEventDestroy(s.copied);
   }
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   delete c;
 }
 
@@ -450,11 +455,21 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   H2D(c->modup, c->hmodup.data(), sizeof(ConvDesc) * c->hmodup.size());
   ALLOC(c->moddown, sizeof(ConvDesc) * c->hmoddown.size());
   H2D(c->moddown, c->hmoddown.data(), sizeof(ConvDesc) * c->hmoddown.size());
+  // Task slots are allocated up front: an allocation inside a stream of
+  // operations would stall the host while the device drains.
+  if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+    fides_context_destroy(c);
+    return fail(FIDES_ERR_HIP, "copy stream creation");
+  }
   for (auto& s : c->ring) {
-    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&s.done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess ||
+        hipHostMalloc((void**)&s.host, SLOT_BYTES, hipHostMallocDefault) != hipSuccess ||
+        hipMalloc((void**)&s.dev, SLOT_BYTES) != hipSuccess) {
       fides_context_destroy(c);
-      return fail(FIDES_ERR_HIP, "event creation");
+      return fail(FIDES_ERR_NOMEM, "task slot allocation");
     }
+    s.cap = SLOT_BYTES;
   }
 #undef ALLOC
 #undef H2D
@@ -546,7 +561,11 @@ static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char*
     s.cap = cap;
   }
   memcpy(s.host, blob.b.data(), blob.b.size());
-  HIPCHK(hipMemcpyAsync(s.dev, s.host, blob.b.size(), hipMemcpyHostToDevice, st));
+  // The upload runs on the copy stream while `st` still executes earlier
+  // work; `st` waits for it only before this operation's first kernel.
+  HIPCHK(hipMemcpyAsync(s.dev, s.host, blob.b.size(), hipMemcpyHostToDevice, c->copy_stream));
+  HIPCHK(hipEventRecord(s.copied, c->copy_stream));
+  HIPCHK(hipStreamWaitEvent(st, s.copied, 0));
   *dev = s.dev;
   return FIDES_OK;
 }
