@@ -668,6 +668,9 @@ static size_t ks_words(const fides_context* c, u32 l, bool shared_modup = false)
 //   one NTT per remaining limb with PRO_MDRS / EPI_MDRS (see ntt.h).
 struct KsLists {
   size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, axpy1, intt_top, axpy2, ntt_mdrs;
+  // dot tasks grouped by digit count for the fused row pass: [start, count) per beta
+  u32 dot_start[ROWDOT_MAXB + 1] = {}, dot_count[ROWDOT_MAXB + 1] = {};
+  bool fused_dot = false;
   u32 n_intt_in = 0, n_conv_up[CONV_MAXS + 1] = {}, n_ntt_ext = 0, n_dot = 0, n_intt_sp = 0, n_conv_down = 0,
       n_ntt_md = 0, n_axpy1 = 0, n_intt_top = 0, n_axpy2 = 0, n_ntt_mdrs = 0;
   bool any_acc = false;
@@ -793,8 +796,22 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   by_prime(intt_sp);
   by_prime(ntt_md);
   by_prime(ntt_mdrs);
-  // key limbs of the same prime (shared by every job using the same key) next to each other
-  std::stable_sort(dot.begin(), dot.end(), [](const DotTask& a, const DotTask& b) { return a.pid < b.pid; });
+  // Key limbs of the same prime (shared by every job using the same key) next
+  // to each other; with the fused row pass, tasks are also grouped by digit
+  // count (one launch per count).
+  u32 maxb = 0;
+  for (const DotTask& d : dot) maxb = std::max(maxb, d.beta);
+  K_.fused_dot = maxb <= (u32)ROWDOT_MAXB;
+  std::stable_sort(dot.begin(), dot.end(), [&](const DotTask& a, const DotTask& b) {
+    if (K_.fused_dot && a.beta != b.beta) return a.beta < b.beta;
+    return a.pid < b.pid;
+  });
+  if (K_.fused_dot)
+    for (u32 i = 0; i < dot.size(); ++i) {
+      const u32 b = dot[i].beta;
+      if (K_.dot_count[b] == 0) K_.dot_start[b] = i;
+      K_.dot_count[b]++;
+    }
   K_.intt_in = blob.add(intt_in);
   K_.n_intt_in = (u32)intt_in.size();
   for (u32 ns = 1; ns <= CONV_MAXS; ++ns) {
@@ -825,10 +842,25 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
   RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_in), K_.n_intt_in, st));
   for (u32 ns = 1; ns <= CONV_MAXS; ++ns)
     if (K_.n_conv_up[ns]) RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_up[ns]), K_.n_conv_up[ns], ns, st));
-  RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_ext), K_.n_ntt_ext, PRO_NONE, EPI_NONE, st));
-  for (u32 o = 0; o < K_.n_dot; o += 65535)
-    fides_launch_dot_batch(at<DotTask>(dev, K_.dot) + o, std::min<u32>(65535, K_.n_dot - o), c->prm, c->logN,
-                           c->L + c->K, st);
+  if (K_.fused_dot) {
+    // column pass of the ModUp outputs, then row pass fused with the inner product
+    for (u32 o = 0; o < K_.n_ntt_ext; o += 65535) {
+      LimbSpan sp = span0();
+      sp.list = at<NttTask>(dev, K_.ntt_ext) + o;
+      sp.total = std::min<u32>(65535, K_.n_ntt_ext - o);
+      if (fides_launch_ntt_col(c->logN, sp, c->T, PRO_NONE, fuse0(), st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
+    }
+    for (int b = 1; b <= ROWDOT_MAXB; ++b)
+      for (u32 o = 0; o < K_.dot_count[b]; o += 65535)
+        if (fides_launch_row_dot(c->logN, at<DotTask>(dev, K_.dot) + K_.dot_start[b] + o,
+                                 std::min<u32>(65535, K_.dot_count[b] - o), b, c->T, c->L + c->K, st))
+          return fail(FIDES_ERR_UNSUPPORTED, "logN");
+  } else {
+    RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_ext), K_.n_ntt_ext, PRO_NONE, EPI_NONE, st));
+    for (u32 o = 0; o < K_.n_dot; o += 65535)
+      fides_launch_dot_batch(at<DotTask>(dev, K_.dot) + o, std::min<u32>(65535, K_.n_dot - o), c->prm, c->logN,
+                             c->L + c->K, st);
+  }
   RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_sp), K_.n_intt_sp, st));
   RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_down), K_.n_conv_down, c->K, st));
   if (K_.n_ntt_md)

@@ -129,3 +129,11 @@ void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st);
 int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);
+// Column pass only (the row pass is fused into fides_launch_row_dot).
+int fides_launch_ntt_col(int logN, const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F,
+                         hipStream_t st);
+// Row pass of the ModUp outputs fused with the key-switch inner product; all
+// tasks of one launch have `beta` digits (1..ROWDOT_MAXB).
+constexpr int ROWDOT_MAXB = 4;
+int fides_launch_row_dot(int logN, const DotTask* tasks, u32 n, int beta, const NttTables& T, u32 kstride,
+                         hipStream_t st);

@@ -234,6 +234,118 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
   }
 }
 
+// Row pass fused with the key-switch inner product (replaces the row pass of
+// the ModUp outputs and k_dot_batch).  One 64-thread workgroup owns 64/TPL
+// rows of one output limb t: for every digit j < BETA except the one owning t
+// it loads that digit's column-pass output rows, runs the row stages and keeps
+// the fully reduced values in registers; then per coefficient
+//   o0 = sum_j e_j b_j,  o1 = sum_j e_j a_j   (e_jown = the input limb itself)
+// with the key in Montgomery form, one column accumulation and one reduction
+// (the arithmetic of k_dot_batch, so results are identical).  The ModUp
+// outputs never go back to HBM in evaluation form.
+// One digit of k_ntt_row_dot: the row stages of digit j's rows (or the input
+// limb itself for the digit owning t), fully reduced, into `out`.
+template <int LOGN>
+__device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, u64 rbase, u32 row, u32 r0,
+                                              u64* L, const u64* tw, const u64* tws, u64 q, u64 q2,
+                                              u64 (&out)[NttCfg<LOGN, 1>::E]) {
+  using C = NttCfg<LOGN, 1>;
+  if (j == tk.jown) {  // uniform per workgroup
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) out[k] = tk.d[rbase + r0 + C::TPL * k];
+    return;
+  }
+  const u64* rowp = tk.e + j * es + rbase;
+  u64 x[C::E];
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) x[k] = rowp[r0 + C::TPL * k];
+#pragma unroll
+  for (int u = 0; u < C::A; ++u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
+      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+  __syncthreads();  // earlier readers of this workgroup's LDS are done
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = x[k];
+  __syncthreads();
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[C::lx((r1 << C::B) + k)];
+#pragma unroll
+    for (int v = 0; v < C::B; ++v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
+        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) out[k] = csub(csub(L[C::lx(r0 + C::TPL * k)], q2), q);
+}
+
+template <int LOGN, int BETA>
+__global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ tasks, NttTables T, u32 kstride) {
+  using C = NttCfg<LOGN, 1>;
+  constexpr int RPW = 64 / C::TPL;  // rows per workgroup
+  __shared__ u64 lds[RPW * C::LDS_LINE];
+  const DotTask tk = tasks[blockIdx.y];
+  const u32 pid = tk.pid;
+  const PrimeC P = T.prm[pid];
+  const u64 q = P.q, q2 = 2 * q;
+  const u64* tw = T.psi + ((u64)pid << LOGN);
+  const u64* tws = T.psi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 r0 = tid % C::TPL, ll = tid / C::TPL;
+  const u32 row = blockIdx.x * RPW + ll;
+  const u64 es = (u64)tk.estride << LOGN, ks = (u64)kstride << LOGN;
+  const u64 rbase = (u64)row << C::L2;
+  u64* L = lds + ll * C::LDS_LINE;
+  u64 e0[C::E], e1[C::E], e2[C::E], e3[C::E];
+  row_dot_digit<LOGN>(tk, 0, es, rbase, row, r0, L, tw, tws, q, q2, e0);
+  if constexpr (BETA > 1) row_dot_digit<LOGN>(tk, 1, es, rbase, row, r0, L, tw, tws, q, q2, e1);
+  if constexpr (BETA > 2) row_dot_digit<LOGN>(tk, 2, es, rbase, row, r0, L, tw, tws, q, q2, e2);
+  if constexpr (BETA > 3) row_dot_digit<LOGN>(tk, 3, es, rbase, row, r0, L, tw, tws, q, q2, e3);
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) {
+    const u64 i = rbase + r0 + C::TPL * k;
+    Acc128 a0, a1;
+    acc_mac_v(a0, e0[k], tk.kb[i]);
+    acc_mac_v(a1, e0[k], tk.ka[i]);
+    if constexpr (BETA > 1) {
+      acc_mac_v(a0, e1[k], tk.kb[ks + i]);
+      acc_mac_v(a1, e1[k], tk.ka[ks + i]);
+    }
+    if constexpr (BETA > 2) {
+      acc_mac_v(a0, e2[k], tk.kb[2 * ks + i]);
+      acc_mac_v(a1, e2[k], tk.ka[2 * ks + i]);
+    }
+    if constexpr (BETA > 3) {
+      acc_mac_v(a0, e3[k], tk.kb[3 * ks + i]);
+      acc_mac_v(a1, e3[k], tk.ka[3 * ks + i]);
+    }
+    u64 h, l;
+    acc_value(a0, h, l);
+    tk.o0[i] = mont_redc(h, l, P);
+    acc_value(a1, h, l);
+    tk.o1[i] = mont_redc(h, l, P);
+  }
+}
+
 // ------------------------------------------------------------------ inverse
 // Row pass: global stages LOGN-1 .. L1 (descending).  [0, q) -> [0, 2q).
 template <int LOGN>
@@ -399,8 +511,64 @@ static void launch_inv(const LimbSpan& s, const NttTables& T, hipStream_t st) {
   k_intt_col<LOGN><<<g0, C0::THREADS, 0, st>>>(s, T);
 }
 
+template <int LOGN>
+static void launch_col(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
+  using C0 = NttCfg<LOGN, 0>;
+  dim3 g0(C0::NLINES / C0::LPW, s.total);
+  if (pro == PRO_SWITCH_MODULUS)
+    k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else if (pro == PRO_MDRS)
+    k_ntt_fwd_col<LOGN, PRO_MDRS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else
+    k_ntt_fwd_col<LOGN, PRO_NONE><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+}
+
+template <int LOGN>
+static void launch_row_dot(const DotTask* tasks, u32 n, int beta, const NttTables& T, u32 kstride, hipStream_t st) {
+  using C1 = NttCfg<LOGN, 1>;
+  dim3 g(C1::NLINES / (64 / C1::TPL), n);
+  switch (beta) {
+    case 1: k_ntt_row_dot<LOGN, 1><<<g, 64, 0, st>>>(tasks, T, kstride); break;
+    case 2: k_ntt_row_dot<LOGN, 2><<<g, 64, 0, st>>>(tasks, T, kstride); break;
+    case 3: k_ntt_row_dot<LOGN, 3><<<g, 64, 0, st>>>(tasks, T, kstride); break;
+    default: k_ntt_row_dot<LOGN, 4><<<g, 64, 0, st>>>(tasks, T, kstride); break;
+  }
+}
+
 // Host entry points (called by the context code).  Return 0 or -1 for an
 // unsupported ring dimension.
+int fides_launch_ntt_col(int logN, const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F,
+                         hipStream_t st) {
+  if (s.total == 0) return 0;
+  switch (logN) {
+    case 10: launch_col<10>(s, T, pro, F, st); return 0;
+    case 11: launch_col<11>(s, T, pro, F, st); return 0;
+    case 12: launch_col<12>(s, T, pro, F, st); return 0;
+    case 13: launch_col<13>(s, T, pro, F, st); return 0;
+    case 14: launch_col<14>(s, T, pro, F, st); return 0;
+    case 15: launch_col<15>(s, T, pro, F, st); return 0;
+    case 16: launch_col<16>(s, T, pro, F, st); return 0;
+    case 17: launch_col<17>(s, T, pro, F, st); return 0;
+  }
+  return -1;
+}
+
+int fides_launch_row_dot(int logN, const DotTask* tasks, u32 n, int beta, const NttTables& T, u32 kstride,
+                         hipStream_t st) {
+  if (n == 0) return 0;
+  if (beta < 1 || beta > ROWDOT_MAXB) return -1;
+  switch (logN) {
+    case 10: launch_row_dot<10>(tasks, n, beta, T, kstride, st); return 0;
+    case 11: launch_row_dot<11>(tasks, n, beta, T, kstride, st); return 0;
+    case 12: launch_row_dot<12>(tasks, n, beta, T, kstride, st); return 0;
+    case 13: launch_row_dot<13>(tasks, n, beta, T, kstride, st); return 0;
+    case 14: launch_row_dot<14>(tasks, n, beta, T, kstride, st); return 0;
+    case 15: launch_row_dot<15>(tasks, n, beta, T, kstride, st); return 0;
+    case 16: launch_row_dot<16>(tasks, n, beta, T, kstride, st); return 0;
+    case 17: launch_row_dot<17>(tasks, n, beta, T, kstride, st); return 0;
+  }
+  return -1;
+}
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st) {
   if (s.total == 0) return 0;
