@@ -200,6 +200,7 @@ struct fides_context {
   TaskSlot ring[RING];
   int ring_next = 0;
   hipStream_t copy_stream = nullptr;  // task-list uploads, overlapped with the previous operation's kernels
+  bool conv_split = false;            // every prime < 2^60: carry-free basis conversion (k_conv_batch)
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -256,7 +257,8 @@ static void make_conv(const fides_context* c, const std::vector<u32>& src, const
       u64 prod = 1;
       for (u32 j = 0; j < d.ns; ++j)
         if (j != i) prod = mulm(prod, c->primes[src[j]] % pt, pt);
-      d.mat[t][i] = mulm(prod, R, pt);
+      const u64 m = mulm(prod, R, pt);
+      d.mat[t][i] = c->conv_split ? ((m & 0x3fffffffull) | ((m >> 30) << 32)) : m;
     }
   }
 }
@@ -309,6 +311,7 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   c->dnum = P->dnum;
   c->alpha = alpha;
   c->primes = pr;
+  c->conv_split = std::all_of(pr.begin(), pr.end(), [](u64 q) { return q < (1ull << 60); });
   const u32 NP = P->L + P->K, N = c->N, L = c->L, K = c->K;
   c->roots.resize(NP);
   for (u32 i = 0; i < NP; ++i) {
@@ -616,7 +619,7 @@ static int launch_intt_list(fides_context* c, const NttTask* list, u32 n, hipStr
 }
 static int launch_conv_list(fides_context* c, const ConvTask* list, u32 n, u32 ns, hipStream_t st) {
   for (u32 o = 0; o < n; o += 65535)
-    if (fides_launch_conv_batch(list + o, std::min<u32>(65535, n - o), ns, c->prm, c->logN, st))
+    if (fides_launch_conv_batch(list + o, std::min<u32>(65535, n - o), ns, c->conv_split, c->prm, c->logN, st))
       return fail(FIDES_ERR_UNSUPPORTED, "conversion size");
   return FIDES_OK;
 }

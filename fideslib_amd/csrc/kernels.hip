@@ -46,7 +46,13 @@ __global__ __launch_bounds__(256) void k_tensor_batch(const TensorTask* __restri
 // space lets them come in through scalar loads.
 #define CONSTANT_AS __attribute__((address_space(4)))
 
-template <int NS>
+// SPLIT (every prime below 2^60): sources and matrix entries are split into
+// 30-bit halves, so each of the four partial-product columns is a sum of at
+// most 16 products below 2^60 and cannot overflow 64 bits -- four
+// v_mad_u64_u32 per term and no carry tracking.  Matrix entries then hold
+// (m mod 2^30) | (m >> 30) << 32.  Otherwise the column accumulator with
+// counted carries handles primes up to 2^62.
+template <int NS, bool SPLIT>
 __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__ tasks,
                                                     const PrimeC* __restrict__ prm_, u32 logN) {
   const CONSTANT_AS ConvTask* tp = (const CONSTANT_AS ConvTask*)(tasks + blockIdx.y);
@@ -56,6 +62,36 @@ __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__
   u64* __restrict__ out = tp->out;
   const u32 t0 = tp->t0, t1 = tp->t1;
   const u32 k = blockIdx.x * 256u + threadIdx.x;
+  if constexpr (SPLIT) {
+    u32 x0[NS], x1[NS];
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const u64 q = prm[d->src_pid[i]].q;
+      const u64 v = shoup(in[((u64)i << logN) + k], d->hatinv[i], d->hatinv_sh[i], q);
+      x0[i] = (u32)v & 0x3fffffffu;
+      x1[i] = (u32)(v >> 30);
+    }
+    for (u32 t = t0; t < t1; ++t) {
+      PrimeC P;
+      P.q = prm[d->dst_pid[t]].q;
+      P.qneg_inv = prm[d->dst_pid[t]].qneg_inv;
+      u64 L = 0, M1 = 0, M2 = 0, H = 0;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) {
+        const u64 m = d->mat[t][i];
+        const u32 m0 = (u32)m, m1 = (u32)(m >> 32);
+        L += (u64)x0[i] * m0;
+        M1 += (u64)x0[i] * m1;
+        M2 += (u64)x1[i] * m0;
+        H += (u64)x1[i] * m1;
+      }
+      // T = L + (M1 + M2) 2^30 + H 2^60
+      const unsigned __int128 T = (unsigned __int128)L + (((unsigned __int128)M1 + M2) << 30) +
+                                  ((unsigned __int128)H << 60);
+      out[((u64)d->dst_off[t] << logN) + k] = mont_redc((u64)(T >> 64), (u64)T, P);
+    }
+    return;
+  }
   u64 xs[NS];
 #pragma unroll
   for (int i = 0; i < NS; ++i) {
@@ -217,12 +253,18 @@ void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* pr
   k_axpy_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
 }
 
-int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, const PrimeC* prm, u32 logN, hipStream_t st) {
+int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, bool split, const PrimeC* prm, u32 logN,
+                            hipStream_t st) {
   if (ntasks == 0) return 0;
   dim3 g((1u << logN) / 256, ntasks);
   switch (ns) {
-#define CASE(n) \
-  case n: k_conv_batch<n><<<g, 256, 0, st>>>(tasks, prm, logN); return 0;
+#define CASE(n)                                                             \
+  case n:                                                                   \
+    if (split)                                                              \
+      k_conv_batch<n, true><<<g, 256, 0, st>>>(tasks, prm, logN);           \
+    else                                                                    \
+      k_conv_batch<n, false><<<g, 256, 0, st>>>(tasks, prm, logN);          \
+    return 0;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
     CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
 #undef CASE
