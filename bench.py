@@ -310,12 +310,13 @@ SERVE_ARENA_GIB = 64     # admission cap: ciphertext storage of admitted request
 def bench_keys(prm, w, boot):
     """Resident synthetic keys for every automorphism the serving mix and the
     bootstrap use: rotations by 2^i (rotate-and-sum), the bootstrap's
-    baby/giant steps and conjugation (distinct HBM copies of one synthetic key)."""
+    baby/giant steps and conjugation, and the two switching keys of the
+    sparse-secret encapsulation (distinct HBM copies of one synthetic key)."""
     N = prm.N
     idx = {pow(5, 1 << i, 2 * N) for i in range(SERVE_KMAX)}
     idx |= {pow(5, r, 2 * N) for r in boot.rotations()}
     idx.add(2 * N - 1)
-    idx = sorted(idx)
+    idx = sorted(idx) + boot.switch_keys()
     if len(idx) > len(w.rot_keys):
         raise RuntimeError("need %d resident keys, have %d" % (len(idx), len(w.rot_keys)))
     return {k: w.rot_keys[i] for i, k in enumerate(idx)}
@@ -340,8 +341,9 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
     srv.run(reqs(n_boot), realtime=False)
     wall = srv.stats["wall_s"]
     return {"workload": "cfg4: %d full-slot bootstraps (2^15 slots) in one request-loop run, N=2^16, 30 limbs "
-                        "(60+29x59 bit), dnum=3, level budget {3,3}, EvalMod Chebyshev degree %d + %d double "
-                        "angles, output 14 limbs" % (n_boot, boot.cfg.degree, boot.cfg.double_angle),
+                        "(60+29x59 bit), dnum=3, level budget {3,3}, sparse-secret encapsulation (h'=%s, "
+                        "K'=%g), EvalMod Chebyshev degree %d + %d double angles, output 14 limbs"
+                        % (n_boot, boot.cfg.encaps_h, boot.cfg.K, boot.cfg.degree, boot.cfg.double_angle),
             "bootstraps_per_s": round(n_boot / wall, 3), "ms_per_bootstrap": round(wall * 1e3 / n_boot, 3),
             "keyswitch_ops": srv.stats["keyswitch_ops"], "steps": srv.stats["steps"],
             "launches": srv.stats["launches"],
@@ -453,7 +455,7 @@ def main():
     ev = HipEvents()
     if args.boot_only:
         from fideslib_amd import bootstrap as BT
-        boot = BT.BootstrapPlan(prm, ctx)
+        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
         boot.build_plaintexts()
         print(json.dumps({"bootstrap": bootstrap_leg(prm, ctx, w, boot, bench_keys(prm, w, boot))}))
         return
@@ -513,7 +515,7 @@ def main():
     if not (args.no_serve and args.no_boot):
         from fideslib_amd import bootstrap as BT
         print("bench: bootstrap plaintexts", file=sys.stderr, flush=True)
-        boot = BT.BootstrapPlan(prm, ctx)
+        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
         boot.build_plaintexts()
         keys = bench_keys(prm, w, boot)
     if not args.no_boot:

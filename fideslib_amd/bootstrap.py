@@ -8,7 +8,11 @@ mathematics and computed on the host):
                 so the sine below is near-linear; undone by an exact integer
                 multiply by 2^c at the end, as the reference's correction factor)
   2. ModRaise   one limb at q_0 -> L limbs: the ciphertext now encrypts
-                t = m + q_0 I(X) with a small integer polynomial I
+                t = m + q_0 I(X) with a small integer polynomial I.  With
+                sparse-secret encapsulation (reference Bootstrap.cu:254-260,
+                592-600) the one-limb ciphertext is first switched to a
+                second, sparser secret s' and switched back to s right after
+                ModRaise, so |I| is bounded by the weight of s', not of s
   3. CoeffToSlot  the inverse of the decoding map, so that slot k holds the
                 complex-packed coefficients t_k + i t_{k+n} (bit-reversed slot
                 order), scaled to y = t / (q_0 K')
@@ -186,6 +190,14 @@ class BootConfig:
     double_angle: int = 3       # r
     K: float = 32.0             # |I| + |m/q_0| bound of the argument
     correction: int = 10        # prescale 2^-c
+    encaps_h: int | None = None  # sparse-secret encapsulation: Hamming weight of s' (None: off)
+
+    @staticmethod
+    def encapsulated(h: int = 32) -> "BootConfig":
+        """Sparse-secret encapsulation with a weight-h secret: |I| stays
+        below ~8 with overwhelming probability for h = 32 (I is a sum of h + 1
+        terms uniform in [-1/2, 1/2]), so K' = 16."""
+        return BootConfig(K=16.0, encaps_h=h)
 
 
 class BootstrapPlan:
@@ -237,6 +249,14 @@ class BootstrapPlan:
 
     def output_level(self) -> int:
         return self.l_raise - self.depth
+
+    # -- switching keys of the sparse-secret encapsulation (key ids of the
+    #    request loop's key table): s -> s' and s' -> s
+    ENCAPS_DOWN = "encaps_down"
+    ENCAPS_UP = "encaps_up"
+
+    def switch_keys(self) -> list:
+        return [self.ENCAPS_DOWN, self.ENCAPS_UP] if self.cfg.encaps_h else []
 
     # -- plaintexts
     def _encode(self, values, level: int, scale: float):
@@ -309,7 +329,12 @@ class BootstrapPlan:
         Pg = Program(self.q, self.prm.logN, self.prm.scale_bits, "bootstrap")
         x = Pg.input(level_in, scale_in)
         x = Pg.mul_real(x, 2.0 ** -cfg.correction)
-        r = Pg.mod_raise(Pg.drop(x, 1), self.l_raise, self.delta)
+        x = Pg.drop(x, 1)
+        if cfg.encaps_h:
+            x = Pg.switch_key(x, self.ENCAPS_DOWN)
+        r = Pg.mod_raise(x, self.l_raise, self.delta)
+        if cfg.encaps_h:
+            r = Pg.switch_key(r, self.ENCAPS_UP)
         for plan, pts in zip(self.cts_plans, self._pts["cts"]):
             r = self._linear(Pg, r, plan, pts)
         cj = Pg.conjugate(r)

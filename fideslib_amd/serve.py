@@ -91,7 +91,7 @@ class Op:
     dst: Reg
     srcs: tuple
     level: int            # level of the inputs
-    arg: tuple = ()       # ROT: (autoIndex,), LIN: (mul, add, sub), DROP: (l_out,)
+    arg: tuple = ()       # ROT: (autoIndex, key id), LIN: (mul, add, sub), DROP: (l_out,)
 
 
 class Program:
@@ -157,10 +157,17 @@ class Program:
 
     def rotate(self, a: Reg, r: int) -> Reg:
         k = pow(5, r % (self.N // 2), 2 * self.N)
-        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (k,))
+        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (k, k))
 
     def conjugate(self, a: Reg) -> Reg:
-        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (2 * self.N - 1,))
+        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (2 * self.N - 1, 2 * self.N - 1))
+
+    def switch_key(self, a: Reg, key_id) -> Reg:
+        """Key switch to another secret (identity automorphism) with the
+        switching key registered under `key_id` (reference
+        Ciphertext::keySwitch, src/CKKS/Ciphertext.cpp, as used by the
+        sparse-secret encapsulation of src/CKKS/Bootstrap.cu:254-260,592-600)."""
+        return self._emit(ROT, self._new(a.level, a.scale), [a], a.level, (1, key_id))
 
     # -- element-wise ops
     def lin(self, a: Reg, mul: int | None = None, b: Reg | None = None, sub: bool = False,
@@ -477,7 +484,7 @@ class GpuBackend:
         self.ctx = ctx
         self.N = ctx.N
         self.rlk = relin_key
-        self.keys = dict(rotation_keys)          # autoIndex -> capi.Key
+        self.keys = dict(rotation_keys)          # key id (autoIndex, or a switching-key name) -> capi.Key
         self.buf = capi.DeviceBuffer(arena_bytes // 8)
         self.base = self.buf.ptr
         self.words = self.buf.nwords
@@ -486,16 +493,17 @@ class GpuBackend:
     def ptr(self, word_off: int) -> int:
         return self.base + 8 * word_off
 
-    def key_for(self, k: int):
+    def key_for(self, k):
         if k not in self.keys:
-            raise ServeError("no rotation key for automorphism index %d" % k)
+            raise ServeError("no key switching key %r" % (k,))
         return self.keys[k]
 
     def hmul(self, a, b, levels, out, rescale):
         self.ctx.hmult_batch(a, b, levels, self.rlk, out, rescale=rescale)
 
-    def rot(self, a, levels, kidx, out):
-        self.ctx.rotate_batch(a, levels, [self.key_for(k) for k in kidx], kidx, out)
+    def rot(self, a, levels, kidx, out, keyid=None):
+        keyid = kidx if keyid is None else keyid
+        self.ctx.rotate_batch(a, levels, [self.key_for(k) for k in keyid], kidx, out)
 
     def lin(self, a, b, levels, mul, add, sub, out):
         self.ctx.lin_batch(a, b, levels, mul, add, sub, out)
@@ -642,7 +650,7 @@ class Server:
             if kind in (HMUL, HMUL_NR):
                 be.hmul(s0, [ptr(a, o.srcs[1]) for a, o in g], lv, dst, kind == HMUL)
             elif kind == ROT:
-                be.rot(s0, lv, [o.arg[0] for _, o in g], dst)
+                be.rot(s0, lv, [o.arg[0] for _, o in g], dst, [o.arg[1] for _, o in g])
             elif kind == LIN:
                 be.lin(s0, [ptr(a, o.srcs[1]) if len(o.srcs) > 1 else None for a, o in g], lv,
                        [o.arg[0] for _, o in g], [o.arg[1] for _, o in g], [o.arg[2] for _, o in g], dst)

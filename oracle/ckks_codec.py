@@ -164,6 +164,17 @@ class Keys:
         s_perm = ctx.automorph(self.s_eval, kinv)
         return self._switch_key(s_perm, self.s_eval)
 
+    def encapsulation_keys(self, h: int) -> tuple:
+        """Sparse-secret encapsulation (reference src/CKKS/Bootstrap.cu:254-260,
+        592-600, BootstrapPrecomputation.cuh:37-38): a second ternary secret
+        s' of Hamming weight h and the two switching keys, s -> s' (used at
+        the bottom level before ModRaise) and s' -> s (after it)."""
+        ctx = self.ctx
+        sp = sample_ternary(self.rng, ctx.N, h)
+        pids = np.arange(len(ctx.primes), dtype=np.uint32)
+        sp_eval = ctx.ntt_limbs(to_rns(sp, ctx.primes), pids)
+        return self._switch_key(sp_eval, self.s_eval), self._switch_key(self.s_eval, sp_eval)
+
     # ---------------------------------------------------------- enc / dec
     def encrypt(self, m_coef: np.ndarray, l: int) -> np.ndarray:
         """Secret-key encryption of integer coefficients at l Q limbs -> [2][l][N]."""

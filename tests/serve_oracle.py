@@ -22,7 +22,7 @@ def run_oracle(octx, P: S.Program, inputs, rlk, rot_keys, keep=None):
         elif o.kind == S.HMUL_NR:
             y = octx.hmult_relin(x[0], x[1], l, rlk)
         elif o.kind == S.ROT:
-            y = octx.rotate(x[0], l, rot_keys[o.arg[0]], o.arg[0])
+            y = octx.rotate(x[0], l, rot_keys[o.arg[1]], o.arg[0])
         elif o.kind == S.RESCALE:
             y = octx.rescale_ct(x[0], l)
         elif o.kind == S.DROP:

@@ -4,8 +4,11 @@
   input levels served together with a Chebyshev request; every result is
   bit-exact against the oracle running the same program, bit-identical to
   the same request served alone, and decrypts to the message (> 20 bits).
+* N = 2^12, dense ternary secret, sparse-secret encapsulation (weight-32
+  s'): bit-exact against the oracle and > 20 bits.
 * N = 2^16 (30 limbs, h = 192, the headline primes): two bootstraps served
-  together decrypt to their messages; the precision is printed.
+  together decrypt to their messages, without and with sparse-secret
+  encapsulation; the precision is printed.
 """
 import time
 
@@ -31,6 +34,8 @@ def _keys(bp, keys, N):
     for r in bp.rotations() + [None]:
         k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
         rot[k] = keys.rotation_key(k)
+    if bp.cfg.encaps_h:
+        rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP] = keys.encapsulation_keys(bp.cfg.encaps_h)
     return rot
 
 
@@ -72,19 +77,44 @@ def test_bootstrap_small_equals_oracle_and_alone():
     assert np.array_equal(alone.result, reqs[1].result)
 
 
-def test_bootstrap_cfg3_precision():
+def test_bootstrap_small_encapsulated_equals_oracle():
+    prm = params.boot_small()
+    N = prm.N
+    say("bootstrap N=2^12, sparse-secret encapsulation")
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 6, h=None)
+    bp = B.BootstrapPlan(prm, gctx, B.BootConfig.encapsulated(), keep_host=True)
+    rlk = keys.relin_key()
+    rot = _keys(bp, keys, N)
+    be = S.GpuBackend(gctx, gctx.upload_key(rlk), {k: gctx.upload_key(v) for k, v in rot.items()}, 1 << 30)
+    rng = np.random.default_rng(13)
+    work = []
+    for lvl in (2, 9):
+        z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+        work.append((bp.program(lvl), keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), lvl), z))
+    reqs = [S.Request(i, P, 0.0, inputs=[ct], keep_result=True) for i, (P, ct, _) in enumerate(work)]
+    S.Server(be, N).run(reqs, realtime=False)
+    for r, (P, ct, z) in zip(reqs, work):
+        assert r.status == "done"
+        assert np.array_equal(r.result, run_oracle(octx, P, [ct], rlk, rot)), r.rid
+        dec = C.decode(keys.decrypt_coef(r.result, P.output.level), P.output.scale, N)
+        bits = C.precision_bits(dec, z)
+        say("encapsulated bootstrap request %d: %.1f bits" % (r.rid, bits))
+        assert bits > 20, bits
+
+
+@pytest.mark.parametrize("encaps", [False, True])
+def test_bootstrap_cfg3_precision(encaps):
     prm = params.config3()
     N = prm.N
-    say("bootstrap N=2^16: keys")
+    say("bootstrap N=2^16 (encapsulation %s): keys" % encaps)
     t0 = time.time()
     octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
     gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
     keys = C.Keys(octx, 192, h=192)
-    bp = B.BootstrapPlan(prm, gctx)
-    gk = {}
-    for r in bp.rotations() + [None]:
-        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
-        gk[k] = gctx.upload_key(keys.rotation_key(k))
+    bp = B.BootstrapPlan(prm, gctx, B.BootConfig.encapsulated() if encaps else B.BootConfig())
+    gk = {k: gctx.upload_key(v) for k, v in _keys(bp, keys, N).items()}
     be = S.GpuBackend(gctx, gctx.upload_key(keys.relin_key()), gk, 8 << 30)
     say("keys and plan %.1f s; plaintexts" % (time.time() - t0))
     bp.build_plaintexts()

@@ -78,20 +78,22 @@ def test_plan_levels_and_rotations():
         B.BootstrapPlan(params.serve_small(), None)   # 12 limbs: too shallow
 
 
-def _oracle_bootstrap(cfg, seed=3):
+def _oracle_bootstrap(cfg, seed=3, h=64):
     from oracle import ckks_codec as C
     from oracle import oracle as O
     from serve_oracle import run_oracle
     prm = params.boot_small()
     N = prm.N
     octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
-    keys = C.Keys(octx, 5, h=64)
+    keys = C.Keys(octx, 5, h=h)
     bp = B.BootstrapPlan(prm, None, cfg, keep_host=True)
     P = bp.program(3)
     rot = {}
     for r in bp.rotations() + [None]:
         k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
         rot[k] = keys.rotation_key(k)
+    if cfg.encaps_h:
+        rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP] = keys.encapsulation_keys(cfg.encaps_h)
     rng = np.random.default_rng(seed)
     z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
     ct = keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), 3)
@@ -104,6 +106,36 @@ def test_bootstrap_on_the_oracle_decrypts():
     bits, P = _oracle_bootstrap(B.BootConfig())
     assert P.output.level == params.boot_small().L - B.BootstrapPlan(params.boot_small(), None).depth
     assert bits > 20, bits
+
+
+def test_evalmod_polynomial_fp64_encapsulated():
+    cfg = B.BootConfig.encapsulated()
+    bp = B.BootstrapPlan(params.boot_small(), None, cfg)
+    eps = np.linspace(-1e-3, 1e-3, 201)
+    worst = 0.0
+    for I in range(-15, 16):
+        u = I + eps
+        v = np.polynomial.chebyshev.chebval(u / cfg.K, bp.cheb)
+        for _ in range(cfg.double_angle):
+            v = 2 * v * v - 1
+        worst = max(worst, np.abs(v - np.sin(2 * np.pi * u)).max())
+    assert worst < 1e-12
+
+
+def test_bootstrap_encapsulated_dense_secret():
+    """Dense ternary secret (h ~ 2N/3): with sparse-secret encapsulation
+    (weight-32 s') K' = 16 suffices; the program switches keys twice."""
+    cfg = B.BootConfig.encapsulated()
+    bits, P = _oracle_bootstrap(cfg, h=None)
+    assert sum(1 for o in P.ops if o.kind == "rot" and o.arg[0] == 1) == 2
+    assert bits > 20, bits
+
+
+def test_bootstrap_dense_secret_without_encapsulation_fails():
+    # the same K' = 16 without encapsulation: |I| of the dense secret reaches
+    # several times K', the modular reduction is wrong
+    bits, _ = _oracle_bootstrap(B.BootConfig(K=16.0), h=None)
+    assert bits < 5, bits
 
 
 def test_bootstrap_planted_mistake_is_caught():

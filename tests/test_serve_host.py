@@ -59,9 +59,9 @@ class MockBackend:
         self._launch("hmul", reads, list(zip(out, lo)),
                      lambda v: [hash(("hmul", rescale, v[2 * i], v[2 * i + 1], lv[i])) for i in range(len(a))])
 
-    def rot(self, a, lv, k, out):
+    def rot(self, a, lv, k, out, keyid):
         self._launch("rot", list(zip(a, lv)), list(zip(out, lv)),
-                     lambda v: [hash(("rot", k[i], v[i], lv[i])) for i in range(len(a))])
+                     lambda v: [hash(("rot", k[i], keyid[i], v[i], lv[i])) for i in range(len(a))])
 
     def lin(self, a, b, lv, mul, add, sub, out):
         reads, idx = [], []
@@ -132,7 +132,7 @@ def sequential(P: S.Program, input_tokens):
         if o.kind in (S.HMUL, S.HMUL_NR):
             val[o.dst.id] = hash(("hmul", o.kind == S.HMUL, v[0], v[1], o.level))
         elif o.kind == S.ROT:
-            val[o.dst.id] = hash(("rot", o.arg[0], v[0], o.level))
+            val[o.dst.id] = hash(("rot", o.arg[0], o.arg[1], v[0], o.level))
         elif o.kind == S.LIN:
             val[o.dst.id] = hash(("lin", o.arg[0], o.arg[1], o.arg[2], v[0], v[1] if len(v) > 1 else None, o.level))
         elif o.kind == S.RESCALE:
