@@ -54,6 +54,12 @@ SIGNATURES = {
     "fides_copy_to_host": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "fides_copy_device": (_int, [_vp, _vp, ctypes.c_size_t, _vp]),
     "fides_stream_sync": (_int, [_vp]),
+    "fides_stream_create": (_int, [ctypes.POINTER(_vp)]),
+    "fides_stream_destroy": (_int, [_vp]),
+    "fides_graph_begin": (_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t]),
+    "fides_graph_end": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "fides_graph_launch": (_int, [_vp, _vp]),
+    "fides_graph_destroy": (_int, [_vp]),
     "fides_ntt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
     "fides_intt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
     "fides_modup": (_int, [_vp, _vp, _u32, _u32, _vp, _vp]),
@@ -134,12 +140,12 @@ class DeviceBuffer:
     def upload(self, a: np.ndarray, stream=None) -> None:
         a = np.ascontiguousarray(a, dtype=np.uint64)
         assert a.size <= self.nwords
-        check(load().fides_copy_to_device(self.ptr, a.ctypes.data, a.size * 8, stream), "copy_to_device")
+        check(load().fides_copy_to_device(self.ptr, a.ctypes.data, a.size * 8, _sptr(stream)), "copy_to_device")
 
     def download(self, shape=None, stream=None, nwords=None) -> np.ndarray:
         n = self.nwords if nwords is None else nwords
         out = np.empty(n, np.uint64)
-        check(load().fides_copy_to_host(out.ctypes.data, self.ptr, n * 8, stream), "copy_to_host")
+        check(load().fides_copy_to_host(out.ctypes.data, self.ptr, n * 8, _sptr(stream)), "copy_to_host")
         return out.reshape(shape) if shape is not None else out
 
     def at(self, word_offset: int) -> int:
@@ -201,39 +207,39 @@ class Context:
     def ntt(self, buf: DeviceBuffer, pids, stream=None, inverse=False, offset=0):
         pid = np.ascontiguousarray(pids, np.uint32)
         f = load().fides_intt if inverse else load().fides_ntt
-        check(f(self.h, buf.at(offset), pid.ctypes.data_as(_u32p), len(pid), stream), "fides_ntt")
+        check(f(self.h, buf.at(offset), pid.ctypes.data_as(_u32p), len(pid), _sptr(stream)), "fides_ntt")
 
     def intt(self, buf: DeviceBuffer, pids, stream=None, offset=0):
         self.ntt(buf, pids, stream, inverse=True, offset=offset)
 
     def modup(self, inp: DeviceBuffer, l: int, digit: int, out: DeviceBuffer, stream=None):
-        check(load().fides_modup(self.h, inp.ptr, l, digit, out.ptr, stream), "fides_modup")
+        check(load().fides_modup(self.h, inp.ptr, l, digit, out.ptr, _sptr(stream)), "fides_modup")
 
     def moddown(self, inp: DeviceBuffer, l: int, out: DeviceBuffer, stream=None):
-        check(load().fides_moddown(self.h, inp.ptr, l, out.ptr, stream), "fides_moddown")
+        check(load().fides_moddown(self.h, inp.ptr, l, out.ptr, _sptr(stream)), "fides_moddown")
 
     # ---- keys
     def upload_key(self, key: np.ndarray) -> "Key":
         return Key(self, key)
 
     def keyswitch(self, inp, l, key, out0, out1, stream=None):
-        check(load().fides_keyswitch(self.h, inp.ptr, l, key.h, out0.ptr, out1.ptr, stream), "fides_keyswitch")
+        check(load().fides_keyswitch(self.h, inp.ptr, l, key.h, out0.ptr, out1.ptr, _sptr(stream)), "fides_keyswitch")
 
     # ---- ciphertext ops
     def hmult(self, a, b, l, rlk, out, stream=None):
-        check(load().fides_hmult(self.h, _ptr(a), _ptr(b), l, rlk.h, _ptr(out), stream), "fides_hmult")
+        check(load().fides_hmult(self.h, _ptr(a), _ptr(b), l, rlk.h, _ptr(out), _sptr(stream)), "fides_hmult")
 
     def square(self, a, l, rlk, out, stream=None):
-        check(load().fides_square(self.h, _ptr(a), l, rlk.h, _ptr(out), stream), "fides_square")
+        check(load().fides_square(self.h, _ptr(a), l, rlk.h, _ptr(out), _sptr(stream)), "fides_square")
 
     def rescale(self, inp, l, out, stream=None):
-        check(load().fides_rescale(self.h, _ptr(inp), l, _ptr(out), stream), "fides_rescale")
+        check(load().fides_rescale(self.h, _ptr(inp), l, _ptr(out), _sptr(stream)), "fides_rescale")
 
     def rotate(self, inp, l, key, k, out, stream=None):
-        check(load().fides_rotate(self.h, _ptr(inp), l, key.h, k, _ptr(out), stream), "fides_rotate")
+        check(load().fides_rotate(self.h, _ptr(inp), l, key.h, k, _ptr(out), _sptr(stream)), "fides_rotate")
 
     def automorph(self, inp, nlimbs, k, out, stream=None):
-        check(load().fides_automorph(self.h, _ptr(inp), nlimbs, k, _ptr(out), stream), "fides_automorph")
+        check(load().fides_automorph(self.h, _ptr(inp), nlimbs, k, _ptr(out), _sptr(stream)), "fides_automorph")
 
     def hmult_batch(self, a, b, levels, rlk, out, rescale=True, stream=None):
         n = len(levels)
@@ -242,7 +248,7 @@ class Context:
         O = (_vp * n)(*[_ptr(x) for x in out])
         lv = np.ascontiguousarray(levels, np.uint32)
         check(load().fides_hmult_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), rlk.h, int(bool(rescale)), O,
-                                       stream), "fides_hmult_batch")
+                                       _sptr(stream)), "fides_hmult_batch")
 
     def rotate_batch(self, inp, levels, keys, kidx, out, stream=None):
         n = len(levels)
@@ -252,7 +258,7 @@ class Context:
         lv = np.ascontiguousarray(levels, np.uint32)
         kx = np.ascontiguousarray(kidx, np.uint32)
         check(load().fides_rotate_batch(self.h, n, I, lv.ctypes.data_as(_u32p), Kh, kx.ctypes.data_as(_u32p), O,
-                                        stream), "fides_rotate_batch")
+                                        _sptr(stream)), "fides_rotate_batch")
 
     def rotate_hoisted(self, inp, level, keys, kidx, out, stream=None):
         """n rotations of one ciphertext sharing one ModUp; keys[i] may be None when kidx[i] == 1."""
@@ -260,11 +266,11 @@ class Context:
         K = (_vp * n)(*[None if k is None else k.h for k in keys])
         O = (_vp * n)(*[_ptr(x) for x in out])
         ki = np.ascontiguousarray(kidx, np.uint32)
-        check(load().fides_rotate_hoisted(self.h, _ptr(inp), level, n, K, ki.ctypes.data_as(_u32p), O, stream),
+        check(load().fides_rotate_hoisted(self.h, _ptr(inp), level, n, K, ki.ctypes.data_as(_u32p), O, _sptr(stream)),
               "fides_rotate_hoisted")
 
     def plaintext_prepare(self, pt, level, stream=None):
-        check(load().fides_plaintext_prepare(self.h, _ptr(pt), level, stream), "fides_plaintext_prepare")
+        check(load().fides_plaintext_prepare(self.h, _ptr(pt), level, _sptr(stream)), "fides_plaintext_prepare")
 
     def ptdot_batch(self, terms, levels, out, stream=None):
         """out[i] = sum_j ct_ij (*) pt_ij for terms[i] = [(ct, pt), ...] (prepared plaintexts)."""
@@ -276,14 +282,14 @@ class Context:
         O = (_vp * n)(*[_ptr(x) for x in out])
         lv = np.ascontiguousarray(levels, np.uint32)
         check(load().fides_ptdot_batch(self.h, n, cnt.ctypes.data_as(_u32p), C, P, lv.ctypes.data_as(_u32p), O,
-                                       stream), "fides_ptdot_batch")
+                                       _sptr(stream)), "fides_ptdot_batch")
 
     def mod_raise_batch(self, inp, l_out, out, stream=None):
         n = len(l_out)
         I = (_vp * n)(*[_ptr(x) for x in inp])
         O = (_vp * n)(*[_ptr(x) for x in out])
         lo = np.ascontiguousarray(l_out, np.uint32)
-        check(load().fides_mod_raise_batch(self.h, n, I, lo.ctypes.data_as(_u32p), O, stream),
+        check(load().fides_mod_raise_batch(self.h, n, I, lo.ctypes.data_as(_u32p), O, _sptr(stream)),
               "fides_mod_raise_batch")
 
     def rescale_batch(self, inp, levels, out, stream=None):
@@ -291,7 +297,7 @@ class Context:
         I = (_vp * n)(*[_ptr(x) for x in inp])
         O = (_vp * n)(*[_ptr(x) for x in out])
         lv = np.ascontiguousarray(levels, np.uint32)
-        check(load().fides_rescale_batch(self.h, n, I, lv.ctypes.data_as(_u32p), O, stream), "fides_rescale_batch")
+        check(load().fides_rescale_batch(self.h, n, I, lv.ctypes.data_as(_u32p), O, _sptr(stream)), "fides_rescale_batch")
 
     def add_batch(self, a, b, levels, out, subtract=False, stream=None):
         n = len(levels)
@@ -299,7 +305,7 @@ class Context:
         B = (_vp * n)(*[_ptr(x) for x in b])
         O = (_vp * n)(*[_ptr(x) for x in out])
         lv = np.ascontiguousarray(levels, np.uint32)
-        check(load().fides_add_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), int(bool(subtract)), O, stream),
+        check(load().fides_add_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), int(bool(subtract)), O, _sptr(stream)),
               "fides_add_batch")
 
     def lin_batch(self, a, b, levels, mul, add, sub, out, stream=None):
@@ -324,7 +330,7 @@ class Context:
         M, C = res(mul), res(add)
         S = np.ascontiguousarray([1 if x else 0 for x in sub], np.uint32)
         check(load().fides_lin_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), M, C, S.ctypes.data_as(_u32p), O,
-                                     stream), "fides_lin_batch")
+                                     _sptr(stream)), "fides_lin_batch")
 
     def lincomb_batch(self, terms, levels, add, out, stream=None):
         """out[i] = sum_j m_ij * x_ij + add[i]; terms[i] = [(x, x_level, m), ...] with Python-int
@@ -349,7 +355,7 @@ class Context:
         O = (_vp * n)(*[_ptr(x) for x in out])
         lv = np.ascontiguousarray(levels, np.uint32)
         check(load().fides_lincomb_batch(self.h, n, cnt.ctypes.data_as(_u32p), X, XL.ctypes.data_as(_u32p), M, A,
-                                         lv.ctypes.data_as(_u32p), O, stream), "fides_lincomb_batch")
+                                         lv.ctypes.data_as(_u32p), O, _sptr(stream)), "fides_lincomb_batch")
 
     def drop_limbs_batch(self, inp, l_in, l_out, out, stream=None):
         n = len(l_in)
@@ -358,7 +364,7 @@ class Context:
         li = np.ascontiguousarray(l_in, np.uint32)
         lo = np.ascontiguousarray(l_out, np.uint32)
         check(load().fides_drop_limbs_batch(self.h, n, I, li.ctypes.data_as(_u32p), lo.ctypes.data_as(_u32p), O,
-                                            stream), "fides_drop_limbs_batch")
+                                            _sptr(stream)), "fides_drop_limbs_batch")
 
     def _const_call(self, fn, a, levels, consts, out, stream, what):
         n = len(levels)
@@ -368,7 +374,7 @@ class Context:
         res = [np.ascontiguousarray([int(c) % self.q[j] for j in range(min(l, self.L))], np.uint64)
                for c, l in zip(consts, levels)]
         R = (_u64p * n)(*[r.ctypes.data_as(_u64p) for r in res])
-        check(fn(self.h, n, A, lv.ctypes.data_as(_u32p), R, O, stream), what)
+        check(fn(self.h, n, A, lv.ctypes.data_as(_u32p), R, O, _sptr(stream)), what)
 
     def mul_const_batch(self, a, levels, consts, out, stream=None):
         """out[i] = a[i] * consts[i] for Python integers (any sign or size)."""
@@ -379,10 +385,68 @@ class Context:
         self._const_call(load().fides_add_const_batch, a, levels, consts, out, stream, "fides_add_const_batch")
 
     def copy(self, dst, src, nwords, stream=None):
-        check(load().fides_copy_device(_ptr(dst), _ptr(src), int(nwords) * 8, stream), "fides_copy_device")
+        check(load().fides_copy_device(_ptr(dst), _ptr(src), int(nwords) * 8, _sptr(stream)), "fides_copy_device")
 
     def sync(self, stream=None):
-        check(load().fides_stream_sync(stream), "sync")
+        check(load().fides_stream_sync(_sptr(stream)), "sync")
+
+    # -- graph capture: operations issued on `stream` until graph_end are
+    #    recorded, not run
+    def graph_begin(self, stream: "Stream", scratch_bytes: int, task_bytes: int) -> None:
+        check(load().fides_graph_begin(self.h, stream.h, int(scratch_bytes), int(task_bytes)), "fides_graph_begin")
+
+    def graph_end(self) -> "Graph":
+        h = _vp()
+        check(load().fides_graph_end(self.h, ctypes.byref(h)), "fides_graph_end")
+        return Graph(h.value)
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        h = _vp()
+        check(load().fides_stream_create(ctypes.byref(h)), "fides_stream_create")
+        self.h = h.value
+
+    def sync(self):
+        check(load().fides_stream_sync(self.h), "sync")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fides_stream_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """A captured sequence of operations (see Context.graph_begin)."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self, stream: "Stream") -> None:
+        check(load().fides_graph_launch(self.h, stream.h), "fides_graph_launch")
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().fides_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _sptr(stream):
+    return stream.h if isinstance(stream, Stream) else stream
 
 
 def _ptr(x) -> int:

@@ -180,6 +180,20 @@ struct TaskSlot {
 };
 constexpr size_t SLOT_BYTES = 8u << 20;  // initial capacity of every task slot
 
+// Capture of a sequence of operations into one HIP graph.  While capturing,
+// task lists go to a pool owned by the graph (copied eagerly on the copy
+// stream, so the graph holds kernel nodes only) and the key-switch scratch
+// comes from an arena owned by the graph, sized up front.
+struct Capture {
+  hipStream_t st = nullptr;
+  char* pool = nullptr;
+  size_t pool_cap = 0, pool_used = 0;
+  u64* arena = nullptr;
+  size_t arena_words = 0;
+  u64* eager_arena = nullptr;  // the context's own arena, restored at the end
+  size_t eager_words = 0;
+};
+
 struct fides_context {
   u32 logN, N, L, K, dnum, alpha;
   std::vector<u64> primes, roots;
@@ -201,6 +215,7 @@ struct fides_context {
   int ring_next = 0;
   hipStream_t copy_stream = nullptr;  // task-list uploads, overlapped with the previous operation's kernels
   bool conv_split = false;            // every prime < 2^60: carry-free basis conversion (k_conv_batch)
+  Capture* cap = nullptr;             // graph capture in progress
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -271,6 +286,16 @@ static u32 num_digits(const fides_context* c, u32 l) { return (l + c->alpha - 1)
 
 extern "C" void fides_context_destroy(fides_context* c) {
   if (!c) return;
+  if (c->cap) {  // capture left open: discard it
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->cap->st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    if (c->cap->pool) (void)hipFree(c->cap->pool);
+    if (c->cap->arena) (void)hipFree(c->cap->arena);
+    c->arena = c->cap->eager_arena;
+    delete c->cap;
+    c->cap = nullptr;
+  }
   (void)hipDeviceSynchronize();
   for (void* p : c->allocs) (void)hipFree(p);
   if (c->arena) (void)hipFree(c->arena);
@@ -528,6 +553,104 @@ extern "C" int fides_stream_sync(void* stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   return FIDES_OK;
 }
+extern "C" int fides_stream_create(void** stream) {
+  if (!stream) return fail(FIDES_ERR_ARG, "null argument");
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return FIDES_OK;
+}
+extern "C" int fides_stream_destroy(void* stream) {
+  if (stream) HIPCHK(hipStreamDestroy((hipStream_t)stream));
+  return FIDES_OK;
+}
+
+// ======================================================================
+// graph capture
+// ======================================================================
+struct fides_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<void*> mem;  // task pool and scratch arena of the captured operations
+};
+
+static void capture_free(Capture* g) {
+  if (g->pool) (void)hipFree(g->pool);
+  if (g->arena) (void)hipFree(g->arena);
+  delete g;
+}
+
+extern "C" int fides_graph_begin(fides_context* c, void* stream, size_t scratch_bytes, size_t task_bytes) {
+  if (!c || !stream) return fail(FIDES_ERR_ARG, "graph capture needs a context and a (non-default) stream");
+  if (c->cap) return fail(FIDES_ERR_ARG, "a graph capture is already in progress on this context");
+  if (task_bytes == 0 || scratch_bytes == 0) return fail(FIDES_ERR_ARG, "empty graph pools");
+  Capture* g = new Capture();
+  g->st = (hipStream_t)stream;
+  if (hipMalloc((void**)&g->pool, task_bytes) != hipSuccess ||
+      hipMalloc((void**)&g->arena, scratch_bytes) != hipSuccess) {
+    capture_free(g);
+    return fail(FIDES_ERR_NOMEM, "graph pool allocation");
+  }
+  g->pool_cap = task_bytes;
+  g->arena_words = scratch_bytes / 8;
+  hipError_t e = hipStreamBeginCapture(g->st, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) {
+    capture_free(g);
+    return fail(FIDES_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+  }
+  g->eager_arena = c->arena;
+  g->eager_words = c->arena_words;
+  c->arena = g->arena;
+  c->arena_words = g->arena_words;
+  c->cap = g;
+  return FIDES_OK;
+}
+
+extern "C" int fides_graph_end(fides_context* c, fides_graph** out) {
+  if (!c || !out) return fail(FIDES_ERR_ARG, "null argument");
+  if (!c->cap) return fail(FIDES_ERR_ARG, "no graph capture in progress");
+  Capture* g = c->cap;
+  c->cap = nullptr;
+  c->arena = g->eager_arena;
+  c->arena_words = g->eager_words;
+  hipGraph_t graph = nullptr;
+  hipError_t e = hipStreamEndCapture(g->st, &graph);
+  if (e != hipSuccess || !graph) {
+    if (graph) (void)hipGraphDestroy(graph);
+    capture_free(g);
+    return fail(FIDES_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  hipGraphExec_t exec = nullptr;
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    capture_free(g);
+    return fail(FIDES_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  fides_graph* G = new fides_graph();
+  G->graph = graph;
+  G->exec = exec;
+  G->mem = {g->pool, g->arena};
+  delete g;
+  *out = G;
+  return FIDES_OK;
+}
+
+extern "C" int fides_graph_launch(fides_graph* g, void* stream) {
+  if (!g) return fail(FIDES_ERR_ARG, "null graph");
+  HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return FIDES_OK;
+}
+
+extern "C" int fides_graph_destroy(fides_graph* g) {
+  if (!g) return FIDES_OK;
+  (void)hipDeviceSynchronize();  // no launch of the graph may still read its pools
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  for (void* p : g->mem) (void)hipFree(p);
+  delete g;
+  return FIDES_OK;
+}
 
 // ======================================================================
 // executor: task blob + arena
@@ -547,7 +670,21 @@ struct Blob {
 };
 
 // Upload a blob through the next ring slot; returns its device base.
+// During a graph capture the blob goes to the graph's task pool instead.
 static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char** dev, int* slot_out) {
+  if (c->cap) {
+    Capture& g = *c->cap;
+    if (st != g.st) return fail(FIDES_ERR_ARG, "operation on another stream during a graph capture");
+    const size_t off = (g.pool_used + 255) & ~(size_t)255;
+    if (off + blob.b.size() > g.pool_cap)
+      return fail(FIDES_ERR_NOMEM, "graph task pool exhausted (task_bytes of fides_graph_begin)");
+    HIPCHK(hipMemcpyAsync(g.pool + off, blob.b.data(), blob.b.size(), hipMemcpyHostToDevice, c->copy_stream));
+    HIPCHK(hipStreamSynchronize(c->copy_stream));
+    g.pool_used = off + blob.b.size();
+    *dev = g.pool + off;
+    *slot_out = -1;
+    return FIDES_OK;
+  }
   TaskSlot& s = c->ring[c->ring_next];
   *slot_out = c->ring_next;
   c->ring_next = (c->ring_next + 1) % RING;
@@ -574,6 +711,7 @@ static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char*
 }
 
 static int finish_slot(fides_context* c, int slot, hipStream_t st) {
+  if (slot < 0) return FIDES_OK;  // graph pool
   TaskSlot& s = c->ring[slot];
   HIPCHK(hipEventRecord(s.done, st));
   s.used = true;
@@ -581,6 +719,13 @@ static int finish_slot(fides_context* c, int slot, hipStream_t st) {
 }
 
 static int ensure_arena(fides_context* c, size_t words, hipStream_t st) {
+  if (c->cap) {
+    if (st != c->cap->st) return fail(FIDES_ERR_ARG, "operation on another stream during a graph capture");
+    if (words > c->arena_words)
+      return fail(FIDES_ERR_NOMEM, "graph scratch arena too small: " + std::to_string(words * 8) +
+                                       " bytes needed (scratch_bytes of fides_graph_begin)");
+    return FIDES_OK;
+  }
   if (words <= c->arena_words) return FIDES_OK;
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipDeviceSynchronize());

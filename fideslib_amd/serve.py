@@ -476,9 +476,10 @@ class Arena:
 
 # ----------------------------------------------------------------- backends
 class GpuBackend:
-    """Issues batched C ABI calls on the context's stream (the default stream)."""
+    """Issues batched C ABI calls on one stream: the default stream, or a
+    capi.Stream (needed for graph capture)."""
 
-    def __init__(self, ctx, relin_key, rotation_keys: dict, arena_bytes: int):
+    def __init__(self, ctx, relin_key, rotation_keys: dict, arena_bytes: int, stream=None):
         from . import capi
         self.capi = capi
         self.ctx = ctx
@@ -488,6 +489,8 @@ class GpuBackend:
         self.buf = capi.DeviceBuffer(arena_bytes // 8)
         self.base = self.buf.ptr
         self.words = self.buf.nwords
+        self.stream = stream
+        self.st = stream.h if stream is not None else None
         self.hip = _Hip()
 
     def ptr(self, word_off: int) -> int:
@@ -499,51 +502,70 @@ class GpuBackend:
         return self.keys[k]
 
     def hmul(self, a, b, levels, out, rescale):
-        self.ctx.hmult_batch(a, b, levels, self.rlk, out, rescale=rescale)
+        self.ctx.hmult_batch(a, b, levels, self.rlk, out, rescale=rescale, stream=self.st)
 
     def rot(self, a, levels, kidx, out, keyid=None):
         keyid = kidx if keyid is None else keyid
-        self.ctx.rotate_batch(a, levels, [self.key_for(k) for k in keyid], kidx, out)
+        self.ctx.rotate_batch(a, levels, [self.key_for(k) for k in keyid], kidx, out, stream=self.st)
 
     def lin(self, a, b, levels, mul, add, sub, out):
-        self.ctx.lin_batch(a, b, levels, mul, add, sub, out)
+        self.ctx.lin_batch(a, b, levels, mul, add, sub, out, stream=self.st)
 
     def rescale(self, a, levels, out):
-        self.ctx.rescale_batch(a, levels, out)
+        self.ctx.rescale_batch(a, levels, out, stream=self.st)
 
     def drop(self, a, l_in, l_out, out):
-        self.ctx.drop_limbs_batch(a, l_in, l_out, out)
+        self.ctx.drop_limbs_batch(a, l_in, l_out, out, stream=self.st)
 
     def ptdot(self, terms, levels, out):
-        self.ctx.ptdot_batch(terms, levels, out)
+        self.ctx.ptdot_batch(terms, levels, out, stream=self.st)
 
     def lincomb(self, terms, levels, add, out):
-        self.ctx.lincomb_batch(terms, levels, add, out)
+        self.ctx.lincomb_batch(terms, levels, add, out, stream=self.st)
 
     def modraise(self, a, l_out, out):
-        self.ctx.mod_raise_batch(a, l_out, out)
+        self.ctx.mod_raise_batch(a, l_out, out, stream=self.st)
 
     def upload(self, ptr, host):
         arr = np.ascontiguousarray(host, np.uint64)
-        self.capi.check(self.capi.load().fides_copy_to_device(ptr, arr.ctypes.data, arr.size * 8, None), "upload")
+        self.capi.check(self.capi.load().fides_copy_to_device(ptr, arr.ctypes.data, arr.size * 8, self.st), "upload")
+
+    def stage(self, host):
+        """A device copy of a host array (graph-capture inputs)."""
+        return self.capi.DeviceBuffer.from_array(np.ascontiguousarray(host, np.uint64), stream=self.st)
 
     def copy(self, dst, src, words):
-        self.capi.check(self.capi.load().fides_copy_device(dst, src, words * 8, None), "copy")
+        self.capi.check(self.capi.load().fides_copy_device(dst, src, words * 8, self.st), "copy")
 
     def download(self, ptr, words) -> np.ndarray:
         out = np.empty(words, np.uint64)
-        self.capi.check(self.capi.load().fides_copy_to_host(out.ctypes.data, ptr, words * 8, None), "download")
+        self.capi.check(self.capi.load().fides_copy_to_host(out.ctypes.data, ptr, words * 8, self.st), "download")
         return out
 
     def record(self):
-        return self.hip.record()
+        return self.hip.record(self.st)
+
+    def sync(self) -> None:
+        self.capi.check(self.capi.load().fides_stream_sync(self.st), "sync")
+
+    # -- graph capture of the issued operations (capi.Context.graph_begin)
+    def begin_capture(self, scratch_bytes: int, task_bytes: int) -> None:
+        if self.stream is None:
+            raise ServeError("graph capture needs a backend created with a stream")
+        self.ctx.graph_begin(self.stream, scratch_bytes, task_bytes)
+
+    def end_capture(self):
+        return self.ctx.graph_end()
+
+    def launch(self, graph) -> None:
+        graph.launch(self.stream)
 
     def wait(self, ev) -> None:
         self.hip.wait(ev)
 
 
 class _Hip:
-    """HIP events on the default stream (plumbing for the step pipeline)."""
+    """HIP events (plumbing for the step pipeline)."""
 
     def __init__(self):
         self.lib = ctypes.CDLL("libamdhip64.so")
@@ -553,14 +575,14 @@ class _Hip:
         self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
         self.pool = []
 
-    def record(self):
+    def record(self, stream=None):
         if self.pool:
             e = self.pool.pop()
         else:
             e = ctypes.c_void_p()
             if self.lib.hipEventCreateWithFlags(ctypes.byref(e), 2) != 0:  # hipEventDisableTiming
                 raise ServeError("hipEventCreate failed")
-        if self.lib.hipEventRecord(e, None) != 0:
+        if self.lib.hipEventRecord(e, stream) != 0:
             raise ServeError("hipEventRecord failed")
         return e
 
@@ -697,6 +719,57 @@ class Server:
         req.done_at = now
         req.status = "done"
 
+    def capture(self, requests, scratch_bytes: int = 4 << 30, task_bytes: int = 512 << 20) -> "CapturedBatch":
+        """A closed batch recorded into one HIP graph: every request gets its
+        arena storage now, then the copy of the inputs into it (one copy
+        kernel) and every step of the same schedule `run` would issue are
+        captured instead of run.  Device inputs ("dev", ptr) are read at
+        every replay from where they are; host inputs are staged in device
+        buffers owned by the batch.  `CapturedBatch.replay()` executes it;
+        results are read after a replay.  All requests must fit the arena
+        at once."""
+        active, staged, src, dst, lv = [], [], [], [], []
+        try:
+            for r in requests:
+                r.compiled = _compile(r.program)
+                off = self.arena.alloc(r.compiled.block)
+                if off is None:
+                    raise ServeError("captured requests do not fit the arena together")
+                a = _Active(r, r.compiled, off, self.N)
+                active.append(a)
+                for reg, x in zip(r.program.inputs, r.inputs):
+                    if isinstance(x, tuple) and x[0] == "dev":
+                        src.append(x[1])
+                    else:
+                        buf = self.be.stage(x)
+                        staged.append(buf)
+                        src.append(buf.ptr)
+                    dst.append(self.be.ptr(a.reg_ptr_words(reg)))
+                    lv.append(reg.level)
+                r.status = "running"
+            self.be.sync()
+            self.be.begin_capture(scratch_bytes, task_bytes)
+            try:
+                if src:
+                    self.be.drop(src, lv, lv, dst)    # the input copy, part of every replay
+                step_no = 0
+                while any(a.ready for a in active):
+                    self._step(active, step_no)
+                    step_no += 1
+                    self.stats["steps"] += 1
+            except BaseException:
+                try:
+                    self.be.end_capture().close()
+                except Exception:
+                    pass
+                raise
+            graph = self.be.end_capture()
+        except BaseException:
+            for a in active:
+                self.arena.release(a.off, a.comp.block)
+            raise
+        return CapturedBatch(self, graph, active, staged)
+
     def run(self, requests, realtime: bool = True, clock=time.perf_counter):
         """Serve `requests` (arrival times relative to the start).  With
         realtime=False every request is available at once (closed batch)."""
@@ -751,6 +824,35 @@ class Server:
             drain_one()
         self.stats["wall_s"] = now()
         return reqs
+
+
+class CapturedBatch:
+    """Requests recorded into one HIP graph by Server.capture."""
+
+    def __init__(self, server: Server, graph, active, staged):
+        self.server, self.graph, self.active, self.staged = server, graph, active, staged
+
+    def replay(self) -> None:
+        self.server.be.launch(self.graph)
+
+    def results(self) -> list:
+        """Outputs of the last replay (waits for it), in request order."""
+        be = self.server.be
+        be.sync()
+        out = []
+        for a in self.active:
+            o = a.req.program.output
+            out.append(be.download(be.ptr(a.reg_ptr_words(o)), o.words * be.N).reshape(2, o.level, be.N))
+        return out
+
+    def close(self) -> None:
+        if self.graph is not None:
+            self.graph.close()
+            self.graph = None
+            for a in self.active:
+                self.server.arena.release(a.off, a.comp.block)
+            self.active = []
+            self.staged = []
 
 
 def latency_summary(reqs) -> dict:

@@ -47,6 +47,7 @@ extern "C" {
 
 typedef struct fides_context fides_context;
 typedef struct fides_key fides_key;
+typedef struct fides_graph fides_graph;
 
 /* Parameters (replaces RawParams, RawCiphertext.cuh:45-83, built by
  * GetRawParams RawCiphertext.cu:213-643 and Parameters::adaptTo
@@ -84,6 +85,10 @@ int fides_copy_to_device(void* dst, const void* src, size_t bytes, void* stream)
 int fides_copy_to_host(void* dst, const void* src, size_t bytes, void* stream);
 int fides_copy_device(void* dst, const void* src, size_t bytes, void* stream); /* async, stream-ordered */
 int fides_stream_sync(void* stream);
+/* A non-blocking stream at the default priority (graph capture needs a
+ * stream other than the default one). */
+int fides_stream_create(void** stream);
+int fides_stream_destroy(void* stream);
 
 /* ---- raw limb operations (config: NTT/INTT + ModUp/ModDown) ---------- */
 
@@ -265,6 +270,26 @@ int fides_mod_raise_batch(fides_context* ctx, uint32_t n, const uint64_t* const*
  * reference automorph_ src/Rotation.cu:10-28).  d_out must not alias. */
 int fides_automorph(fides_context* ctx, const uint64_t* d_in, uint32_t nlimbs, uint32_t auto_index,
                     uint64_t* d_out, void* stream);
+
+/* ---- graph capture ----
+ * Every operation issued on `stream` between fides_graph_begin and
+ * fides_graph_end is recorded into one HIP graph instead of running
+ * (the reference caches CUDA graphs of its key-switch sequence,
+ * src/CKKS/LimbPartitionMGPU.cu:601-609, and of RNSPoly operations,
+ * src/CKKS/RNSPoly.cpp).  Task lists are copied to a device pool of
+ * task_bytes owned by the graph while capturing; the key-switch scratch of
+ * the captured operations is an arena of scratch_bytes owned by the graph
+ * (an operation that needs more fails with FIDES_ERR_NOMEM and the capture
+ * must be ended and discarded).  Issuing operations of the same context on
+ * another stream while capturing is an error (refused by every operation
+ * that builds task lists: the batched, key-switch and bootstrap calls).  A launch re-executes the recorded
+ * kernels on the buffers named at capture time (update inputs in place to
+ * run on new data); launches of one graph must not overlap each other. */
+int fides_graph_begin(fides_context* ctx, void* stream, size_t scratch_bytes, size_t task_bytes);
+int fides_graph_end(fides_context* ctx, fides_graph** graph);
+int fides_graph_launch(fides_graph* graph, void* stream);
+/* Waits for the device, then frees the graph and its pools. */
+int fides_graph_destroy(fides_graph* graph);
 
 #ifdef __cplusplus
 }
