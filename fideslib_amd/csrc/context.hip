@@ -824,6 +824,13 @@ struct KsLists {
   bool any_acc = false;
 };
 
+// Targets per conversion task: the sources of a task are read and pre-scaled
+// once for all its targets, so large batches take long chunks; a small
+// operation keeps short ones to give the GPU enough workgroups.
+static u32 conv_chunk(size_t total_targets) {
+  return total_targets >= 1024 ? 32 : total_targets >= 512 ? 16 : 8;
+}
+
 static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scratch, Blob& blob, KsLists& K_) {
   const u32 N = c->N, K = c->K, L = c->L;
   std::vector<NttTask> intt_in, ntt_ext, intt_sp, ntt_md, intt_top, ntt_mdrs;
@@ -832,6 +839,13 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   std::vector<AxpyTask> axpy1, axpy2;
   u64* w = scratch;
   std::vector<u64*> ext_of(jobs.size(), nullptr);
+  size_t up_targets = 0, down_targets = 0;
+  for (const KsJob& J : jobs) {
+    if (J.share < 0)
+      for (u32 j = 0; j < num_digits(c, J.l); ++j) up_targets += c->hmodup[(size_t)J.l * c->dnum + j].nt;
+    down_targets += 2 * J.l;
+  }
+  const u32 up_chunk = conv_chunk(up_targets), down_chunk = conv_chunk(down_targets);
   for (size_t jn = 0; jn < jobs.size(); ++jn) {
     const KsJob& J = jobs[jn];
     const u32 l = J.l, beta = num_digits(c, l), ext_stride = l + K;
@@ -855,9 +869,9 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
       const u32 ns = en - st;
       const ConvDesc& hd = c->hmodup[(size_t)l * c->dnum + j];
       u64* slab = EXT + (size_t)j * ext_stride * N;
-      for (u32 t0 = 0; t0 < hd.nt; t0 += CONV_TCHUNK)
+      for (u32 t0 = 0; t0 < hd.nt; t0 += up_chunk)
         conv_up[ns].push_back(ConvTask{c->limb(Dc, st), slab, c->modup + (size_t)l * c->dnum + j, t0,
-                                       std::min(hd.nt, t0 + CONV_TCHUNK)});
+                                       std::min(hd.nt, t0 + up_chunk)});
       for (u32 t = 0; t < hd.nt; ++t) ntt_ext.push_back(ntask(c->limb(slab, hd.dst_off[t]), hd.dst_pid[t]));
     }
     const u64* ka = J.key->d;
@@ -884,8 +898,8 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
     for (u32 p = 0; p < 2; ++p) {
       for (u32 k = 0; k < K; ++k) intt_sp.push_back(ntask(c->limb(Tp[p], l + k), L + k));
       u64* Yp = Y + (size_t)p * l * N;
-      for (u32 t0 = 0; t0 < md.nt; t0 += CONV_TCHUNK)
-        conv_down.push_back(ConvTask{c->limb(Tp[p], l), Yp, c->moddown + l, t0, std::min(md.nt, t0 + CONV_TCHUNK)});
+      for (u32 t0 = 0; t0 < md.nt; t0 += down_chunk)
+        conv_down.push_back(ConvTask{c->limb(Tp[p], l), Yp, c->moddown + l, t0, std::min(md.nt, t0 + down_chunk)});
       if (!J.rescale) {
         for (u32 i = 0; i < l; ++i) {
           NttTask t = ntask(c->limb(Yp, i), i);

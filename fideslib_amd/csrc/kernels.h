@@ -4,7 +4,7 @@
 
 constexpr int CONV_MAXS = 16;   // max source limbs of one conversion (digit size alpha, or K)
 constexpr int CONV_MAXT = 96;   // max target limbs of one conversion
-constexpr int CONV_TCHUNK = 8;  // targets per workgroup row
+constexpr int CONV_TCHUNK = 8;  // targets per workgroup row of a single conversion (key-switch batches take 8-32)
 
 // Tables of one approximate CRT basis switch (ModUp of one digit at one
 // level, or ModDown at one level): reference PartQlHatInvModq /
