@@ -13,7 +13,9 @@ struct NttCfg {
   static constexpr int TPL = 1 << (LS - A);         // threads per line
   static constexpr int M = 1 << LS;                 // line length
   static constexpr int NLINES = PASS == 0 ? (1 << L2) : (1 << L1);
-  static constexpr int LPW0 = PASS == 0 ? 16 : 256 / TPL;
+  // row pass: one wave per workgroup (the LDS footprint, 8.5 KiB per wave at
+  // N = 2^16, then allows 18 waves per CU instead of 16 in 4-wave groups)
+  static constexpr int LPW0 = PASS == 0 ? 16 : 64 / TPL;
   static constexpr int LPW = LPW0 < NLINES ? LPW0 : NLINES;  // lines per workgroup
   static constexpr int THREADS = LPW * TPL;
   static constexpr int S0 = PASS == 0 ? 0 : L1;     // first global stage
