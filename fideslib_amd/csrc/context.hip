@@ -1023,7 +1023,17 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
       fides_launch_dot_batch(at<DotTask>(dev, K_.dot) + o, std::min<u32>(65535, K_.n_dot - o), c->prm, c->logN,
                              c->L + c->K, st);
   }
-  RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_sp), K_.n_intt_sp, st));
+  if (K_.fused_dot) {
+    // the row stages of the special limbs' INTT ran inside the inner product
+    for (u32 o = 0; o < K_.n_intt_sp; o += 65535) {
+      LimbSpan sp = span0();
+      sp.list = at<NttTask>(dev, K_.intt_sp) + o;
+      sp.total = std::min<u32>(65535, K_.n_intt_sp - o);
+      if (fides_launch_intt_col(c->logN, sp, c->T, st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
+    }
+  } else {
+    RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_sp), K_.n_intt_sp, st));
+  }
   RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_down), K_.n_conv_down, c->K, st));
   if (K_.n_ntt_md)
     RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_md), K_.n_ntt_md, PRO_NONE,

@@ -130,6 +130,9 @@ void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st);
 int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);
+// Column pass of the inverse NTT alone (after k_ntt_row_dot ran the row
+// stages of the special limbs).
+int fides_launch_intt_col(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);
 // Column pass only (the row pass is fused into fides_launch_row_dot).
 int fides_launch_ntt_col(int logN, const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F,
                          hipStream_t st);

@@ -236,6 +236,47 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
   }
 }
 
+// Inverse row stages (global stages LOGN-1 .. L1) of one row held by its TPL
+// threads: in, thread r0 holds the 2^B consecutive elements (r1 << B) + k of
+// each of its G groups (r1 = r0 G + g); out, it holds elements r0 + TPL k.
+// One LDS exchange through the row's line L.  [0, q) -> [0, 2q).
+template <int LOGN>
+__device__ __forceinline__ void intt_row_stages(u32 row, u32 r0, u64* L, const u64* tw, const u64* tws, u64 q,
+                                                u64 q2, u64 (&x)[NttCfg<LOGN, 1>::E]) {
+  using C = NttCfg<LOGN, 1>;
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int g = 0; g < C::G; ++g) {
+    const u32 r1 = r0 * C::G + g;
+#pragma unroll
+    for (int v = C::B - 1; v >= 0; --v) {
+      const int u = C::A + v;
+      const int half = EB >> (v + 1);
+#pragma unroll
+      for (int k = 0; k < EB; ++k) {
+        if (k & half) continue;
+        const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
+        gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < C::E; ++k) x[k] = L[C::lx(r0 + C::TPL * k)];
+#pragma unroll
+  for (int u = C::A - 1; u >= 0; --u) {
+    const int half = C::E >> (u + 1);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
+      gs_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+    }
+  }
+}
+
 // Row pass fused with the key-switch inner product (replaces the row pass of
 // the ModUp outputs and k_dot_batch).  One 64-thread workgroup owns 64/TPL
 // rows of one output limb t: for every digit j < BETA except the one owning t
@@ -244,7 +285,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
 //   o0 = sum_j e_j b_j,  o1 = sum_j e_j a_j   (e_jown = the input limb itself)
 // with the key in Montgomery form, one column accumulation and one reduction
 // (the arithmetic of k_dot_batch, so results are identical).  The ModUp
-// outputs never go back to HBM in evaluation form.
+// outputs never go back to HBM in evaluation form.  For a special limb
+// (t >= l), whose two outputs ModDown transforms back first, the inverse row
+// stages follow in the same workgroup: the outputs are stored after the row
+// pass of their INTT.
 // One digit of k_ntt_row_dot: the row stages of digit j's rows (or the input
 // limb itself for the digit owning t), fully reduced, into `out`.
 template <int LOGN>
@@ -342,9 +386,39 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
     }
     u64 h, l;
     acc_value(a0, h, l);
-    tk.o0[i] = mont_redc(h, l, P);
+    e0[k] = mont_redc(h, l, P);
     acc_value(a1, h, l);
-    tk.o1[i] = mont_redc(h, l, P);
+    e1[k] = mont_redc(h, l, P);
+  }
+  if (tk.jown != 0xffffffffu) {
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) {
+      const u64 i = rbase + r0 + C::TPL * k;
+      tk.o0[i] = e0[k];
+      tk.o1[i] = e1[k];
+    }
+    return;
+  }
+  // Special limb (uniform per workgroup): ModDown starts with its INTT, so
+  // the inverse row stages run here and only the column pass is left.
+  const u64* itw = T.ipsi + ((u64)pid << LOGN);
+  const u64* itws = T.ipsi_sh + ((u64)pid << LOGN);
+  constexpr int EB = 1 << C::B;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    u64(&y)[C::E] = c == 0 ? e0 : e1;
+    u64* o = c == 0 ? tk.o0 : tk.o1;
+    __syncthreads();  // earlier readers of this workgroup's LDS are done
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = y[k];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) y[g * EB + k] = L[C::lx(((r0 * C::G + g) << C::B) + k)];
+    intt_row_stages<LOGN>(row, r0, L, itw, itws, q, q2, y);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) o[rbase + r0 + C::TPL * k] = y[k];
   }
 }
 
@@ -384,33 +458,8 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
     // measured faster than a coalesced load through LDS)
 #pragma unroll
     for (int k = 0; k < EB; ++k) x[g * EB + k] = srow[(r1 << C::B) + k];
-#pragma unroll
-    for (int v = C::B - 1; v >= 0; --v) {
-      const int u = C::A + v;
-      const int half = EB >> (v + 1);
-#pragma unroll
-      for (int k = 0; k < EB; ++k) {
-        if (k & half) continue;
-        const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
-        gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
   }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < C::E; ++k) x[k] = L[C::lx(r0 + C::TPL * k)];
-#pragma unroll
-  for (int u = C::A - 1; u >= 0; --u) {
-    const int half = C::E >> (u + 1);
-#pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-      if (k & half) continue;
-      const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
-      gs_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
-    }
-  }
+  intt_row_stages<LOGN>(row, r0, L, tw, tws, q, q2, x);
 #pragma unroll
   for (int k = 0; k < C::E; ++k) rowp[r0 + C::TPL * k] = x[k];
 }
@@ -551,6 +600,21 @@ int fides_launch_ntt_col(int logN, const LimbSpan& s, const NttTables& T, int pr
     case 15: launch_col<15>(s, T, pro, F, st); return 0;
     case 16: launch_col<16>(s, T, pro, F, st); return 0;
     case 17: launch_col<17>(s, T, pro, F, st); return 0;
+  }
+  return -1;
+}
+
+int fides_launch_intt_col(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st) {
+  if (s.total == 0) return 0;
+  switch (logN) {
+#define CASE(n)                                                                                         \
+  case n: {                                                                                             \
+    using C0 = NttCfg<n, 0>;                                                                            \
+    k_intt_col<n><<<dim3(C0::NLINES / C0::LPW, s.total), C0::THREADS, 0, st>>>(s, T);                  \
+    return 0;                                                                                           \
+  }
+    CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16) CASE(17)
+#undef CASE
   }
   return -1;
 }
