@@ -362,8 +362,9 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
     stream.close()
     return {"workload": "cfg4: %d full-slot bootstraps (2^15 slots) captured in one HIP graph, N=2^16, 30 limbs "
                         "(60+29x59 bit), dnum=3, level budget {3,3}, sparse-secret encapsulation (h'=%s, "
-                        "K'=%g), EvalMod Chebyshev degree %d + %d double angles, output 14 limbs"
-                        % (n_boot, boot.cfg.encaps_h, boot.cfg.K, boot.cfg.degree, boot.cfg.double_angle),
+                        "K'=%g), EvalMod Chebyshev degree %d (baby-step giant-step) + %d double angles, output %d limbs"
+                        % (n_boot, boot.cfg.encaps_h, boot.cfg.K, boot.cfg.degree, boot.cfg.double_angle,
+                           boot.output_level()),
             "bootstraps_per_s": round(n_boot * 1e3 / graph_ms, 3), "ms_per_bootstrap": round(graph_ms / n_boot, 3),
             "graph_ms": round(graph_ms, 3), "graph_launches": launches, "replays": reps,
             "request_loop_bootstraps_per_s": round(n_boot / wall, 3),

@@ -45,7 +45,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import codec
-from .serve import Plaintext, Program, Reg, chebyshev_coefficients, chebyshev_eval
+from .serve import Plaintext, Program, Reg, chebyshev_coefficients, chebyshev_eval, chebyshev_eval_depth
 
 
 # ----------------------------------------------------------------- linear algebra on diagonals
@@ -232,8 +232,8 @@ class BootstrapPlan:
             lambda y: math.cos(2 * math.pi * (cfg.K * y - 0.25) / 2 ** cfg.double_angle), cfg.degree, -1.0, 1.0)
         # levels
         self.l_raise = self.L
-        self.depth = (cfg.budget[0] + cfg.budget[1] + max(1, math.ceil(math.log2(cfg.degree))) + 1
-                      + cfg.double_angle)
+        self.evalmod_depth = chebyshev_eval_depth(self.cheb) + cfg.double_angle
+        self.depth = cfg.budget[0] + cfg.budget[1] + self.evalmod_depth
         if self.l_raise - self.depth < 1:
             raise ValueError("not enough levels for bootstrapping: need %d, have %d" % (self.depth + 1, self.L))
         self._pt_cache = {}
@@ -295,7 +295,7 @@ class BootstrapPlan:
             l -= 1
         l_split = l
         self._minus_i = self._monomial(3 * self.N // 2, l_split)
-        l -= max(1, math.ceil(math.log2(self.cfg.degree))) + 1 + self.cfg.double_angle
+        l -= self.evalmod_depth
         self._plus_i = self._monomial(self.N // 2, l)
         for P in self.stc_plans:
             pts["stc"].append(self._group_pts(P, l))

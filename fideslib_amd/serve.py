@@ -270,22 +270,23 @@ def chebyshev_coefficients(f, degree: int, lo: float, hi: float) -> np.ndarray:
     return c
 
 
+def chebyshev_eval_depth(coeffs) -> int:
+    """Levels consumed by `chebyshev_eval` for these coefficients (structural:
+    a dry run on a program with 64 levels)."""
+    P = Program([float(1 << 59)] * 64, 12, 59, "depth")
+    y = P.input(64)
+    return 64 - chebyshev_eval(P, y, coeffs).level
+
+
 def chebyshev_depth(degree: int) -> int:
-    """Levels consumed by `chebyshev_program` (input map, T_k tree, final combination)."""
-    return 1 + max(1, math.ceil(math.log2(degree))) + 1
+    """Levels consumed by `chebyshev_program` of this degree (input map and
+    series; coefficients all non-zero, the deepest case)."""
+    return 1 + chebyshev_eval_depth([1.0] * (degree + 1))
 
 
 def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
-    """Evaluate sum_k c_k T_k(y), y = (2x - lo - hi)/(hi - lo), on one ciphertext.
-
-    T_k by the product rule T_{m+n} = 2 T_m T_n - T_{|m-n|} with m the largest
-    power of two below m+n, so T_1..T_{2^j} are ready after j HMult levels;
-    each T costs one HMult+Rescale and one fused element-wise op
-    (2*prod - T_{|m-n|}, or 2*prod - 1).  The combination multiplies every
-    T_k by round(c_k * delta), adds them in a tree, adds c_0 and rescales once.
-    (The reference evaluates Chebyshev series with Paterson-Stockmeyer,
-    src/PolyApprox.cu; this is a shallower linear-depth-per-level variant.)
-    """
+    """Evaluate sum_k c_k T_k(y), y = (2x - lo - hi)/(hi - lo), on one ciphertext
+    (see chebyshev_eval)."""
     P = Program(prm.q, prm.logN, prm.scale_bits, "chebyshev%d" % (len(coeffs) - 1))
     x = P.input(level)
     # y = a*x + b with a = 2/(hi-lo), b = -(hi+lo)/(hi-lo)
@@ -298,14 +299,14 @@ def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
     return P
 
 
-def chebyshev_eval(P: Program, y: Reg, coeffs) -> Reg:
-    """sum_k c_k T_k(y) for a register whose values lie in [-1, 1]; uses
-    ceil(log2 deg) + 1 levels (see chebyshev_program)."""
-    deg = len(coeffs) - 1
+def _chebyshev_basis(P: Program, y: Reg, upto: int) -> dict:
+    """T_1..T_upto by the product rule T_{m+n} = 2 T_m T_n - T_{|m-n|} with m
+    the largest power of two below m+n (T_1..T_{2^j} ready after j HMult
+    levels); each T costs one HMult+Rescale and one fused element-wise op."""
     T = {1: y}
     m = 1
-    while m < deg:
-        for k in range(m + 1, min(2 * m, deg) + 1):
+    while m < upto:
+        for k in range(m + 1, min(2 * m, upto) + 1):
             n = k - m
             prod = P.hmul(T[m], T[n])
             if n == m:  # T_2m = 2 T_m^2 - 1
@@ -316,12 +317,81 @@ def chebyshev_eval(P: Program, y: Reg, coeffs) -> Reg:
                     b = P.match_scale(b, prod.scale, prod.level)
                 T[k] = P.lin(prod, mul=2, b=b, sub=True)
         m *= 2
-    # one fused combination; every term lands on one scale because each
-    # T_k's own scale is folded into its integer constant
-    target = P.delta * T[deg].scale
-    terms = [(T[k], int(round(coeffs[k] * target / T[k].scale))) for k in range(1, deg + 1) if coeffs[k] != 0.0]
-    s = P.lincomb(terms, add=int(round(coeffs[0] * target)), scale=target)
-    return P.rescale(s)
+    return T
+
+
+def chebyshev_eval(P: Program, y: Reg, coeffs) -> Reg:
+    """sum_k c_k T_k(y) for a register whose values lie in [-1, 1], baby-step /
+    giant-step (the reference's EvalChebyshevSeriesPS, src/PolyApprox.cu, and
+    OpenFHE's Paterson-Stockmeyer variant for Chebyshev series): baby steps
+    T_1..T_m (m the power of two near sqrt(deg)), giant steps T_m, T_2m, T_4m..
+    by doubling, and a recursive Chebyshev division
+        p = q T_M + r,  q = c_M + sum_{k>=1} 2 c_{M+k} T_k,  r_j = c_j - c_{2M-j}
+    (from T_{M+k} = 2 T_M T_k - T_{M-k}) down to pieces of degree < m, each
+    one fused integer linear combination of the baby steps.  deg = 55: 15
+    HMults instead of 54 for the full product-rule basis, same depth
+    (ceil(log2 deg) + 1 levels).
+
+    Scales: every node is built for a required output scale, so products and
+    remainders meet on exactly the same nominal scale; leaves fold each T_k's
+    own scale into its integer constant."""
+    c = [float(x) for x in coeffs]
+    deg = len(c) - 1
+    while deg > 0 and c[deg] == 0.0:
+        deg -= 1
+    c = c[:deg + 1]
+    m = 2
+    while m * m < deg + 1:
+        m *= 2
+    T = _chebyshev_basis(P, y, max(1, min(m, deg)))
+    M = m
+    while 2 * M <= deg:
+        sq = P.hmul(T[M], T[M])
+        T[2 * M] = P.lin(sq, mul=2, add=-int(round(sq.scale)))
+        M *= 2
+    q = P.q
+
+    def giant(n):  # largest giant step M (m 2^i) with n <= 2M, n > M
+        g = m
+        while 2 * g < n:
+            g *= 2
+        return g
+
+    def split(cs, Mg):
+        qc = [cs[Mg]] + [2.0 * cs[Mg + k] for k in range(1, len(cs) - Mg)]
+        rc = [cs[j] - (cs[2 * Mg - j] if 2 * Mg - j < len(cs) else 0.0) for j in range(Mg)]
+        return qc, rc
+
+    def leaf_terms(cs):
+        ks = [k for k in range(1, len(cs)) if cs[k] != 0.0]
+        return ks if ks else [1]
+
+    def level(cs):
+        """Level of the node's result (structure only)."""
+        if len(cs) <= m:
+            return min(T[k].level for k in leaf_terms(cs)) - 1
+        Mg = giant(len(cs))
+        qc, rc = split(cs, Mg)
+        return min(min(level(qc), T[Mg].level) - 1, level(rc))
+
+    def build(cs, S):
+        """Register holding sum_k cs_k T_k at nominal scale S."""
+        if len(cs) <= m:
+            ks = leaf_terms(cs)
+            l = min(T[k].level for k in ks)
+            target = S * q[l - 1]
+            terms = [(T[k], int(round(cs[k] * target / T[k].scale)) if k < len(cs) else 0) for k in ks]
+            return P.rescale(P.lincomb(terms, add=int(round(cs[0] * target)), scale=target))
+        Mg = giant(len(cs))
+        qc, rc = split(cs, Mg)
+        lp = min(level(qc), T[Mg].level)  # product level before its Rescale
+        qr = build(qc, S * q[lp - 1] / T[Mg].scale)
+        prod = P.hmul(qr, T[Mg])
+        return P.add(prod, build(rc, S))
+
+    if deg < 1:
+        raise ServeError("a Chebyshev series needs degree >= 1")
+    return build(c, P.delta)
 
 
 def rotsum_program(prm, level: int, k: int) -> Program:

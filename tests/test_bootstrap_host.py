@@ -71,7 +71,9 @@ def test_evalmod_polynomial_fp64():
 
 def test_plan_levels_and_rotations():
     bp = B.BootstrapPlan(params.config3(), None)
-    assert bp.depth == 3 + 3 + 6 + 1 + 3
+    # CoeffToSlot 3 + SlotToCoeff 3 + degree-55 series 6 (baby steps to T_8,
+    # giants T_16 and T_32, three division levels) + 3 double angles
+    assert bp.depth == 3 + 3 + 6 + 3
     assert bp.output_level() == 30 - bp.depth
     assert 0 < len(bp.rotations()) <= 48
     with pytest.raises(ValueError):

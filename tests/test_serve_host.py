@@ -173,7 +173,9 @@ def make_requests(rng, n, t_scale=0.0):
 def test_chebyshev_program_shape():
     P = S.chebyshev_program(PRM, 30, COEF, -8, 8)
     assert P.output.level == 30 - S.chebyshev_depth(15)
-    assert P.count([S.HMUL]) == 14          # T_2 .. T_15
+    # baby steps T_2..T_4, giant T_8, three Chebyshev divisions (p = q T_8 + r,
+    # q = q' T_4 + r', r = q'' T_4 + r'')
+    assert P.count([S.HMUL]) == 7
     assert all(o.level >= 1 for o in P.ops)
     with pytest.raises(S.ServeError):
         S.chebyshev_program(PRM, S.chebyshev_depth(15), COEF, -8, 8)   # runs out of levels
