@@ -21,7 +21,8 @@ rotate-and-sum requests through the continuous-batching loop
 "serve" object.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain|batched]
-                       [--no-serve] [--serve-requests R] [--serve-rate LAMBDA] [--no-cpu]
+                       [--no-serve] [--serve-requests R] [--serve-rate LAMBDA]
+                       [--serve-load-points L1,L2] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -374,7 +375,7 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
                               "(tests/test_bootstrap_gpu.py, real keys)"}
 
 
-def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed):
+def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=()):
     """Poisson(rate) arrivals of the serving mix through the continuous-
     batching loop; keys resident.  Mix: 60% degree-15 Chebyshev series of
     the sigmoid on slots in [-8, 8] (start limbs U{7..30}), 30% rotate-and-sum
@@ -444,6 +445,19 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed):
         "arena_peak_gib": round(srv.arena.peak * N * 8 / 2 ** 30, 2),
         "capacity_closed_batch": cap,
     }
+    # the same mix at higher arrival rates (fresh draws)
+    pts = []
+    for r2 in more_rates:
+        arr = np.cumsum(rng.exponential(1.0 / r2, n_req))
+        rq = make(n_req, arr)
+        s2 = S.Server(be, N)
+        s2.run(rq, realtime=True)
+        l2 = S.latency_summary(rq)
+        pts.append({"rate_req_per_s": r2, "p50_ms": round(l2["p50_ms"], 3), "p99_ms": round(l2["p99_ms"], 3),
+                    "achieved_req_per_s": round(n_req / s2.stats["wall_s"], 2)})
+        print("bench: serve at %g req/s %s" % (r2, pts[-1]), file=sys.stderr, flush=True)
+    if pts:
+        res["load_points"] = pts
     return res
 
 
@@ -466,6 +480,8 @@ def main():
     ap.add_argument("--boot-only", action="store_true", help="only the bootstrap leg (for profiling)")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=80.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-load-points", default="160,240",
+                    help="further arrival rates (comma-separated) reported as load_points; empty for none")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -547,7 +563,8 @@ def main():
         print("bench: bootstrap %s" % out["bootstrap"], file=sys.stderr, flush=True)
     if not args.no_serve:
         print("bench: serving leg", file=sys.stderr, flush=True)
-        out["serve"] = serve_leg(prm, ctx, w, boot, keys, args.serve_requests, args.serve_rate, SEED + 5)
+        more = [float(x) for x in args.serve_load_points.split(",") if x.strip()]
+        out["serve"] = serve_leg(prm, ctx, w, boot, keys, args.serve_requests, args.serve_rate, SEED + 5, more)
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(prm)
