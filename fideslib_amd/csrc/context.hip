@@ -1738,21 +1738,28 @@ extern "C" int fides_ptdot_batch(fides_context* c, uint32_t n, const uint32_t* c
     if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     for (u32 j = 0; j < cnt[i]; ++j)
       if (!ct[o + j] || !pt[o + j]) return fail(FIDES_ERR_ARG, "null term");
-    for (u32 p = 0; p < 2; ++p)
-      for (u32 t = 0; t < l[i]; ++t) {
-        PtDotTask tk;
-        tk.out = c->limb(out[i], (size_t)p * l[i] + t);
-        tk.pid = t;
-        tk.cnt = cnt[i];
-        tk.first = (u32)terms.size();
-        tk.pad = 0;
-        for (u32 j = 0; j < cnt[i]; ++j)
-          terms.push_back(PtTerm{c->limb(ct[o + j], (size_t)p * l[i] + t), c->limb(pt[o + j], t)});
-        tasks.push_back(tk);
-      }
+    for (u32 t = 0; t < l[i]; ++t) {
+      PtDotTask tk;
+      tk.out0 = c->limb(out[i], t);
+      tk.out1 = c->limb(out[i], (size_t)l[i] + t);
+      tk.pid = t;
+      tk.cnt = cnt[i];
+      tk.first = (u32)terms.size();
+      tk.pad = 0;
+      for (u32 j = 0; j < cnt[i]; ++j)
+        terms.push_back(
+            PtTerm{c->limb(ct[o + j], t), c->limb(ct[o + j], (size_t)l[i] + t), c->limb(pt[o + j], t)});
+      tasks.push_back(tk);
+    }
     o += cnt[i];
   }
   if (tasks.empty()) return FIDES_OK;
+  // Tasks reading the same plaintext limbs (the same linear transform applied
+  // to several ciphertexts) next to each other, so the plaintext stays in L2.
+  std::stable_sort(tasks.begin(), tasks.end(), [&](const PtDotTask& a, const PtDotTask& b) {
+    if (a.pid != b.pid) return a.pid < b.pid;
+    return terms[a.first].m < terms[b.first].m;
+  });
   hipStream_t st = (hipStream_t)stream;
   Blob blob;
   const size_t ot = blob.add(tasks);

@@ -65,12 +65,17 @@ struct AxpyTask {
 // One output limb of sum_j x_j * m_j (mod q_pid), m_j in Montgomery form
 // (plaintext limbs prepared by fides_plaintext_prepare); terms[first ..
 // first + cnt) hold the operand limbs, cnt <= 16.
+// One term of a plaintext dot product at one limb: both ciphertext
+// components' limbs and the plaintext limb they are multiplied by.
 struct PtTerm {
-  const u64* x;
+  const u64* x0;
+  const u64* x1;
   const u64* m;
 };
+// One limb of one output: out0/out1 = sum of the cnt terms from `first`.
 struct PtDotTask {
-  u64* out;
+  u64* out0;
+  u64* out1;
   u32 pid;
   u32 cnt;
   u32 first;

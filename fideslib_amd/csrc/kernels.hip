@@ -188,21 +188,26 @@ __global__ __launch_bounds__(256) void k_lin_batch(const LinTask* __restrict__ t
 // Sum of up to 16 element-wise ciphertext-limb x plaintext-limb products with
 // one Montgomery reduction (the reference's plaintext dot products of the
 // linear transforms, ElemenwiseBatchKernels.cu:1007-1107).  x < q and
-// m < q: the 128-bit sum stays below 16 q^2 < q 2^64.
+// m < q: the 128-bit sum stays below 16 q^2 < q 2^64.  Both ciphertext
+// components in one task: each plaintext limb is read once for the two.
 __global__ __launch_bounds__(256) void k_ptdot_batch(const PtDotTask* __restrict__ tasks,
                                                      const PtTerm* __restrict__ terms,
                                                      const PrimeC* __restrict__ prm, u32 logN) {
   const PtDotTask t = tasks[blockIdx.y];
   const u32 k = blockIdx.x * 256u + threadIdx.x;
   const PrimeC P = prm[t.pid];
-  Acc128 a;
+  Acc128 a0, a1;
   for (u32 j = 0; j < t.cnt; ++j) {
     const PtTerm tm = terms[t.first + j];
-    acc_mac_v(a, tm.x[k], tm.m[k]);
+    const u64 m = tm.m[k];
+    acc_mac_v(a0, tm.x0[k], m);
+    acc_mac_v(a1, tm.x1[k], m);
   }
   u64 h, l;
-  acc_value(a, h, l);
-  t.out[k] = mont_redc(h, l, P);
+  acc_value(a0, h, l);
+  t.out0[k] = mont_redc(h, l, P);
+  acc_value(a1, h, l);
+  t.out1[k] = mont_redc(h, l, P);
 }
 
 // ---------------------------------------------------------------- linear combination
