@@ -313,7 +313,11 @@ class BootstrapPlan:
         for k in sorted(pts):
             inner = Pg.ptdot([(babies[b], pt) for b, pt in pts[k]])
             parts.append(inner if k == 0 else Pg.rotate(inner, k * plan.g * plan.stride))
-        return Pg.rescale(Pg.sum_tree(parts))
+        # one fused sum (a linear combination with unit constants) instead of
+        # a tree of pairwise additions: one launch and one step
+        if len(parts) == 1:
+            return Pg.rescale(parts[0])
+        return Pg.rescale(Pg.lincomb([(p, 1) for p in parts], add=None, scale=parts[0].scale))
 
     def _evalmod(self, Pg: Program, y: Reg) -> Reg:
         c = chebyshev_eval(Pg, y, self.cheb)
