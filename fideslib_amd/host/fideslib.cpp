@@ -34,6 +34,16 @@ CiphertextImpl::~CiphertextImpl() {
   if (d) fides_device_free(d);
 }
 
+PlaintextImpl::PlaintextImpl(std::shared_ptr<CryptoContextImpl> c, uint32_t l) : limbs(l), cc(std::move(c)) {
+  void* p = nullptr;
+  chk(fides_device_alloc((size_t)l * cc->GetRingDimension() * 8, &p), "plaintext allocation");
+  d = (uint64_t*)p;
+}
+
+PlaintextImpl::~PlaintextImpl() {
+  if (d) fides_device_free(d);
+}
+
 CryptoContext CryptoContextImpl::Create(const RawParams& P) {
   if (P.moduli.empty() || P.SPECIALmoduli.empty()) throw std::runtime_error("RawParams: empty moduli");
   std::shared_ptr<CryptoContextImpl> c(new CryptoContextImpl());
@@ -200,6 +210,56 @@ Ciphertext CryptoContextImpl::LoadCiphertext(const RawCipherText& raw) {
   ct->slots = raw.slots;
   ct->keyid = raw.keyid;
   return ct;
+}
+
+Plaintext CryptoContextImpl::LoadPlaintext(const RawPlainText& raw) {
+  const uint32_t l = (uint32_t)raw.sub_0.size();
+  if (l == 0 || l > L) throw std::runtime_error("RawPlainText: bad limb count");
+  if (!raw.moduli.empty()) {
+    if (raw.moduli.size() != l) throw std::runtime_error("RawPlainText: moduli count");
+    for (uint32_t i = 0; i < l; ++i)
+      if (raw.moduli[i] != q[i]) throw std::runtime_error("RawPlainText: moduli differ from the context");
+  }
+  std::vector<uint64_t> flat((size_t)l * N);
+  for (uint32_t i = 0; i < l; ++i) {
+    if (raw.sub_0[i].size() != N) throw std::runtime_error("RawPlainText: limb size");
+    for (uint64_t v : raw.sub_0[i])
+      if (v >= q[i]) throw std::runtime_error("RawPlainText: residue not reduced");
+    std::copy(raw.sub_0[i].begin(), raw.sub_0[i].end(), flat.begin() + (size_t)i * N);
+  }
+  Plaintext pt(new PlaintextImpl(shared_from_this(), l));
+  chk(fides_copy_to_device(pt->d, flat.data(), flat.size() * 8, nullptr), "plaintext upload");
+  if (raw.format == EVALUATION) {  // back to coefficients: the preparation transforms forward
+    std::vector<uint32_t> pids(l);
+    for (uint32_t i = 0; i < l; ++i) pids[i] = i;
+    chk(fides_intt(ctx, pt->d, pids.data(), l, nullptr), "plaintext INTT");
+  }
+  chk(fides_plaintext_prepare(ctx, pt->d, l, nullptr), "plaintext preparation");
+  pt->NoiseLevel = raw.NoiseLevel;
+  pt->NoiseFactor = raw.Noise;
+  pt->slots = raw.slots;
+  return pt;
+}
+
+Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Plaintext& p) {
+  Ciphertext a = a0;
+  if (technique != FIXEDMANUAL && a->NoiseLevel == 2) a = Rescale(a);
+  if (p->limbs < a->limbs) {
+    if (technique == FIXEDMANUAL || Flexible())
+      throw std::runtime_error("EvalMult(plaintext): plaintext below the ciphertext level");
+    a = DropToLimbs(a, p->limbs);  // FIXEDAUTO: align down (Plaintext::adjustPlaintextToCiphertext)
+  }
+  const uint32_t l = a->limbs, one = 1;
+  Ciphertext o = NewCiphertext(l);
+  const uint64_t* ct = a->d;
+  const uint64_t* pp = p->d;
+  uint64_t* out = o->d;
+  chk(fides_ptdot_batch(ctx, 1, &one, &ct, &pp, &l, &out, nullptr), "EvalMult(plaintext)");
+  o->NoiseLevel = a->NoiseLevel + p->NoiseLevel;
+  o->NoiseFactor = a->NoiseFactor * p->NoiseFactor;
+  o->slots = a->slots;
+  o->keyid = a->keyid;
+  return o;
 }
 
 RawCipherText CryptoContextImpl::StoreCiphertext(const Ciphertext& ct) {

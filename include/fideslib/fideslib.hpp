@@ -62,6 +62,20 @@ struct RawCipherText {
   std::string keyid;
 };
 
+// Plaintext in RNS form (RawPlainText, RawCiphertext.cuh:37-48): evaluation
+// form as FIDESlib keeps it on the device, or coefficient form.
+enum Format { COEFFICIENT = 0, EVALUATION = 1 };
+struct RawPlainText {
+  std::vector<std::vector<uint64_t>> sub_0;  // [numRes][N]
+  std::vector<uint64_t> moduli;              // optional check: q_0..q_{numRes-1}
+  int numRes = 0;
+  int N = 0;
+  Format format = EVALUATION;
+  double Noise = 0.0;  // scaling factor
+  int slots = 0;
+  int NoiseLevel = 1;
+};
+
 // Key-switching key (RawKeySwitchKey, RawCiphertext.cuh:85-90):
 // r_key[0] = a, r_key[1] = b, each [dnum][L+K][N].
 struct RawKeySwitchKey {
@@ -101,6 +115,33 @@ class CiphertextImpl {
 };
 using Ciphertext = std::shared_ptr<CiphertextImpl>;
 
+// A device-resident plaintext: [l][N] evaluation form in the Montgomery
+// representation the plaintext products consume (reference Plaintext,
+// src/CKKS/Plaintext.cuh).
+class PlaintextImpl {
+ public:
+  ~PlaintextImpl();
+  PlaintextImpl(const PlaintextImpl&) = delete;
+  PlaintextImpl& operator=(const PlaintextImpl&) = delete;
+
+  uint32_t GetLevel() const { return limbs - 1; }
+  uint32_t GetLimbs() const { return limbs; }
+  int GetNoiseScaleDeg() const { return NoiseLevel; }
+  double GetScalingFactor() const { return NoiseFactor; }
+
+  uint32_t limbs = 0;
+  int NoiseLevel = 1;
+  double NoiseFactor = 0.0;
+  int slots = 0;
+
+ private:
+  friend class CryptoContextImpl;
+  explicit PlaintextImpl(std::shared_ptr<CryptoContextImpl> cc, uint32_t limbs);
+  std::shared_ptr<CryptoContextImpl> cc;
+  uint64_t* d = nullptr;
+};
+using Plaintext = std::shared_ptr<PlaintextImpl>;
+
 class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl> {
  public:
   // GenCryptoContext + LoadContext (api/CryptoContext.cpp:199-249).
@@ -124,6 +165,15 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   // ---- operations (api/CryptoContext.hpp:149-181)
   Ciphertext EvalMult(const Ciphertext& a, const Ciphertext& b);
   void EvalMultInPlace(Ciphertext& a, const Ciphertext& b);
+  // Plaintexts (api/CryptoContext.hpp:60-66 LoadPlaintext, :150-159
+  // EvalMult(ct, pt)): the plaintext needs at least as many limbs as the
+  // ciphertext (its first limbs are used); with automatic rescaling a
+  // degree-2 ciphertext is rescaled first (Ciphertext::multPt,
+  // src/CKKS/Ciphertext.cpp:358-423).
+  Plaintext LoadPlaintext(const RawPlainText& raw);
+  Ciphertext EvalMult(const Ciphertext& a, const Plaintext& p);
+  Ciphertext EvalMult(const Plaintext& p, const Ciphertext& a) { return EvalMult(a, p); }
+  void EvalMultInPlace(Ciphertext& a, const Plaintext& p) { a = EvalMult(a, p); }
   Ciphertext EvalSquare(const Ciphertext& a);
   void EvalSquareInPlace(Ciphertext& a);
   Ciphertext EvalRotate(const Ciphertext& a, int32_t index);

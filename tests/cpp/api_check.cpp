@@ -214,12 +214,76 @@ static int serial(const std::string& dir) {
   return ok ? 0 : 1;
 }
 
+// Plaintext products: pt_eval.bin ([l][N] evaluation form) and pt_coef.bin
+// (the same plaintext in coefficient form, one limb more) multiply c1.bin
+// ([2][l][N]); outputs ptm_eval.bin, ptm_coef.bin, ptm_auto.bin (FIXEDAUTO,
+// the product of a degree-2 ciphertext: rescaled first) and meta.txt.
+static int plain(const std::string& dir) {
+  const auto P = read_u64(dir + "/params.bin");
+  const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+  const uint32_t l = (uint32_t)P[4], N = 1u << logN;
+  std::ofstream meta(dir + "/meta.txt");
+  meta << std::setprecision(17);
+  for (int tech = 0; tech < 2; ++tech) {
+    fideslib::RawParams rp;
+    rp.logN = (int)logN;
+    rp.moduli.assign(P.begin() + 6, P.begin() + 6 + L);
+    rp.SPECIALmoduli.assign(P.begin() + 6 + L, P.begin() + 6 + L + K);
+    rp.dnum = (int)dnum;
+    rp.scalingTechnique = tech == 0 ? fideslib::FIXEDMANUAL : fideslib::FIXEDAUTO;
+    auto cc = fideslib::CryptoContextImpl::Create(rp);
+    cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
+    const double sc = (double)(1ull << 40);
+    auto c1 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), l, N, sc));
+    auto pe = read_u64(dir + "/pt_eval.bin"), pc = read_u64(dir + "/pt_coef.bin");
+    fideslib::RawPlainText re, rc;
+    for (uint32_t i = 0; i < l; ++i) re.sub_0.emplace_back(pe.begin() + (size_t)i * N, pe.begin() + (size_t)(i + 1) * N);
+    for (uint32_t i = 0; i < l + 1; ++i)
+      rc.sub_0.emplace_back(pc.begin() + (size_t)i * N, pc.begin() + (size_t)(i + 1) * N);
+    re.format = fideslib::EVALUATION;
+    rc.format = fideslib::COEFFICIENT;
+    re.Noise = rc.Noise = sc;
+    rc.moduli.assign(rp.moduli.begin(), rp.moduli.begin() + l + 1);
+    auto pte = cc->LoadPlaintext(re);
+    auto ptc = cc->LoadPlaintext(rc);
+    if (tech == 0) {
+      auto m1 = cc->EvalMult(c1, pte);
+      auto m2 = cc->EvalMult(ptc, c1);
+      store(dir + "/ptm_eval.bin", cc, m1, meta);
+      store(dir + "/ptm_coef.bin", cc, m2, meta);
+      bool threw = false;
+      try {
+        auto c2 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c2.bin"), l, N, sc));
+        fideslib::RawPlainText r1 = re;
+        r1.sub_0.pop_back();
+        cc->EvalMult(c2, cc->LoadPlaintext(r1));  // plaintext one limb short
+      } catch (const std::runtime_error&) {
+        threw = true;
+      }
+      meta << (threw ? "short-plaintext-throws" : "short-plaintext-accepted") << "\n";
+    } else {
+      auto sq = cc->EvalMult(c1, c1);            // degree 2, not rescaled yet (FIXEDAUTO)
+      auto m3 = cc->EvalMult(sq, ptc);           // rescales sq first
+      store(dir + "/ptm_auto.bin", cc, m3, meta);
+    }
+  }
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
-    std::fprintf(stderr, "usage: api_check DIR [flex|serial|serial-host]\n");
+    std::fprintf(stderr, "usage: api_check DIR [flex|plain|serial|serial-host]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  if (argc > 2 && std::string(argv[2]) == "plain") {
+    try {
+      return plain(dir);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "api_check: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc > 2 && (std::string(argv[2]) == "serial-host" || std::string(argv[2]) == "serial")) {
     try {
       return std::string(argv[2]) == "serial" ? serial(dir) : serial_host(dir);

@@ -106,3 +106,47 @@ def test_cpp_api_flexibleauto(tmp_path):
         bits = C.precision_bits(dec, want)
         print("flex %s: limbs %d, degree %s, %.1f bits" % (name, limbs, nl, bits), flush=True)
         assert bits > 14, (name, bits)
+
+
+def test_cpp_api_plaintext_products(tmp_path):
+    """LoadPlaintext (evaluation and coefficient form) and EvalMult(ct, pt)
+    through the C++ API, against the oracle's element-wise products."""
+    assert os.path.exists(BIN), "tests/cpp/api_check is not built (run __graft_entry__.build())"
+    prm = params.config1()
+    N, L = prm.N, prm.L
+    l = L - 1
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 19)
+    rng = np.random.default_rng(21)
+    cts = [keys.encrypt(C.encode(rng.uniform(-1, 1, N // 2), 2.0 ** 40, N), l) for _ in range(2)]
+    coef = C.encode(rng.uniform(-1, 1, N // 2), 2.0 ** 40, N)
+    pt_coef = C.to_rns(coef, prm.q[:l + 1])                       # l + 1 limbs
+    pt_eval = octx.ntt_limbs(np.ascontiguousarray(pt_coef[:l]), list(range(l)))
+    rlk = keys.relin_key()
+    d = str(tmp_path)
+    hdr = [prm.logN, prm.L, prm.K, prm.dnum, l, 0] + list(prm.q) + list(prm.p)
+    np.asarray(hdr, np.uint64).tofile(os.path.join(d, "params.bin"))
+    rlk.astype(np.uint64).tofile(os.path.join(d, "rlk.bin"))
+    cts[0].astype(np.uint64).tofile(os.path.join(d, "c1.bin"))
+    cts[1].astype(np.uint64).tofile(os.path.join(d, "c2.bin"))
+    np.ascontiguousarray(pt_eval, np.uint64).tofile(os.path.join(d, "pt_eval.bin"))
+    np.ascontiguousarray(pt_coef, np.uint64).tofile(os.path.join(d, "pt_coef.bin"))
+    r = subprocess.run([BIN, d, "plain"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+    def times_pt(ct, lv):
+        pid = list(range(lv)) * 2
+        return octx.mul_limbs(ct[:, :lv].reshape(2 * lv, N), np.concatenate([pt_eval[:lv]] * 2), pid).reshape(2, lv, N)
+
+    want = times_pt(cts[0], l)
+    for name in ("ptm_eval", "ptm_coef"):
+        assert np.array_equal(np.fromfile(os.path.join(d, name + ".bin"), np.uint64).reshape(2, l, N), want), name
+    sq = octx.rescale_ct(octx.hmult_relin(cts[0], cts[0], l, rlk), l)
+    got = np.fromfile(os.path.join(d, "ptm_auto.bin"), np.uint64).reshape(2, l - 1, N)
+    assert np.array_equal(got, times_pt(sq, l - 1))
+    meta = open(os.path.join(d, "meta.txt")).read().split("\n")
+    limbs, nl, nf = meta[0].split()
+    assert int(limbs) == l and int(nl) == 2 and abs(float(nf) / 2.0 ** 80 - 1) < 1e-12
+    assert meta[2] == "short-plaintext-throws"
+    limbs, nl, _ = meta[3].split()
+    assert int(limbs) == l - 1 and int(nl) == 2
