@@ -15,6 +15,7 @@
 #ifndef FIDESLIB_AMD_FIDESLIB_HPP
 #define FIDESLIB_AMD_FIDESLIB_HPP
 
+#include <cmath>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -85,6 +86,19 @@ struct RawKeySwitchKey {
 
 class CryptoContextImpl;
 using CryptoContext = std::shared_ptr<CryptoContextImpl>;
+
+// Bootstrapping parameters (reference EvalBootstrapSetup arguments,
+// api/CryptoContext.hpp:183-186): level budget of CoeffToSlot and
+// SlotToCoeff, the Chebyshev degree of the modular reduction and its double
+// angles, the bound K' of t / q_0 it covers, and the prescale 2^-correction.
+struct BootstrapConfig {
+  uint32_t budgetCtS = 3, budgetStC = 3;
+  uint32_t degree = 55;
+  uint32_t doubleAngle = 3;
+  double K = 32.0;
+  uint32_t correction = 10;
+};
+struct BootPlan;
 
 // A device-resident ciphertext: [2][l][N] evaluation form plus metadata
 // (reference Ciphertext, src/CKKS/Ciphertext.cuh:34-54).
@@ -213,6 +227,16 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
 
   void Synchronize() const;
 
+  // Bootstrapping of full-slot ciphertexts (api/CryptoContext.hpp:183-186;
+  // reference Bootstrap.cu:169-333): the setup computes the linear-transform
+  // plaintexts and the modular-reduction series and returns the rotation
+  // indices whose keys EvalBootstrap needs (plus the conjugation key,
+  // automorphism index 2N-1).  The result has limbs - depth limbs.
+  std::vector<int32_t> EvalBootstrapSetup(const BootstrapConfig& cfg = BootstrapConfig());
+  Ciphertext EvalBootstrap(const Ciphertext& ct);
+  // The key-switching key loaded for an automorphism index (throws if absent).
+  const fides_key* RotationKeyFor(uint32_t autoIndex) const { return RotationKey(autoIndex); }
+
   fides_context* Handle() const { return ctx; }
 
   // ---- serialization support (fideslib/serialize.hpp)
@@ -248,6 +272,9 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   std::vector<double> sfReal, sfBig;  // by level = limbs - 1
   fides_key* evalMultKey = nullptr;
   std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
+  std::shared_ptr<BootPlan> boot;                // EvalBootstrapSetup
+  // the nominal scale of a fresh ciphertext: 2^(bit length of q_1, rounded)
+  double boot_delta() const { return std::ldexp(1.0, (int)std::lround(std::log2((double)q[q.size() > 1 ? 1 : 0]))); }
 };
 
 }  // namespace fideslib

@@ -13,6 +13,7 @@
 //          fast.bin ([3][2][l-1][N], EvalFastRotation(mult, {rot, 0, rot})),
 //          many.bin ([2][2][l-1][N], EvalMultMany({c1, c2}, {c2, c1}, rescale)),
 //          meta.txt (NoiseLevel / NoiseFactor / limbs of the results).
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <fstream>
@@ -270,12 +271,55 @@ static int plain(const std::string& dir) {
   return 0;
 }
 
+// Bootstrapping: boot.bin = [scale bits, nkeys, auto indices...], rotkeys.bin
+// = nkeys keys [2][dnum][L+K][N], c1.bin = [2][l][N].  Outputs boot_rots.txt
+// (the rotation indices EvalBootstrapSetup asks for), boot_out.bin and
+// meta.txt (limbs, NoiseLevel, NoiseFactor).
+static int boot(const std::string& dir) {
+  const auto P = read_u64(dir + "/params.bin");
+  const uint32_t logN = (uint32_t)P[0], L = (uint32_t)P[1], K = (uint32_t)P[2], dnum = (uint32_t)P[3];
+  const uint32_t l = (uint32_t)P[4], N = 1u << logN;
+  fideslib::RawParams rp;
+  rp.logN = (int)logN;
+  rp.moduli.assign(P.begin() + 6, P.begin() + 6 + L);
+  rp.SPECIALmoduli.assign(P.begin() + 6 + L, P.begin() + 6 + L + K);
+  rp.dnum = (int)dnum;
+  auto cc = fideslib::CryptoContextImpl::Create(rp);
+  cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
+  const auto B = read_u64(dir + "/boot.bin");
+  const auto keys = read_u64(dir + "/rotkeys.bin");
+  const size_t ksz = (size_t)2 * dnum * (L + K) * N;
+  for (uint64_t i = 0; i < B[1]; ++i) {
+    std::vector<uint64_t> one(keys.begin() + i * ksz, keys.begin() + (i + 1) * ksz);
+    cc->AddAutomorphismKey((uint32_t)B[2 + i], key_of(one, dnum, L + K, N));
+  }
+  const auto rots = cc->EvalBootstrapSetup();
+  {
+    std::ofstream f(dir + "/boot_rots.txt");
+    for (int32_t r : rots) f << r << "\n";
+  }
+  auto c1 = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), l, N, std::ldexp(1.0, (int)B[0])));
+  auto out = cc->EvalBootstrap(c1);
+  std::ofstream meta(dir + "/meta.txt");
+  meta << std::setprecision(17);
+  store(dir + "/boot_out.bin", cc, out, meta);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: api_check DIR [flex|plain|serial|serial-host]\n");
     return 2;
   }
   const std::string dir = argv[1];
+  if (argc > 2 && std::string(argv[2]) == "boot") {
+    try {
+      return boot(dir);
+    } catch (const std::exception& e) {
+      std::fprintf(stderr, "api_check: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc > 2 && std::string(argv[2]) == "plain") {
     try {
       return plain(dir);

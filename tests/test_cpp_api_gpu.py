@@ -150,3 +150,43 @@ def test_cpp_api_plaintext_products(tmp_path):
     assert meta[2] == "short-plaintext-throws"
     limbs, nl, _ = meta[3].split()
     assert int(limbs) == l - 1 and int(nl) == 2
+
+
+def test_cpp_api_bootstrap(tmp_path):
+    """EvalBootstrapSetup + EvalBootstrap through the C++ API (N = 2^12, 22
+    limbs, h = 64): the rotation indices it asks for are the Python plan's,
+    the result has limbs - depth limbs and decrypts to the message."""
+    from fideslib_amd import bootstrap as B
+    assert os.path.exists(BIN), "tests/cpp/api_check is not built (run __graft_entry__.build())"
+    prm = params.boot_small()
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 5, h=64)
+    bp = B.BootstrapPlan(prm, None)
+    idx = [pow(5, r, 2 * N) for r in bp.rotations()] + [2 * N - 1]
+    rng = np.random.default_rng(4)
+    z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    lvl = 3
+    ct = keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), lvl)
+    d = str(tmp_path)
+    hdr = [prm.logN, prm.L, prm.K, prm.dnum, lvl, 0] + list(prm.q) + list(prm.p)
+    np.asarray(hdr, np.uint64).tofile(os.path.join(d, "params.bin"))
+    keys.relin_key().astype(np.uint64).tofile(os.path.join(d, "rlk.bin"))
+    np.asarray([prm.scale_bits, len(idx)] + idx, np.uint64).tofile(os.path.join(d, "boot.bin"))
+    with open(os.path.join(d, "rotkeys.bin"), "wb") as f:
+        for k in idx:
+            f.write(keys.rotation_key(k).astype(np.uint64).tobytes())
+    ct.astype(np.uint64).tofile(os.path.join(d, "c1.bin"))
+    print("cpp api: bootstrap", flush=True)
+    r = subprocess.run([BIN, d, "boot"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rots = [int(x) for x in open(os.path.join(d, "boot_rots.txt")).read().split()]
+    assert rots == bp.rotations()
+    limbs, nl, nf = open(os.path.join(d, "meta.txt")).read().split("\n")[0].split()
+    lo = int(limbs)
+    assert lo == L - bp.depth and int(nl) == 1
+    out = np.fromfile(os.path.join(d, "boot_out.bin"), np.uint64).reshape(2, lo, N)
+    dec = C.decode(keys.decrypt_coef(out, lo), float(nf), N)
+    bits = C.precision_bits(dec, z)
+    print("C++ EvalBootstrap: %.1f bits" % bits, flush=True)
+    assert bits > 20, bits
