@@ -193,7 +193,20 @@ def cpu_baseline(prm, budget_s=12.0):
         if time.time() - t0 > budget_s:
             break
     el = sum(ops)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = O.lib().or_get_threads()
+    # the same operation on one thread, the first levels of the same draw
+    prev = O.set_threads(1)
+    st, t1 = [], time.time()
+    for i in range(len(levels)):
+        l = levels[i]
+        a = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        b = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        t = time.time()
+        octx.rescale_ct(octx.hmult_relin(a, b, l, key), l)
+        st.append(time.time() - t)
+        if time.time() - t1 > budget_s / 2:
+            break
+    O.set_threads(prev)
     return {
         "value": round(done / el, 3),
         "unit": "key-switch ops/s (HMult+relin+Rescale)",
@@ -201,6 +214,9 @@ def cpu_baseline(prm, budget_s=12.0):
         "kind": "port",
         "sample": "%d HMult+relin+Rescale at N=2^16, limbs drawn U{6..30} (first %s), %.1f s of oracle time; "
                   "own CPU oracle (OpenMP), not OpenFHE" % (done, levels[:4], el),
+        "single_thread": {"value": round(len(st) / sum(st), 3), "cores": 1,
+                          "sample": "%d of the same operations (the first levels of the draw), %.1f s"
+                                    % (len(st), sum(st))},
     }
 
 

@@ -36,6 +36,8 @@
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
 
+#include <omp.h>
+
 #define OR_API __attribute__((visibility("default")))
 
 /* ------------------------------------------------------------------ */
@@ -639,3 +641,7 @@ OR_API void or_sub_limbs(const or_ctx* c, const u64* x, const u64* y, u64* out, 
     for (uint32_t k = 0; k < c->N; ++k) out[(size_t)t * c->N + k] = submod(x[(size_t)t * c->N + k], y[(size_t)t * c->N + k], q);
   }
 }
+
+/* Worker threads of the parallel loops (timing of the CPU baseline). */
+OR_API void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+OR_API int or_get_threads(void) { return omp_get_max_threads(); }

@@ -66,6 +66,8 @@ def lib() -> ctypes.CDLL:
             "or_rescale_poly": (None, [vp, u64p, i]),
             "or_rescale_ct": (None, [vp, u64p, i, u64p]),
             "or_mod_raise": (None, [vp, u64p, i, u64p]),
+            "or_set_threads": (None, [ctypes.c_int]),
+            "or_get_threads": (ctypes.c_int, []),
             "or_automorph": (None, [vp, u64p, u64p, i, i]),
             "or_hmult_relin": (None, [vp, u64p, u64p, i, u64p, u64p]),
             "or_rotate": (None, [vp, u64p, i, u64p, i, u64p]),
@@ -282,3 +284,10 @@ class Context:
         out = np.zeros((len(pid), self.N), np.uint64)
         lib().or_sub_limbs(self._h, _p64(_u64(x)), _p64(_u64(y)), _p64(out), _p32(pid), len(pid))
         return out
+
+
+def set_threads(n: int) -> int:
+    """Set the oracle's OpenMP worker count; returns the previous one."""
+    prev = lib().or_get_threads()
+    lib().or_set_threads(int(n))
+    return prev
