@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
 template <int LOGN, int EPI>
 __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
   using C = NttCfg<LOGN, 1>;
-  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  __shared__ u32 lds32[C::LPW * C::LDS_LINE];
   const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
   u64* data = tk.data;
   const u32 pid = tk.pid;
@@ -183,16 +183,30 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
       ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
     }
   }
-  u64* L = lds + ll * C::LDS_LINE;
-#pragma unroll
-  for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = x[k];
-  __syncthreads();
+  u32* L = lds32 + ll * C::LDS_LINE;
   constexpr int EB = 1 << C::B;
+  {  // strided -> groups of 2^B consecutive, low words then high words
+    u32 lo[C::E];
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = (u32)x[k];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) lo[g * EB + k] = L[C::lx(((r0 * C::G + g) << C::B) + k)];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) L[C::lx(r0 + C::TPL * k)] = (u32)(x[k] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k)
+        x[g * EB + k] = ((u64)L[C::lx(((r0 * C::G + g) << C::B) + k)] << 32) | lo[g * EB + k];
+  }
 #pragma unroll
   for (int g = 0; g < C::G; ++g) {
     const u32 r1 = r0 * C::G + g;
-#pragma unroll
-    for (int k = 0; k < EB; ++k) x[g * EB + k] = L[C::lx((r1 << C::B) + k)];
 #pragma unroll
     for (int v = 0; v < C::B; ++v) {
       const int u = C::A + v;
@@ -204,10 +218,26 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
         ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
       }
     }
-#pragma unroll
-    for (int k = 0; k < EB; ++k) L[C::lx((r1 << C::B) + k)] = x[g * EB + k];
   }
-  __syncthreads();
+  {  // groups -> strided
+    u32 lo[C::E];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) L[C::lx(((r0 * C::G + g) << C::B) + k)] = (u32)x[g * EB + k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) lo[k] = L[C::lx(r0 + C::TPL * k)];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) L[C::lx(((r0 * C::G + g) << C::B) + k)] = (u32)(x[g * EB + k] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x[k] = ((u64)L[C::lx(r0 + C::TPL * k)] << 32) | lo[k];
+  }
   u64* base = (EPI != EPI_NONE && tk.out) ? tk.out : data;
   if (EPI != EPI_NONE && tk.perm) {
     // store through the automorphism X -> X^k as a scatter with k^-1:
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) {
       const u32 c = r0 + C::TPL * k;
       const u32 j = (row << C::L2) | c;
-      u64 v = csub(csub(L[C::lx(c)], q2), q);
+      u64 v = csub(csub(x[k], q2), q);
       v = epilogue<EPI>(v, j, tk, q);
       const u32 jr = __brev(j) >> (32 - LOGN);
       const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
@@ -230,7 +260,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
 #pragma unroll
   for (int k = 0; k < C::E; ++k) {
     const u32 c = r0 + C::TPL * k;
-    u64 v = csub(csub(L[C::lx(c)], q2), q);
+    u64 v = csub(csub(x[k], q2), q);
     v = epilogue<EPI>(v, (row << C::L2) | c, tk, q);
     dst[c] = v;
   }

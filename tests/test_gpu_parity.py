@@ -31,7 +31,7 @@ def say(msg):
 
 
 # ------------------------------------------------------------------ NTT
-@pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 16, 17])
+@pytest.mark.parametrize("logN", [10, 11, 12, 13, 14, 15, 16, 17])
 def test_ntt_intt_bit_exact(logN):
     say("ntt logN=%d" % logN)
     rng = np.random.default_rng(logN)
