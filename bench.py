@@ -494,6 +494,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-boot", action="store_true")
     ap.add_argument("--boot-only", action="store_true", help="only the bootstrap leg (for profiling)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=80.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-load-points", default="160,240",
@@ -509,6 +510,14 @@ def main():
     print("bench: building workload (64 ciphertexts, 65 keys)", file=sys.stderr, flush=True)
     w = Workload(prm, ctx, rng)
     ev = HipEvents()
+    tuning = None
+    if not args.no_autotune:
+        # launch shapes for the batched step's size classes, then for a single
+        # ciphertext (the plain path); results are the same under every setting
+        t0 = time.time()
+        ctx.autotune(w.levels, reps=2)
+        tuning = ctx.autotune([prm.L], reps=3)
+        print("bench: autotune %s (%.1f s)" % (tuning, time.time() - t0), file=sys.stderr, flush=True)
     if args.boot_only:
         from fideslib_amd import bootstrap as BT
         boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
@@ -563,6 +572,8 @@ def main():
             "batched_over_plain": round(ms_plain / ms_batch, 3),
             "K": prm.K,
             "seed": SEED,
+            "tuning": tuning if tuning is not None else ctx.tuning,
+            "autotuned": tuning is not None,
         },
         "roofline": roof,
     }

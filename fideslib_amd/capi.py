@@ -34,6 +34,25 @@ class Params(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """Launch-shape settings of a context (fides_tuning); none changes a result."""
+    FIELDS = ("conv_chunk", "group_by_prime", "fuse_row_dot")  # each per size class 0..2
+    _fields_ = [(f, ctypes.c_uint32 * 3) for f in FIELDS]
+
+    def as_dict(self) -> dict:
+        return {f: [int(v) for v in getattr(self, f)] for f in self.FIELDS}
+
+    @classmethod
+    def from_dict(cls, d: dict) -> "Tuning":
+        t = cls()
+        for f in cls.FIELDS:
+            if len(d[f]) != 3:
+                raise FidesError("tuning field %s needs 3 entries (one per size class)" % f)
+            for i, v in enumerate(d[f]):
+                getattr(t, f)[i] = int(v)
+        return t
+
+
 _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
@@ -60,6 +79,9 @@ SIGNATURES = {
     "fides_graph_end": (_int, [_vp, ctypes.POINTER(_vp)]),
     "fides_graph_launch": (_int, [_vp, _vp]),
     "fides_graph_destroy": (_int, [_vp]),
+    "fides_tuning_get": (_int, [_vp, ctypes.POINTER(Tuning)]),
+    "fides_tuning_set": (_int, [_vp, ctypes.POINTER(Tuning)]),
+    "fides_autotune": (_int, [_vp, _u32, _u32p, _u32, ctypes.POINTER(Tuning)]),
     "fides_ntt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
     "fides_intt": (_int, [_vp, _vp, _u32p, _u32, _vp]),
     "fides_modup": (_int, [_vp, _vp, _u32, _u32, _vp, _vp]),
@@ -399,6 +421,26 @@ class Context:
         h = _vp()
         check(load().fides_graph_end(self.h, ctypes.byref(h)), "fides_graph_end")
         return Graph(h.value)
+
+    # -- tuning (launch shapes only; results are the same under every setting)
+    @property
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(load().fides_tuning_get(self.h, ctypes.byref(t)), "fides_tuning_get")
+        return t.as_dict()
+
+    @tuning.setter
+    def tuning(self, d: dict) -> None:
+        t = Tuning.from_dict(d)
+        check(load().fides_tuning_set(self.h, ctypes.byref(t)), "fides_tuning_set")
+
+    def autotune(self, levels, reps: int = 3) -> dict:
+        """Time candidate settings on a synthetic batch of HMult+Rescale and
+        Rotate at these limb counts; keep and return the fastest."""
+        lv = (ctypes.c_uint32 * len(levels))(*[int(x) for x in levels])
+        t = Tuning()
+        check(load().fides_autotune(self.h, len(levels), lv, int(reps), ctypes.byref(t)), "fides_autotune")
+        return t.as_dict()
 
 
 class Stream:

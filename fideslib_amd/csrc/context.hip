@@ -216,6 +216,7 @@ struct fides_context {
   hipStream_t copy_stream = nullptr;  // task-list uploads, overlapped with the previous operation's kernels
   bool conv_split = false;            // every prime < 2^60: carry-free basis conversion (k_conv_batch)
   Capture* cap = nullptr;             // graph capture in progress
+  fides_tuning tune{{8, 16, 32}, {1, 1, 1}, {1, 1, 1}};  // launch shapes per size class (fides_tuning_set)
   std::vector<void*> allocs;
 
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
@@ -827,8 +828,9 @@ struct KsLists {
 // Targets per conversion task: the sources of a task are read and pre-scaled
 // once for all its targets, so large batches take long chunks; a small
 // operation keeps short ones to give the GPU enough workgroups.
-static u32 conv_chunk(size_t total_targets) {
-  return total_targets >= 1024 ? 32 : total_targets >= 512 ? 16 : 8;
+static u32 conv_class(size_t total_targets) { return total_targets >= 1024 ? 2 : total_targets >= 512 ? 1 : 0; }
+static u32 conv_chunk(const fides_context* c, size_t total_targets) {
+  return c->tune.conv_chunk[conv_class(total_targets)];
 }
 
 static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scratch, Blob& blob, KsLists& K_) {
@@ -845,7 +847,8 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
       for (u32 j = 0; j < num_digits(c, J.l); ++j) up_targets += c->hmodup[(size_t)J.l * c->dnum + j].nt;
     down_targets += 2 * J.l;
   }
-  const u32 up_chunk = conv_chunk(up_targets), down_chunk = conv_chunk(down_targets);
+  const u32 up_chunk = conv_chunk(c, up_targets), down_chunk = conv_chunk(c, down_targets);
+  const u32 cls = conv_class(up_targets);
   for (size_t jn = 0; jn < jobs.size(); ++jn) {
     const KsJob& J = jobs[jn];
     const u32 l = J.l, beta = num_digits(c, l), ext_stride = l + K;
@@ -950,8 +953,9 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   }
   // Group limbs of the same prime so a prime's twiddle tables stay in L2
   // while its limbs are transformed (order does not affect results).
-  auto by_prime = [](std::vector<NttTask>& v) {
-    std::stable_sort(v.begin(), v.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
+  auto by_prime = [c, cls](std::vector<NttTask>& v) {
+    if (c->tune.group_by_prime[cls])
+      std::stable_sort(v.begin(), v.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
   };
   by_prime(intt_in);
   by_prime(ntt_ext);
@@ -963,7 +967,7 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   // count (one launch per count).
   u32 maxb = 0;
   for (const DotTask& d : dot) maxb = std::max(maxb, d.beta);
-  K_.fused_dot = maxb <= (u32)ROWDOT_MAXB;
+  K_.fused_dot = c->tune.fuse_row_dot[cls] && maxb <= (u32)ROWDOT_MAXB;
   std::stable_sort(dot.begin(), dot.end(), [&](const DotTask& a, const DotTask& b) {
     if (K_.fused_dot && a.beta != b.beta) return a.beta < b.beta;
     return a.pid < b.pid;
@@ -1545,6 +1549,158 @@ extern "C" int fides_rotate_batch(fides_context* c, uint32_t n, const uint64_t* 
   if (!c || (n && (!in || !l || !keys || !k || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return rotate_batch(c, n, in, l, keys, k, out, (hipStream_t)stream);
+}
+
+// ======================================================================
+// tuning
+// ======================================================================
+extern "C" int fides_tuning_get(const fides_context* c, fides_tuning* t) {
+  if (!c || !t) return fail(FIDES_ERR_ARG, "null argument");
+  *t = c->tune;
+  return FIDES_OK;
+}
+
+extern "C" int fides_tuning_set(fides_context* c, const fides_tuning* t) {
+  if (!c || !t) return fail(FIDES_ERR_ARG, "null argument");
+  for (u32 b = 0; b < 3; ++b) {
+    if (t->conv_chunk[b] < 1 || t->conv_chunk[b] > 32) return fail(FIDES_ERR_ARG, "conv_chunk must be in 1..32");
+    if (t->group_by_prime[b] > 1 || t->fuse_row_dot[b] > 1) return fail(FIDES_ERR_ARG, "flags must be 0 or 1");
+  }
+  c->tune = *t;
+  return FIDES_OK;
+}
+
+// Synthetic batch of the autotuner: random residues (xorshift, reduced per
+// prime), one random key, n HMult+relin+Rescale and n Rotate (index 5).
+namespace {
+struct TuneBatch {
+  fides_context* c;
+  fides_key* key = nullptr;
+  std::vector<u64*> in, out;
+  std::vector<u32> lv, kidx;
+  std::vector<const fides_key*> keys;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~TuneBatch() {
+    for (u64* p : in) (void)hipFree(p);
+    for (u64* p : out) (void)hipFree(p);
+    if (key) fides_key_destroy(key);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+  int step() {
+    const u32 n = (u32)lv.size();
+    std::vector<const u64*> a(n), b(n);
+    for (u32 i = 0; i < n; ++i) a[i] = in[i], b[i] = in[(i + 1) % n];
+    // inputs of a pair must have the same limb count: pair i with itself when they differ
+    for (u32 i = 0; i < n; ++i)
+      if (lv[(i + 1) % n] != lv[i]) b[i] = in[i];
+    RCHK(hmult_batch(c, n, a.data(), b.data(), lv.data(), key, false, true, out.data(), nullptr));
+    RCHK(rotate_batch(c, n, a.data(), lv.data(), keys.data(), kidx.data(), out.data(), nullptr));
+    return FIDES_OK;
+  }
+  // best of 3 runs of `reps` steps, after one warm-up step
+  int time(u32 reps, float* ms) {
+    RCHK(step());
+    HIPCHK(hipDeviceSynchronize());
+    *ms = 0;
+    for (int run = 0; run < 3; ++run) {
+      HIPCHK(hipEventRecord(e0, nullptr));
+      for (u32 r = 0; r < reps; ++r) RCHK(step());
+      HIPCHK(hipEventRecord(e1, nullptr));
+      HIPCHK(hipEventSynchronize(e1));
+      float t;
+      HIPCHK(hipEventElapsedTime(&t, e0, e1));
+      t /= reps;
+      if (run == 0 || t < *ms) *ms = t;
+    }
+    return FIDES_OK;
+  }
+};
+}  // namespace
+
+static void fill_residues(std::vector<u64>& h, const fides_context* c, u32 nlimbs, u32 LK, u64& st) {
+  const size_t N = c->N;
+  h.resize((size_t)nlimbs * N);
+  for (u32 m = 0; m < nlimbs; ++m) {
+    const u64 q = c->primes[m % LK];
+    for (size_t i = 0; i < N; ++i) {
+      st ^= st << 13, st ^= st >> 7, st ^= st << 17;
+      h[m * N + i] = st % q;
+    }
+  }
+}
+
+extern "C" int fides_autotune(fides_context* c, uint32_t n, const uint32_t* l, uint32_t reps, fides_tuning* out) {
+  if (!c || !l || n == 0) return fail(FIDES_ERR_ARG, "null argument or empty batch");
+  if (c->cap) return fail(FIDES_ERR_ARG, "autotune during graph capture");
+  for (u32 i = 0; i < n; ++i)
+    if (l[i] < 2 || l[i] > c->L) return fail(FIDES_ERR_ARG, "autotune needs 2 <= l[i] <= L");
+  if (reps == 0) reps = 1;
+  const u32 N = c->N, LK = c->L + c->K;
+  TuneBatch B{c};
+  u64 seed = 0x9e3779b97f4a7c15ull;
+  std::vector<u64> h;
+  fill_residues(h, c, 2 * c->dnum * LK, LK, seed);
+  RCHK(fides_key_upload(c, h.data(), &B.key));
+  // one random ciphertext per distinct limb count, copied to every input of that count
+  fill_residues(h, c, 2 * c->L, c->L, seed);  // component c, limb m at (c * L + m) -> prime m
+  for (u32 i = 0; i < n; ++i) {
+    u64 *pi = nullptr, *po = nullptr;
+    const size_t S = (size_t)l[i] * N;
+    HIPCHK(hipMalloc((void**)&pi, 2 * S * 8));
+    B.in.push_back(pi);
+    HIPCHK(hipMalloc((void**)&po, 2 * S * 8));
+    B.out.push_back(po);
+    HIPCHK(hipMemcpy(pi, h.data(), S * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(pi + S, h.data() + (size_t)c->L * N, S * 8, hipMemcpyHostToDevice));
+    B.lv.push_back(l[i]);
+    B.kidx.push_back(5);
+    B.keys.push_back(B.key);
+  }
+  HIPCHK(hipEventCreate(&B.e0));
+  HIPCHK(hipEventCreate(&B.e1));
+  // size classes of the batch's conversions (ModUp and ModDown of one call)
+  size_t up = 0, down = 0;
+  for (u32 i = 0; i < n; ++i) {
+    for (u32 j = 0; j < num_digits(c, l[i]); ++j) up += c->hmodup[(size_t)l[i] * c->dnum + j].nt;
+    down += 2 * l[i];
+  }
+  const u32 cu = conv_class(up);  // class of the flags
+  std::vector<u32> classes{cu};    // classes of the conversion chunks
+  if (conv_class(down) != cu) classes.push_back(conv_class(down));
+  const fides_tuning start = c->tune;
+  fides_tuning best = start;
+  float tb;
+  int rc = B.time(reps, &tb);
+  auto trial = [&](const fides_tuning& t) -> int {
+    c->tune = t;
+    float ms;
+    const int r = B.time(reps, &ms);
+    if (r != FIDES_OK) return r;
+    if (ms < 0.98f * tb) best = t, tb = ms;
+    return FIDES_OK;
+  };
+  if (rc == FIDES_OK) {
+    fides_tuning t = best;
+    t.fuse_row_dot[cu] ^= 1;
+    rc = trial(t);
+  }
+  if (rc == FIDES_OK) {
+    fides_tuning t = best;
+    t.group_by_prime[cu] ^= 1;
+    rc = trial(t);
+  }
+  for (u32 cl : classes)
+    for (u32 v : {4u, 8u, 16u, 24u, 32u}) {
+      if (rc != FIDES_OK) break;
+      fides_tuning t = best;
+      t.conv_chunk[cl] = v;
+      if (v != best.conv_chunk[cl]) rc = trial(t);
+    }
+  c->tune = rc == FIDES_OK ? best : start;
+  if (rc != FIDES_OK) return rc;
+  if (out) *out = best;
+  return FIDES_OK;
 }
 
 // ======================================================================

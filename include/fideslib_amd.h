@@ -291,6 +291,46 @@ int fides_graph_launch(fides_graph* graph, void* stream);
 /* Waits for the device, then frees the graph and its pools. */
 int fides_graph_destroy(fides_graph* graph);
 
+/* ---- tuning ----
+ * Launch-shape settings of one context.  None of them changes a result bit:
+ * they choose how the same arithmetic is split into tasks and launches.
+ * The reference fixes these at compile time (block sizes and limb batches in
+ * src/CKKS/LimbPartition*.cu); this interface has no reference counterpart.
+ * Each setting is kept per size class c of a key-switch batch: a conversion
+ * (ModUp of all digits, or ModDown) with < 512 (c = 0), < 1024 (1) or more
+ * (2) target limbs in total; the flags follow the class of the ModUp.
+ *   conv_chunk[c]      targets per basis-conversion task, 1..32
+ *                      (defaults 8, 16, 32)
+ *   group_by_prime[c]  order each NTT task list by prime so one prime's
+ *                      twiddle tables stay in L2 (1, default) or keep
+ *                      ciphertext order (0)
+ *   fuse_row_dot[c]    run the row pass of the ModUp outputs inside the
+ *                      key-switch inner product, and the row stages of the
+ *                      special limbs' inverse NTT after it (1, default), or
+ *                      as separate NTT, inner-product and INTT launches (0) */
+typedef struct fides_tuning {
+  uint32_t conv_chunk[3];
+  uint32_t group_by_prime[3];
+  uint32_t fuse_row_dot[3];
+} fides_tuning;
+
+int fides_tuning_get(const fides_context* ctx, fides_tuning* out);
+int fides_tuning_set(fides_context* ctx, const fides_tuning* t);
+
+/* Autotuner: times candidate settings on a synthetic batch shaped like the
+ * caller's (n HMult+relin+Rescale and n Rotate at limb counts l[i], random
+ * residues and a random key; a candidate's time is the best of 3 runs of
+ * `reps` repetitions after one warm-up) by coordinate descent from the
+ * current settings over the size classes the batch reaches (fusion,
+ * grouping, then the conversion chunks); a candidate replaces the incumbent
+ * only when it is at least 2% faster.  Settings of other classes are left
+ * as they are.  Keeps the result in the context and writes it to *out (may
+ * be NULL).  Runs on the
+ * default stream and waits for the device; allocates its inputs, key and
+ * outputs (about n + 3 full ciphertexts and one key) for the duration of the
+ * call.  Refused during graph capture. */
+int fides_autotune(fides_context* ctx, uint32_t n, const uint32_t* l, uint32_t reps, fides_tuning* out);
+
 #ifdef __cplusplus
 }
 #endif
