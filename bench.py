@@ -246,16 +246,41 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
         for k, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
             out.append("| %s | %d | %.3f | %.2f | %.1f%% |" % (k, len(v), sum(v) / 1e6, sum(v) / len(v) / 1e3,
                                                              100.0 * sum(v) / tot))
+        # the last batched step: from its first launch (the HMult batch's
+        # tensor product) to the first launch of the NTT roofline that follows
+        seq = sorted(tr, key=lambda r: int(r["Start_Timestamp"]))
+        name = lambda r: r["Kernel_Name"].split("(")[0]
+        grid = lambda r: int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"])
+        r0 = next((i for i, r in enumerate(seq) if grid(r) == roof_threads and name(r) in ROOF_KERNELS), None)
+        s0 = max((i for i in range(r0 or 0) if name(seq[i]) == "k_tensor_batch"), default=None)
+        if r0 is not None and s0 is not None:
+            win, agg = seq[s0:r0], {}
+            for r in win:
+                k = name(r)
+                if k.startswith("void k_conv_batch<") and not k.startswith("void k_conv_batch<10"):
+                    k = "k_conv_batch<1..9, true> (ModUp of partial digits)"
+                a = agg.setdefault(k, [0, 0.0, 0])
+                a[0] += 1
+                a[1] += (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                a[2] += int(r["Grid_Size_Y"])
+            busy = sum(a[1] for a in agg.values())
+            out.append("")
+            out.append("last batched step: %d launches, %.3f ms of kernel time" % (len(win), busy / 1e3))
+            out.append("| kernel | launches | us | share | tasks |")
+            out.append("|---|---|---|---|---|")
+            for k, (n, us, tasks) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+                out.append("| %s | %d | %.0f | %.1f%% | %d |" % (k, n, us, 100.0 * us / busy, tasks))
         roof = {}
-        for r in sorted(tr, key=lambda r: int(r["Start_Timestamp"])):
+        for r in seq:
             k = r["Kernel_Name"].split("(")[0]
             if int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) == roof_threads and k in ROOF_KERNELS:
                 if len(roof.get(k, [])) < ROOF_LAUNCHES:
                     roof.setdefault(k, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
-        out.append("")
-        out.append("roofline launch (%d limbs): " % ROOF_LIMBS + ", ".join(
-            "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
-        out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in roof.values()))
+        if roof:
+            out.append("")
+            out.append("roofline launch (%d limbs): " % ROOF_LIMBS + ", ".join(
+                "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
+            out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in roof.values()))
     res = {}
     if fetch_dir and write_dir:
         def counter(d, name):
