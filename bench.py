@@ -113,6 +113,7 @@ ROOF_LIMBS = 960  # 32 limbs for each of the 30 Q primes: the size of the batche
 ROOF_REPS = 10
 ROOF_LAUNCHES = 2 + ROOF_REPS  # the roofline's warm-up and timed launches, the first of this size in a run
 ROOF_KERNELS = ("void k_ntt_fwd_col<16, 0>", "void k_ntt_fwd_row<16, 0>")
+VALU_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
 def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
@@ -152,11 +153,24 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
     ms = ev.elapsed_ms(e0, e1) / reps
     alg_bytes = 2 * prm.N * 8 * ROOF_LIMBS
     achieved = alg_bytes / (ms * 1e-3) / 1e9
-    traffic = None
+    traffic, valu = None, None
     tf = os.path.join(ROOT, "profiles", "ntt_fwd_960x65536.json")
     if os.path.exists(tf):
         with open(tf) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            prof = json.load(f)
+        traffic = prof.get("hbm_bytes_per_launch")
+        v = prof.get("valu")
+        if v and all(k in v for k in ROOF_KERNELS):
+            # VALU issue bound: every wave64 VALU instruction holds its SIMD
+            # for 4 cycles; 1024 SIMDs at the 2.4 GHz peak engine clock
+            inst = sum(v[k]["valu_inst_per_butterfly"] for k in ROOF_KERNELS)  # per butterfly of each pass
+            wave_inst = inst * (prm.N // 2) * 8 * ROOF_LIMBS / 64
+            t_min = wave_inst * 4 / (1024 * VALU_CLOCK_GHZ * 1e9)
+            valu = {"bound": "valu issue (1024 SIMDs x %.1f GHz, 4 cycles per wave64 instruction)" % VALU_CLOCK_GHZ,
+                    "inst_per_butterfly": {k: v[k]["valu_inst_per_butterfly"] for k in ROOF_KERNELS},
+                    "min_launch_us": round(t_min * 1e6, 2), "frac": round(t_min / (ms * 1e-3), 4),
+                    "profiled_busy": {k: v[k]["valu_busy"] for k in ROOF_KERNELS},
+                    "profiled_clock_ghz": {k: v[k]["clock_ghz"] for k in ROOF_KERNELS}}
     return {
         "kernel": "ntt_fwd = k_ntt_fwd_col + k_ntt_fwd_row, %d limbs x 2^16 (one fides_ntt call)" % ROOF_LIMBS,
         "bound": "hbm",
@@ -168,6 +182,7 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
         "launch_us": round(ms * 1e3, 2),
         "us_per_limb": round(ms * 1e3 / ROOF_LIMBS, 4),
         "alg_bytes_per_launch": alg_bytes,
+        "valu": valu,
     }
 
 
@@ -220,10 +235,12 @@ def cpu_baseline(prm, budget_s=12.0):
     }
 
 
-def summarize(trace_dir, fetch_dir=None, write_dir=None):
+def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
     """Summarise a rocprofv3 run of this script: per-kernel time of the
-    batched step, the roofline launch's kernels, and (with the two --pmc
-    runs) the HBM bytes per roofline launch scaled by the calibration copy."""
+    batched step, the roofline launch's kernels, (with the two --pmc runs) the
+    HBM bytes per roofline launch scaled by the calibration copy, and (with a
+    --pmc run of SQ_INSTS_VALU, SQ_ACTIVE_INST_VALU, SQ_BUSY_CYCLES and
+    GRBM_GUI_ACTIVE) the VALU work of the roofline launch's two passes."""
     import csv
     import glob
 
@@ -309,6 +326,42 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None):
         out.append("HBM traffic per roofline launch: %.1f MB (fetch scale %.3f, write scale %.3f "
                    "from a 256 MiB calibration copy); algorithmic %.1f MB" % (
                        tot / 1e6, fs, ws, res["alg_bytes_per_launch"] / 1e6))
+    if valu_dir:
+        # per pass of the roofline launch: VALU instructions per butterfly
+        # (wave instructions x 64 lanes / butterflies of the pass) and VALU
+        # busy as rocprofv3 derives it: SQ_ACTIVE_INST_VALU counts 4-cycle
+        # issue slots, GRBM_GUI_ACTIVE is summed over the 8 XCDs, so
+        # busy = 4 ACTIVE / (1024 SIMDs x GUI / 8); a multiply that needs more
+        # than one pass counts once, so this is a lower bound
+        names = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE")
+        per = {}
+        for r in sorted(rows(valu_dir, "*counter_collection.csv"), key=lambda r: int(r["Start_Timestamp"])):
+            k = r["Kernel_Name"].split("(")[0]
+            if int(r["Grid_Size"]) != roof_threads or k not in ROOF_KERNELS or r["Counter_Name"] not in names:
+                continue
+            d = per.setdefault(k, {}).setdefault(r["Dispatch_Id"], {})
+            d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            d["ns"] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        bfly = ROOF_LIMBS * (65536 // 2) * 8  # butterflies of one pass (8 of the 16 stages)
+        simds, xcds = 256 * 4, 8
+        vres = {}
+        for k, ds in per.items():
+            ds = [d for d in ds.values() if all(n in d for n in names)][:ROOF_LAUNCHES]
+            if not ds:
+                continue
+            inst = sum(d["SQ_INSTS_VALU"] for d in ds) / len(ds)
+            act = sum(d["SQ_ACTIVE_INST_VALU"] for d in ds) / len(ds)
+            gui = sum(d["GRBM_GUI_ACTIVE"] for d in ds) / len(ds)
+            ns = sum(d["ns"] for d in ds) / len(ds)
+            vres[k] = {"valu_inst_per_butterfly": round(inst * 64 / bfly, 2),
+                       "valu_busy": round(4 * act / (simds * gui / xcds), 3),
+                       "clock_ghz": round(gui / xcds / ns, 3), "launches": len(ds)}
+        res["valu"] = vres
+        out.append("")
+        out.append("VALU per roofline pass (busy = 4 SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8)): " + "; ".join(
+            "%s %.2f VALU instructions per butterfly, busy %.1f%% at %.2f GHz" % (
+                k, v["valu_inst_per_butterfly"], 100 * v["valu_busy"], v["clock_ghz"])
+            for k, v in vres.items()))
     return "\n".join(out), res
 
 
@@ -505,7 +558,7 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=()):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "--summarize":
         a = sys.argv[2:]
-        text, res = summarize(a[0], a[1] if len(a) > 1 else None, a[2] if len(a) > 2 else None)
+        text, res = summarize(*(a + [None] * (4 - len(a)))[:4])
         print(text)
         if res:
             print(json.dumps(res))
