@@ -61,22 +61,43 @@ __device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, const PrimeC& NP, u
   return shoup(a, 1, NP.one_sh, nm);
 }
 
+// Limb data is in device (global) memory.  Pointers taken from task records
+// are generic, which makes the compiler emit flat loads and stores; those
+// also count against the LDS wait counter, so a wait for LDS data would wait
+// for outstanding memory traffic too.  Accesses through these helpers are
+// global loads and stores.
+__device__ __forceinline__ u64 ldg(const u64* p, u64 i) {
+  return ((const __attribute__((address_space(1))) u64*)p)[i];
+}
+__device__ __forceinline__ void stg(u64* p, u64 i, u64 v) { ((__attribute__((address_space(1))) u64*)p)[i] = v; }
+
+// Epilogue in two parts: the operands of coefficient idx are loaded first (a
+// thread issues a batch of loads before it waits for them), then combined
+// with the transform output v in [0, q).
 template <int EPI>
-__device__ __forceinline__ u64 epilogue(u64 v, u32 idx, const NttTask& t, u64 q) {
+__device__ __forceinline__ void epilogue_load(u32 idx, const NttTask& t, u64& a, u64& b) {
+  if constexpr (EPI == EPI_RESCALE || EPI == EPI_MODDOWN) {
+    a = ldg(t.x, idx);
+  } else if constexpr (EPI == EPI_MODDOWN_ADD) {
+    a = ldg(t.x, idx);
+    b = t.acc ? ldg(t.acc, idx) : 0;
+  } else if constexpr (EPI == EPI_MDRS) {
+    a = ldg(t.acc, idx);
+    b = ldg(t.x, idx);
+  }
+}
+template <int EPI>
+__device__ __forceinline__ u64 epilogue_apply(u64 v, u64 a, u64 b, const NttTask& t, u64 q) {
   if constexpr (EPI == EPI_NONE) {
     return v;
   } else if constexpr (EPI == EPI_RESCALE) {
-    const u64 a = shoup(t.x[idx], t.c0, t.c0sh, q);
-    const u64 b = shoup(v, t.c1, t.c1sh, q);
-    return add_mod(a, b, q);
+    return add_mod(shoup(a, t.c0, t.c0sh, q), shoup(v, t.c1, t.c1sh, q), q);
   } else if constexpr (EPI == EPI_MDRS) {
-    const u64 a = shoup(t.acc[idx], t.c0, t.c0sh, q);
-    const u64 b = shoup(t.x[idx], t.c3, t.c3sh, q);
-    return add_mod(add_mod(a, b, q), v, q);
+    return add_mod(add_mod(shoup(a, t.c0, t.c0sh, q), shoup(b, t.c3, t.c3sh, q), q), v, q);
   } else {
-    u64 r = shoup(t.x[idx] + q - v, t.c0, t.c0sh, q);
+    u64 r = shoup(a + q - v, t.c0, t.c0sh, q);
     if constexpr (EPI == EPI_MODDOWN_ADD) {
-      if (t.acc) r = add_mod(r, t.acc[idx], q);
+      if (t.acc) r = add_mod(r, b, q);
     }
     return r;
   }
@@ -103,21 +124,21 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
     const PrimeC NP = T.prm[pid];
 #pragma unroll
     for (int k = 0; k < C::E; ++k)
-      x[k] = switch_modulus(tk.src[((r0 + C::TPL * k) << C::L2) | col], om, NP, tk.swdiff);
+      x[k] = switch_modulus(ldg(tk.src, ((r0 + C::TPL * k) << C::L2) | col), om, NP, tk.swdiff);
   } else if constexpr (PRO == PRO_MDRS) {
     const u64 om = T.prm[tk.src_pid].q;
     const PrimeC NP = T.prm[pid];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
       const u32 e = ((r0 + C::TPL * k) << C::L2) | col;
-      const u64 a = shoup(switch_modulus(tk.src[e], om, NP, tk.swdiff), tk.c2, tk.c2sh, q);
-      const u64 b = shoup(tk.src2[e], tk.c3, tk.c3sh, q);
+      const u64 a = shoup(switch_modulus(ldg(tk.src, e), om, NP, tk.swdiff), tk.c2, tk.c2sh, q);
+      const u64 b = shoup(ldg(tk.src2, e), tk.c3, tk.c3sh, q);
       x[k] = sub_mod(a, b, q);
     }
   } else {
     const u64* in = tk.src ? tk.src : data;
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) x[k] = in[((r0 + C::TPL * k) << C::L2) | col];
+    for (int k = 0; k < C::E; ++k) x[k] = ldg(in, ((r0 + C::TPL * k) << C::L2) | col);
   }
 #pragma unroll
   for (int u = 0; u < C::A; ++u) {
@@ -151,7 +172,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
       }
     }
 #pragma unroll
-    for (int k = 0; k < EB; ++k) data[(((r1 << C::B) + k) << C::L2) | col] = x[g * EB + k];
+    for (int k = 0; k < EB; ++k) stg(data, (((r1 << C::B) + k) << C::L2) | col, x[g * EB + k]);
   }
 }
 
@@ -172,7 +193,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
   u64* rowp = data + ((u64)row << C::L2);
   u64 x[C::E];
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) x[k] = rowp[r0 + C::TPL * k];
+  for (int k = 0; k < C::E; ++k) x[k] = ldg(rowp, r0 + C::TPL * k);
 #pragma unroll
   for (int u = 0; u < C::A; ++u) {
     const int half = C::E >> (u + 1);
@@ -239,30 +260,44 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) x[k] = ((u64)L[C::lx(r0 + C::TPL * k)] << 32) | lo[k];
   }
   u64* base = (EPI != EPI_NONE && tk.out) ? tk.out : data;
+  constexpr int EPC = C::E < 8 ? C::E : 8;  // epilogue operands loaded per batch
   if (EPI != EPI_NONE && tk.perm) {
     // store through the automorphism X -> X^k as a scatter with k^-1:
     // out[brev(((2 brev(j) + 1) k^-1 mod 2N) >> 1)] = v_j (reference
     // automorph_slot, src/Rotation.cuh:14-26)
     const u32 mask = (2u << LOGN) - 1;
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) {
-      const u32 c = r0 + C::TPL * k;
-      const u32 j = (row << C::L2) | c;
-      u64 v = csub(csub(x[k], q2), q);
-      v = epilogue<EPI>(v, j, tk, q);
-      const u32 jr = __brev(j) >> (32 - LOGN);
-      const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
-      base[__brev(s) >> (32 - LOGN)] = v;
+    for (int k0 = 0; k0 < C::E; k0 += EPC) {
+      u64 ea[EPC], eb[EPC];
+#pragma unroll
+      for (int k = 0; k < EPC; ++k)
+        epilogue_load<EPI>((row << C::L2) | (r0 + C::TPL * (k0 + k)), tk, ea[k], eb[k]);
+#pragma unroll
+      for (int k = 0; k < EPC; ++k) {
+        const u32 c = r0 + C::TPL * (k0 + k);
+        const u32 j = (row << C::L2) | c;
+        u64 v = csub(csub(x[k0 + k], q2), q);
+        v = epilogue_apply<EPI>(v, ea[k], eb[k], tk, q);
+        const u32 jr = __brev(j) >> (32 - LOGN);
+        const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
+        stg(base, __brev(s) >> (32 - LOGN), v);
+      }
     }
     return;
   }
   u64* dst = base + ((u64)row << C::L2);
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) {
-    const u32 c = r0 + C::TPL * k;
-    u64 v = csub(csub(x[k], q2), q);
-    v = epilogue<EPI>(v, (row << C::L2) | c, tk, q);
-    dst[c] = v;
+  for (int k0 = 0; k0 < C::E; k0 += EPC) {
+    u64 ea[EPC], eb[EPC];
+#pragma unroll
+    for (int k = 0; k < EPC; ++k) epilogue_load<EPI>((row << C::L2) | (r0 + C::TPL * (k0 + k)), tk, ea[k], eb[k]);
+#pragma unroll
+    for (int k = 0; k < EPC; ++k) {
+      const u32 c = r0 + C::TPL * (k0 + k);
+      u64 v = csub(csub(x[k0 + k], q2), q);
+      v = epilogue_apply<EPI>(v, ea[k], eb[k], tk, q);
+      stg(dst, c, v);
+    }
   }
 }
 
@@ -328,13 +363,13 @@ __device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, 
   using C = NttCfg<LOGN, 1>;
   if (j == tk.jown) {  // uniform per workgroup
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) out[k] = tk.d[rbase + r0 + C::TPL * k];
+    for (int k = 0; k < C::E; ++k) out[k] = ldg(tk.d, rbase + r0 + C::TPL * k);
     return;
   }
   const u64* rowp = tk.e + j * es + rbase;
   u64 x[C::E];
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) x[k] = rowp[r0 + C::TPL * k];
+  for (int k = 0; k < C::E; ++k) x[k] = ldg(rowp, r0 + C::TPL * k);
 #pragma unroll
   for (int u = 0; u < C::A; ++u) {
     const int half = C::E >> (u + 1);
@@ -400,19 +435,19 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
   for (int k = 0; k < C::E; ++k) {
     const u64 i = rbase + r0 + C::TPL * k;
     Acc128 a0, a1;
-    acc_mac_v(a0, e0[k], tk.kb[i]);
-    acc_mac_v(a1, e0[k], tk.ka[i]);
+    acc_mac_v(a0, e0[k], ldg(tk.kb, i));
+    acc_mac_v(a1, e0[k], ldg(tk.ka, i));
     if constexpr (BETA > 1) {
-      acc_mac_v(a0, e1[k], tk.kb[ks + i]);
-      acc_mac_v(a1, e1[k], tk.ka[ks + i]);
+      acc_mac_v(a0, e1[k], ldg(tk.kb, ks + i));
+      acc_mac_v(a1, e1[k], ldg(tk.ka, ks + i));
     }
     if constexpr (BETA > 2) {
-      acc_mac_v(a0, e2[k], tk.kb[2 * ks + i]);
-      acc_mac_v(a1, e2[k], tk.ka[2 * ks + i]);
+      acc_mac_v(a0, e2[k], ldg(tk.kb, 2 * ks + i));
+      acc_mac_v(a1, e2[k], ldg(tk.ka, 2 * ks + i));
     }
     if constexpr (BETA > 3) {
-      acc_mac_v(a0, e3[k], tk.kb[3 * ks + i]);
-      acc_mac_v(a1, e3[k], tk.ka[3 * ks + i]);
+      acc_mac_v(a0, e3[k], ldg(tk.kb, 3 * ks + i));
+      acc_mac_v(a1, e3[k], ldg(tk.ka, 3 * ks + i));
     }
     u64 h, l;
     acc_value(a0, h, l);
@@ -424,8 +459,8 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
       const u64 i = rbase + r0 + C::TPL * k;
-      tk.o0[i] = e0[k];
-      tk.o1[i] = e1[k];
+      stg(tk.o0, i, e0[k]);
+      stg(tk.o1, i, e1[k]);
     }
     return;
   }
@@ -448,7 +483,7 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
       for (int k = 0; k < EB; ++k) y[g * EB + k] = L[C::lx(((r0 * C::G + g) << C::B) + k)];
     intt_row_stages<LOGN>(row, r0, L, itw, itws, q, q2, y);
 #pragma unroll
-    for (int k = 0; k < C::E; ++k) o[rbase + r0 + C::TPL * k] = y[k];
+    for (int k = 0; k < C::E; ++k) stg(o, rbase + r0 + C::TPL * k, y[k]);
   }
 }
 
@@ -487,11 +522,11 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
     // direct load of the thread's 2^B consecutive elements (no LDS staging;
     // measured faster than a coalesced load through LDS)
 #pragma unroll
-    for (int k = 0; k < EB; ++k) x[g * EB + k] = srow[(r1 << C::B) + k];
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = ldg(srow, (r1 << C::B) + k);
   }
   intt_row_stages<LOGN>(row, r0, L, tw, tws, q, q2, x);
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) rowp[r0 + C::TPL * k] = x[k];
+  for (int k = 0; k < C::E; ++k) stg(rowp, r0 + C::TPL * k, x[k]);
 }
 
 // Column pass: global stages L1-1 .. 0; N^-1 folded into stage 0.  [0, 2q) -> [0, q).
@@ -521,7 +556,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
   for (int g = 0; g < C::G; ++g) {
     const u32 r1 = r0 * C::G + g;
 #pragma unroll
-    for (int k = 0; k < EB; ++k) x[g * EB + k] = data[(((r1 << C::B) + k) << C::L2) | col];
+    for (int k = 0; k < EB; ++k) x[g * EB + k] = ldg(data, (((r1 << C::B) + k) << C::L2) | col);
 #pragma unroll
     for (int v = C::B - 1; v >= 0; --v) {
       const int u = C::A + v;
@@ -559,7 +594,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
     }
   }
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) data[((r0 + C::TPL * k) << C::L2) | col] = x[k];
+  for (int k = 0; k < C::E; ++k) stg(data, ((r0 + C::TPL * k) << C::L2) | col, x[k]);
 }
 
 // ------------------------------------------------------------------ launchers
