@@ -49,17 +49,6 @@ __device__ __forceinline__ void gs_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u6
 }
 
 // ------------------------------------------------------------- prologue/epilogue
-// OpenFHE NativeVector::SwitchModulus (centred lift) of a < om to modulus nm
-// (reference src/CKKS/Rescale.cuh:14-40 quoted, :47-68): values above om/2
-// are treated as negative.  diff = nm - (om mod nm) comes precomputed with
-// the task; the final reduction is a Shoup product by 1 (exact for any
-// input below 2^64) instead of a division.
-__device__ __forceinline__ u64 switch_modulus(u64 a, u64 om, const PrimeC& NP, u64 diff) {
-  const u64 nm = NP.q;
-  if (nm > om) return a > (om >> 1) ? a + (nm - om) : a;
-  if (a > (om >> 1)) a += diff;
-  return shoup(a, 1, NP.one_sh, nm);
-}
 
 // Limb data is in device (global) memory.  Pointers taken from task records
 // are generic, which makes the compiler emit flat loads and stores; those
@@ -74,6 +63,24 @@ __device__ __forceinline__ void stg(u64* p, u64 i, u64 v) { ((__attribute__((add
 // Epilogue in two parts: the operands of coefficient idx are loaded first (a
 // thread issues a batch of loads before it waits for them), then combined
 // with the transform output v in [0, q).
+// OpenFHE NativeVector::SwitchModulus (centred lift) of values a[k] < om to
+// modulus nm (reference src/CKKS/Rescale.cuh:14-40 quoted, :47-68): values
+// above om/2 are treated as negative.  diff = nm - (om mod nm) comes
+// precomputed with the task; the final reduction is a Shoup product by 1
+// (exact for any input below 2^64) instead of a division.  The comparison of
+// the moduli is uniform, so it is made once for all n values.
+template <int n>
+__device__ __forceinline__ void switch_modulus_all(u64 (&a)[n], u64 om, const PrimeC& NP, u64 diff) {
+  const u64 nm = NP.q, half = om >> 1;
+  if (nm > om) {
+#pragma unroll
+    for (int k = 0; k < n; ++k) a[k] = a[k] > half ? a[k] + (nm - om) : a[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < n; ++k) a[k] = shoup(a[k] > half ? a[k] + diff : a[k], 1, NP.one_sh, nm);
+  }
+}
+
 template <int EPI>
 __device__ __forceinline__ void epilogue_load(u32 idx, const NttTask& t, u64& a, u64& b) {
   if constexpr (EPI == EPI_RESCALE || EPI == EPI_MODDOWN) {
@@ -119,22 +126,26 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
   const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
   const u32 col = blockIdx.x * C::LPW + ll;
   u64 x[C::E];
+  // The prologues load all their operands before the arithmetic, and the
+  // workgroup-uniform branch of switch_modulus is taken once for all of them
+  // (per element it would split the loop into blocks, and the compiler would
+  // wait for each load in turn).
   if constexpr (PRO == PRO_SWITCH_MODULUS) {
-    const u64 om = T.prm[tk.src_pid].q;
-    const PrimeC NP = T.prm[pid];
 #pragma unroll
-    for (int k = 0; k < C::E; ++k)
-      x[k] = switch_modulus(ldg(tk.src, ((r0 + C::TPL * k) << C::L2) | col), om, NP, tk.swdiff);
+    for (int k = 0; k < C::E; ++k) x[k] = ldg(tk.src, ((r0 + C::TPL * k) << C::L2) | col);
+    switch_modulus_all<C::E>(x, T.prm[tk.src_pid].q, T.prm[pid], tk.swdiff);
   } else if constexpr (PRO == PRO_MDRS) {
-    const u64 om = T.prm[tk.src_pid].q;
-    const PrimeC NP = T.prm[pid];
+    u64 y[C::E];
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
       const u32 e = ((r0 + C::TPL * k) << C::L2) | col;
-      const u64 a = shoup(switch_modulus(ldg(tk.src, e), om, NP, tk.swdiff), tk.c2, tk.c2sh, q);
-      const u64 b = shoup(ldg(tk.src2, e), tk.c3, tk.c3sh, q);
-      x[k] = sub_mod(a, b, q);
+      x[k] = ldg(tk.src, e);
+      y[k] = ldg(tk.src2, e);
     }
+    switch_modulus_all<C::E>(x, T.prm[tk.src_pid].q, T.prm[pid], tk.swdiff);
+#pragma unroll
+    for (int k = 0; k < C::E; ++k)
+      x[k] = sub_mod(shoup(x[k], tk.c2, tk.c2sh, q), shoup(y[k], tk.c3, tk.c3sh, q), q);
   } else {
     const u64* in = tk.src ? tk.src : data;
 #pragma unroll
