@@ -16,18 +16,19 @@ __global__ __launch_bounds__(256) void k_tensor_batch(const TensorTask* __restri
   const u32 k = blockIdx.x * 256u + threadIdx.x;
   const PrimeC P = prm[t.pid];
   const u64 x0 = t.a0[k], x1 = t.a1[k];
-  u64 r0, r1, r2;
-  if (t.square) {
-    r0 = barrett_mul(x0, x0, P);
-    const u64 m = barrett_mul(x0, x1, P);
-    r1 = add_mod(m, m, P.q);
-    r2 = barrett_mul(x1, x1, P);
-  } else {
-    const u64 y0 = t.b0[k], y1 = t.b1[k];
-    r0 = barrett_mul(x0, y0, P);
-    r1 = add_mod(barrett_mul(x0, y1, P), barrett_mul(x1, y0, P), P.q);
-    r2 = barrett_mul(x1, y1, P);
-  }
+  // Montgomery products: the second factors are taken to Montgomery form
+  // (y R mod q), so REDC(x * yR) = x y mod q; the cross term (< 2 q^2 < q R)
+  // is reduced once.  Every result is the canonical residue.
+  const u64 y0 = t.square ? x0 : t.b0[k], y1 = t.square ? x1 : t.b1[k];
+  const u64 m0 = to_mont(y0, P), m1 = to_mont(y1, P);
+  u64 h, l;
+  mul64wide(x0, m0, h, l);
+  const u64 r0 = redc_once(h, l, P);
+  mul64wide(x0, m1, h, l);
+  mac128(h, l, x1, m0);
+  const u64 r1 = redc_once(h, l, P);
+  mul64wide(x1, m1, h, l);
+  const u64 r2 = redc_once(h, l, P);
   t.o0[k] = r0;
   t.o1[k] = r1;
   t.o2[k] = r2;

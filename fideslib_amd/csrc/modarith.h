@@ -7,13 +7,14 @@
 //   shoup_lazy   x * w for a constant w with precomputed w' = floor(w 2^64 / q)
 //                (Shoup; reference src/ModMult.cuh:83-95).  x < 2^64, result
 //                in [0, 2q).  Used by every NTT butterfly and constant scale.
-//   barrett_mul  a * b for two variable operands in [0, q) (Barrett with
-//                mu = floor(2^(2b) / q), b = bit length of q; reference
-//                src/ModMult.cuh:42-69).  Result in [0, q).
 //   mont_redc    reduction of a 128-bit accumulator T (sum of products with
 //                one operand stored pre-multiplied by R = 2^64).  Used by the
 //                basis conversions and the key-switch inner product, which
 //                accumulate several 128-bit products before one reduction.
+//   redc_once    the same for T < qR, with one conditional subtraction; with
+//                to_mont (y -> yR mod q) it multiplies two variable operands
+//                (the tensor product), where the reference uses Barrett
+//                (src/ModMult.cuh:42-69): both give the canonical residue.
 //
 // All residues are kept below 4q (lazy reduction); q < 2^62 is required so
 // that 4q fits in 64 bits.
@@ -130,20 +131,6 @@ __device__ __forceinline__ u64 csub(u64 x, u64 m) {
 // x * w mod q in [0, q).
 __device__ __forceinline__ u64 shoup(u64 x, u64 w, u64 wp, u64 q) { return csub(shoup_lazy(x, w, wp, q), q); }
 
-// a * b mod q in [0, q) for a, b < q (Barrett, 2 corrections).
-__device__ __forceinline__ u64 barrett_mul(u64 a, u64 b, const PrimeC& P) {
-  u64 hi, lo;
-  mul64wide(a, b, hi, lo);
-  const u32 sh = P.bits - 1;  // T >> (bits-1) < 2^(bits+1) <= 2^63
-  const u64 t = (hi << (64 - sh)) | (lo >> sh);
-  u64 h2, l2;
-  mul64wide(t, P.mu, h2, l2);
-  const u32 s2 = P.bits + 1;  // (t * mu) >> (bits+1)
-  const u64 qh = (h2 << (64 - s2)) | (l2 >> s2);
-  u64 r = lo - qh * P.q;
-  r = csub(r, P.q2);
-  return csub(r, P.q);
-}
 
 // 128-bit accumulate: (hi, lo) += a * b, with the low-word carry taken from
 // an explicit add-with-carry.
@@ -153,7 +140,7 @@ __device__ __forceinline__ void mac128(u64& hi, u64& lo, u64 a, u64 b) {
   const u64 t = (u64)a0 * b1 + (p00 >> 32);  // < 2^64
   const u64 u = (u64)a1 * b0 + (u32)t;       // < 2^64
   const u64 h = (u64)a1 * b1 + (t >> 32) + (u >> 32);
-  const u64 l = (u >> 32 << 32) | (u32)p00;
+  const u64 l = (u << 32) | (u32)p00;
   unsigned long long c;
   lo = __builtin_addcll(lo, l, 0ull, &c);
   hi = hi + h + c;
@@ -213,6 +200,19 @@ __device__ __forceinline__ u64 mont_redc(u64 hi, u64 lo, const PrimeC& P) {
   u64 u = hi + mulhi64(m, P.q) + (lo != 0);
   while (u >= P.q) u -= P.q;
   return u;
+}
+
+// Montgomery reduction of T = (hi, lo) < q R: T R^-1 mod q in [0, q)
+// ((T + m q) / R < 2q, one conditional subtraction).
+__device__ __forceinline__ u64 redc_once(u64 hi, u64 lo, const PrimeC& P) {
+  const u64 m = lo * P.qneg_inv;
+  return csub(hi + mulhi64(m, P.q) + (lo != 0), P.q);
+}
+// y R mod q for y < q: REDC(y (R^2 mod q)).
+__device__ __forceinline__ u64 to_mont(u64 y, const PrimeC& P) {
+  u64 hi, lo;
+  mul64wide(y, P.r2, hi, lo);
+  return redc_once(hi, lo, P);
 }
 
 __device__ __forceinline__ u64 add_mod(u64 a, u64 b, u64 q) { return csub(a + b, q); }
