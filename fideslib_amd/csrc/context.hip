@@ -361,7 +361,6 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     for (int it = 0; it < 6; ++it) inv *= 2 - q * inv;
     p.qneg_inv = (u64)0 - inv;
     p.bits = bitlen(q);
-    p.mu = (u64)(((u128)1 << (2 * p.bits)) / q);
     const u64 r1 = (u64)(((u128)1 << 64) % q);
     p.r2 = mulm(r1, r1, q);
     const u64 ipsi = invm(c->roots[i], q);

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__
       // T = L + (M1 + M2) 2^30 + H 2^60
       const unsigned __int128 T = (unsigned __int128)L + (((unsigned __int128)M1 + M2) << 30) +
                                   ((unsigned __int128)H << 60);
-      out[((u64)d->dst_off[t] << logN) + k] = mont_redc((u64)(T >> 64), (u64)T, P);
+      // T < NS q^2 <= 16 q^2 < q 2^64 (q < 2^60): one conditional subtraction
+      out[((u64)d->dst_off[t] << logN) + k] = redc_once((u64)(T >> 64), (u64)T, P);
     }
     return;
   }

@@ -30,7 +30,6 @@ struct PrimeC {
   u64 q;         // the prime
   u64 q2;        // 2q
   u64 qneg_inv;  // -q^-1 mod 2^64 (Montgomery)
-  u64 mu;        // floor(2^(2*bits) / q) (Barrett)
   u64 r2;        // R^2 mod q, R = 2^64
   u64 ninv;      // N^-1 mod q
   u64 ninv_sh;   // Shoup companion of ninv
