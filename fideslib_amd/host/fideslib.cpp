@@ -712,4 +712,36 @@ std::vector<Ciphertext> CryptoContextImpl::EvalRotateMany(const std::vector<Ciph
 
 void CryptoContextImpl::Synchronize() const { chk(fides_stream_sync(nullptr), "Synchronize"); }
 
+static CryptoContextImpl::Tuning from_c(const fides_tuning& t) {
+  CryptoContextImpl::Tuning r;
+  for (int c = 0; c < 3; ++c) {
+    r.convChunk[c] = t.conv_chunk[c];
+    r.groupByPrime[c] = t.group_by_prime[c];
+    r.fuseRowDot[c] = t.fuse_row_dot[c];
+  }
+  return r;
+}
+
+CryptoContextImpl::Tuning CryptoContextImpl::GetTuning() const {
+  fides_tuning t;
+  chk(fides_tuning_get(ctx, &t), "GetTuning");
+  return from_c(t);
+}
+
+void CryptoContextImpl::SetTuning(const Tuning& r) {
+  fides_tuning t;
+  for (int c = 0; c < 3; ++c) {
+    t.conv_chunk[c] = r.convChunk[c];
+    t.group_by_prime[c] = r.groupByPrime[c];
+    t.fuse_row_dot[c] = r.fuseRowDot[c];
+  }
+  chk(fides_tuning_set(ctx, &t), "SetTuning");
+}
+
+CryptoContextImpl::Tuning CryptoContextImpl::Autotune(const std::vector<uint32_t>& limbs, uint32_t reps) {
+  fides_tuning t;
+  chk(fides_autotune(ctx, (uint32_t)limbs.size(), limbs.data(), reps, &t), "Autotune");
+  return from_c(t);
+}
+
 }  // namespace fideslib

@@ -227,6 +227,20 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
 
   void Synchronize() const;
 
+  // Launch shapes (the C ABI's fides_tuning, per size class of a key-switch
+  // batch); no setting changes a result bit.  No reference counterpart: the
+  // reference fixes them at compile time.
+  struct Tuning {
+    uint32_t convChunk[3];
+    uint32_t groupByPrime[3];
+    uint32_t fuseRowDot[3];
+  };
+  Tuning GetTuning() const;
+  void SetTuning(const Tuning& t);
+  // Times candidate shapes on a synthetic batch of HMult+Rescale and Rotate
+  // at these limb counts, keeps the fastest in the context and returns it.
+  Tuning Autotune(const std::vector<uint32_t>& limbs, uint32_t reps = 2);
+
   // Bootstrapping of full-slot ciphertexts (api/CryptoContext.hpp:183-186;
   // reference Bootstrap.cu:169-333): the setup computes the linear-transform
   // plaintexts and the modular-reduction series and returns the rotation

@@ -359,6 +359,14 @@ int main(int argc, char** argv) {
     auto cc = fideslib::CryptoContextImpl::Create(rp);
     cc->AddEvalMultKey(key_of(read_u64(dir + "/rlk.bin"), dnum, L + K, N));
     cc->AddRotationKey(rot, key_of(read_u64(dir + "/rot.bin"), dnum, L + K, N));
+    // the operations below run under autotuned launch shapes (same bits)
+    const auto tuned = cc->Autotune({l, l, l - 1});
+    const auto now = cc->GetTuning();
+    for (int c = 0; c < 3; ++c)
+      if (now.convChunk[c] != tuned.convChunk[c] || now.groupByPrime[c] != tuned.groupByPrime[c] ||
+          now.fuseRowDot[c] != tuned.fuseRowDot[c])
+        throw std::runtime_error("autotuned settings not kept");
+    cc->SetTuning(now);
     const double scale = (double)rp.moduli[l - 1];
     auto a = cc->LoadCiphertext(ct_of(read_u64(dir + "/c1.bin"), l, N, scale));
     auto b = cc->LoadCiphertext(ct_of(read_u64(dir + "/c2.bin"), l, N, scale));

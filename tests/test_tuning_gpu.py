@@ -6,7 +6,8 @@ the same bits: a batch of HMult+relin+Rescale and Rotate at mixed levels
 (N = 2^16, 30 Q limbs, K = 10, dnum = 3, random residues and a random key) is
 run under each setting and compared with the default.  The autotuner keeps a
 valid setting in the context, and results after it are unchanged.  Invalid
-settings are refused.
+settings, and the autotuner during a graph capture, are refused; settings
+belong to their context.
 """
 import hashlib
 
@@ -112,4 +113,24 @@ def test_autotune_keeps_a_valid_setting(setup):
     # the batch reaches classes 1 (ModDown) and 2 (ModUp): class 0 is untouched
     assert chosen["conv_chunk"][0] == 8 and chosen["group_by_prime"][0] == 1 and chosen["fuse_row_dot"][0] == 1
     assert run_batch(setup, 24) == want
+    gctx.tuning = DEFAULT
+
+
+def test_autotune_refused_during_capture_and_settings_per_context(setup):
+    prm, gctx = setup[0], setup[1]
+    levels, cts = setup[3], setup[4]
+    st = capi.Stream()
+    out = capi.DeviceBuffer(2 * levels[1] * prm.N)
+    gctx.graph_begin(st, 1 << 26, 1 << 20)
+    with pytest.raises(capi.FidesError):
+        gctx.autotune([prm.L])
+    gctx.add_batch([cts[1]], [cts[1]], [levels[1]], [out], stream=st.h)
+    g = gctx.graph_end()
+    g.launch(st)
+    st.sync()
+    g.close()
+    st.close()
+    other = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx.tuning = dict(DEFAULT, conv_chunk=[4, 4, 4])
+    assert other.tuning == DEFAULT  # settings belong to their context
     gctx.tuning = DEFAULT
