@@ -209,6 +209,31 @@ def test_cfg1_keyswitch_hmult_rescale_rotate(cfg1):
     assert C.precision_bits(dec, z[0] * z[1]) > 20
 
 
+def test_cfg1_hmult_square_extreme_residues(cfg1):
+    """HMult and Square with residues at the ends of [0, q): every coefficient
+    q - 1, 0 or 1 in the inputs (the largest products the tensor product's
+    Montgomery reductions see), against the oracle."""
+    prm, octx, gctx, keys = cfg1
+    N, l = prm.N, prm.L
+    rlk = keys.relin_key()
+    g_rlk = gctx.upload_key(rlk)
+    rng = np.random.default_rng(44)
+    qs = np.asarray(prm.q[:l], np.uint64)[None, :, None]
+    full = np.broadcast_to(qs - np.uint64(1), (2, l, N)).copy()
+    mixed = np.where(rng.integers(0, 3, (2, l, N)) == 0, qs - np.uint64(1),
+                     rng.integers(0, 2, (2, l, N)).astype(np.uint64))
+    for x, y in ((full, full), (full, mixed), (mixed, full)):
+        a, b = capi.DeviceBuffer.from_array(x), capi.DeviceBuffer.from_array(y)
+        m = capi.DeviceBuffer(2 * l * N)
+        gctx.hmult(a, b, l, g_rlk, m)
+        assert np.array_equal(m.download((2, l, N)), octx.hmult_relin(x, y, l, rlk))
+    for x in (full, mixed):
+        a = capi.DeviceBuffer.from_array(x)
+        m = capi.DeviceBuffer(2 * l * N)
+        gctx.square(a, l, g_rlk, m)
+        assert np.array_equal(m.download((2, l, N)), octx.hmult_relin(x, x, l, rlk))
+
+
 def test_cfg1_square_and_lower_levels(cfg1):
     prm, octx, gctx, keys = cfg1
     N = prm.N
