@@ -186,6 +186,91 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
     }
 
 
+# ----------------------------------------------------------------- single primitives
+def primitive_inputs():
+    """Seeded inputs of the primitive timings, shared by the GPU leg and the
+    CPU baseline: config 2 (N = 2^16, 32 Q limbs of FIDESlib's 51-bit test
+    primes + 8 special, dnum 4: digits of 8 limbs) and config 1 (N = 2^13, 4
+    limbs, dnum 2) with a synthetic key."""
+    rng = np.random.default_rng(SEED + 5)
+    p2, p1 = params.config2(), params.config1()
+    x2 = rand_limbs(rng, p2.q, p2.N)                                    # [32][N]
+    ext2 = rand_limbs(rng, p2.q + p2.p, p2.N)                           # [40][N]
+    a1 = np.stack([rand_limbs(rng, p1.q, p1.N) for _ in range(2)])      # [2][4][N]
+    b1 = np.stack([rand_limbs(rng, p1.q, p1.N) for _ in range(2)])
+    k1 = np.stack([np.stack([rand_limbs(rng, p1.q + p1.p, p1.N) for _ in range(p1.dnum)]) for _ in range(2)])
+    return p2, p1, x2, ext2, a1, b1, k1
+
+
+def primitives_leg(ev, reps=20):
+    """GPU time of single operations (HIP events over `reps` calls): NTT and
+    INTT of 32 limbs, ModUp of one 8-limb digit to 40 limbs and ModDown 40 ->
+    32 at N = 2^16; HMult+relin+Rescale and Rotate(+1) at N = 2^13, 4 limbs."""
+    p2, p1, x2, ext2, a1, b1, k1 = primitive_inputs()
+    c2 = capi.Context(p2.logN, p2.q, p2.p, p2.dnum)
+    c1 = capi.Context(p1.logN, p1.q, p1.p, p1.dnum)
+    L2, N2, L1, N1 = p2.L, p2.N, p1.L, p1.N
+    x = capi.DeviceBuffer.from_array(x2)
+    e = capi.DeviceBuffer.from_array(ext2)
+    up = capi.DeviceBuffer((L2 + p2.K) * N2)
+    dn = capi.DeviceBuffer(L2 * N2)
+    a, b = capi.DeviceBuffer.from_array(a1), capi.DeviceBuffer.from_array(b1)
+    m = capi.DeviceBuffer(2 * L1 * N1)
+    r = capi.DeviceBuffer(2 * (L1 - 1) * N1)
+    o = capi.DeviceBuffer(2 * (L1 - 1) * N1)
+    key = c1.upload_key(k1)
+    pids = list(range(L2))
+
+    def t(ctx, fn):
+        fn()
+        ctx.sync()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(reps):
+            fn()
+        ev.record(e1)
+        return round(ev.elapsed_ms(e0, e1) / reps * 1e3, 2)
+
+    res = {
+        "cfg2": {"ntt_32_limbs_us": t(c2, lambda: c2.ntt(x, pids)),
+                 "intt_32_limbs_us": t(c2, lambda: c2.intt(x, pids)),
+                 "modup_8_to_40_us": t(c2, lambda: c2.modup(x, L2, 0, up)),
+                 "moddown_40_to_32_us": t(c2, lambda: c2.moddown(e, L2, dn))},
+        "cfg1": {"hmult_relin_rescale_us": t(c1, lambda: (c1.hmult(a, b, L1, key, m), c1.rescale(m, L1, r))),
+                 "rotate_us": t(c1, lambda: c1.rotate(r, L1 - 1, key, 5, o))},
+    }
+    res["note"] = ("single calls, inputs resident; cfg2 = N 2^16, 32+8 limbs of FIDESlib's 51-bit test primes; "
+                   "cfg1 = N 2^13, 4 limbs, dnum 2")
+    return res
+
+
+def cpu_primitives(reps=2):
+    """The oracle on the inputs of primitives_leg (best of `reps`, ms)."""
+    from oracle import oracle as O
+    p2, p1, x2, ext2, a1, b1, k1 = primitive_inputs()
+    o2 = O.Context(p2.logN, p2.q, p2.p, p2.dnum)
+    o1 = O.Context(p1.logN, p1.q, p1.p, p1.dnum)
+    pids = list(range(p2.L))
+
+    def t(fn):
+        best = None
+        for _ in range(reps):
+            t0 = time.time()
+            fn()
+            d = (time.time() - t0) * 1e3
+            best = d if best is None else min(best, d)
+        return round(best, 3)
+
+    return {
+        "cfg2": {"ntt_32_limbs_ms": t(lambda: o2.ntt_limbs(x2, pids)),
+                 "intt_32_limbs_ms": t(lambda: o2.intt_limbs(x2, pids)),
+                 "modup_8_to_40_ms": t(lambda: o2.modup_digit(x2, p2.L, 0)),
+                 "moddown_40_to_32_ms": t(lambda: o2.moddown(ext2, p2.L))},
+        "cfg1": {"hmult_relin_rescale_ms": t(lambda: o1.rescale_ct(o1.hmult_relin(a1, b1, p1.L, k1), p1.L)),
+                 "rotate_ms": t(lambda: o1.rotate(np.ascontiguousarray(a1[:, :p1.L - 1]), p1.L - 1, k1, 5))},
+    }
+
+
 def cpu_baseline(prm, budget_s=12.0):
     """Oracle HMult + relin + Rescale at the same parameters, levels drawn as
     in the workload, for about budget_s seconds (own CPU reference)."""
@@ -222,6 +307,7 @@ def cpu_baseline(prm, budget_s=12.0):
         if time.time() - t1 > budget_s / 2:
             break
     O.set_threads(prev)
+    prims = cpu_primitives()
     return {
         "value": round(done / el, 3),
         "unit": "key-switch ops/s (HMult+relin+Rescale)",
@@ -232,6 +318,7 @@ def cpu_baseline(prm, budget_s=12.0):
         "single_thread": {"value": round(len(st) / sum(st), 3), "cores": 1,
                           "sample": "%d of the same operations (the first levels of the draw), %.1f s"
                                     % (len(st), sum(st))},
+        "primitives": prims,
     }
 
 
@@ -661,6 +748,8 @@ def main():
     }
     out["config"]["hoisted_rotations"] = hoisted_leg(prm, ctx, w, ev)
     print("bench: hoisted %s" % out["config"]["hoisted_rotations"], file=sys.stderr, flush=True)
+    out["config"]["primitives"] = primitives_leg(ev)
+    print("bench: primitives %s" % out["config"]["primitives"], file=sys.stderr, flush=True)
     if not (args.no_serve and args.no_boot):
         from fideslib_amd import bootstrap as BT
         print("bench: bootstrap plaintexts", file=sys.stderr, flush=True)
