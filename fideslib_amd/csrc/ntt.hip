@@ -544,7 +544,12 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
 template <int LOGN>
 __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
   using C = NttCfg<LOGN, 0>;
-  __shared__ u64 lds[C::LPW * C::LDS_LINE];
+  // The exchange moves 32-bit halves (low words, then high words) through a
+  // line of M + 4 words per column: half the LDS of a 64-bit line, so LDS no
+  // longer caps the pass at 4 waves per SIMD; the stride keeps the strided
+  // writes of a wave on distinct banks.
+  constexpr int S32 = C::M + 4;
+  __shared__ u32 lds32[C::LPW * S32];
   u64* data;
   u32 pid;
   if (span.list) {
@@ -560,7 +565,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
   const u32 tid = threadIdx.x;
   const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
   const u32 col = blockIdx.x * C::LPW + ll;
-  u64* L = lds + ll * C::LDS_LINE;
+  u32* L = lds32 + ll * S32;
   u64 x[C::E];
   constexpr int EB = 1 << C::B;
 #pragma unroll
@@ -579,12 +584,25 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
         gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
       }
     }
-#pragma unroll
-    for (int k = 0; k < EB; ++k) L[(r1 << C::B) + k] = x[g * EB + k];
   }
-  __syncthreads();
+  {
+    u32 lo[C::E];
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) x[k] = L[r0 + C::TPL * k];
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) L[((r0 * C::G + g) << C::B) + k] = (u32)x[g * EB + k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) lo[k] = L[r0 + C::TPL * k];
+    __syncthreads();
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int k = 0; k < EB; ++k) L[((r0 * C::G + g) << C::B) + k] = (u32)(x[g * EB + k] >> 32);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < C::E; ++k) x[k] = ((u64)L[r0 + C::TPL * k] << 32) | lo[k];
+  }
 #pragma unroll
   for (int u = C::A - 1; u >= 1; --u) {
     const int half = C::E >> (u + 1);
