@@ -234,6 +234,43 @@ def test_cfg1_hmult_square_extreme_residues(cfg1):
         assert np.array_equal(m.download((2, l, N)), octx.hmult_relin(x, x, l, rlk))
 
 
+@pytest.mark.parametrize("logN", [10, 11, 12, 14, 15, 17])
+def test_key_switch_ops_every_ring_dimension(logN):
+    """HMult+relin, Rescale, Rotate and a mixed-level batched HMult+Rescale at
+    the ring dimensions the other tests do not use (each has its own NTT,
+    fused row-pass and inner-product kernels): 4 Q limbs (60 + 3 x 50 bits),
+    dnum 2, random residues and a random key, against the oracle."""
+    prm = params.ckks_params(logN, 60, 50, 3, dnum=2)
+    say("key switch logN=%d K=%d" % (logN, prm.K))
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(100 + logN)
+    key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    gkey = gctx.upload_key(key)
+    cts = [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in (L, L, 3, 2)]
+    a, b = capi.DeviceBuffer.from_array(cts[0]), capi.DeviceBuffer.from_array(cts[1])
+    m = capi.DeviceBuffer(2 * L * N)
+    gctx.hmult(a, b, L, gkey, m)
+    want_m = octx.hmult_relin(cts[0], cts[1], L, key)
+    assert np.array_equal(m.download((2, L, N)), want_m)
+    r = capi.DeviceBuffer(2 * (L - 1) * N)
+    gctx.rescale(m, L, r)
+    want_r = octx.rescale_ct(want_m, L)
+    assert np.array_equal(r.download((2, L - 1, N)), want_r)
+    k = pow(5, 3, 2 * N)
+    o = capi.DeviceBuffer(2 * (L - 1) * N)
+    gctx.rotate(r, L - 1, gkey, k, o)
+    assert np.array_equal(o.download((2, L - 1, N)), octx.rotate(want_r, L - 1, key, k))
+    levels = [L, 3, 2]
+    A = [capi.DeviceBuffer.from_array(c) for c in (cts[0], cts[2], cts[3])]
+    OB = [capi.DeviceBuffer(2 * (l - 1) * N) for l in levels]
+    gctx.hmult_batch(A, A, levels, gkey, OB, rescale=True)
+    for c, l, ob in zip((cts[0], cts[2], cts[3]), levels, OB):
+        want = octx.rescale_ct(octx.hmult_relin(c, c, l, key), l)
+        assert np.array_equal(ob.download((2, l - 1, N)), want), l
+
+
 def test_cfg1_square_and_lower_levels(cfg1):
     prm, octx, gctx, keys = cfg1
     N = prm.N
