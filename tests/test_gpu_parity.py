@@ -271,6 +271,34 @@ def test_key_switch_ops_every_ring_dimension(logN):
         assert np.array_equal(ob.download((2, l - 1, N)), want), l
 
 
+@pytest.mark.parametrize("dnum", [1, 6, 12])
+def test_key_switch_digit_counts(dnum):
+    """Digit counts outside the fused inner product's range (more than 4 active
+    digits run the row pass and the inner product as separate launches) and a
+    single digit: N = 2^12, 12 Q limbs, HMult+relin+Rescale and Rotate at
+    l = 12 and l = 7, against the oracle."""
+    prm = params.ckks_params(12, 60, 40, 11, dnum=dnum)
+    say("key switch dnum=%d K=%d" % (dnum, prm.K))
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(200 + dnum)
+    key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    gkey = gctx.upload_key(key)
+    for l in (L, 7):
+        x = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        y = np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])
+        a, b = capi.DeviceBuffer.from_array(x), capi.DeviceBuffer.from_array(y)
+        m, r = capi.DeviceBuffer(2 * l * N), capi.DeviceBuffer(2 * (l - 1) * N)
+        gctx.hmult(a, b, l, gkey, m)
+        gctx.rescale(m, l, r)
+        want_r = octx.rescale_ct(octx.hmult_relin(x, y, l, key), l)
+        assert np.array_equal(r.download((2, l - 1, N)), want_r), l
+        o = capi.DeviceBuffer(2 * l * N)
+        gctx.rotate(a, l, gkey, 5, o)
+        assert np.array_equal(o.download((2, l, N)), octx.rotate(x, l, key, 5)), l
+
+
 def test_cfg1_square_and_lower_levels(cfg1):
     prm, octx, gctx, keys = cfg1
     N = prm.N
