@@ -299,6 +299,35 @@ def test_key_switch_digit_counts(dnum):
         assert np.array_equal(o.download((2, l, N)), octx.rotate(x, l, key, 5)), l
 
 
+@pytest.mark.parametrize("bits", [61, 62])
+def test_key_switch_wide_primes(bits):
+    """Primes at or above 2^60 (up to the 2^62 limit): the basis conversions
+    switch from the carry-free split multiply-accumulate to the column
+    accumulator with counted carries.  N = 2^12, 4 Q limbs + special primes of
+    the same width, HMult+relin+Rescale and Rotate against the oracle."""
+    prm = params.ckks_params(12, bits, bits, 3, dnum=2, special_bits=bits)
+    assert max(prm.q + prm.p) >= 1 << 60
+    say("key switch %d-bit primes" % bits)
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(300 + bits)
+    key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    gkey = gctx.upload_key(key)
+    x = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])
+    y = np.stack([rand_limbs(rng, prm.q, N) for _ in range(2)])
+    a, b = capi.DeviceBuffer.from_array(x), capi.DeviceBuffer.from_array(y)
+    m, r = capi.DeviceBuffer(2 * L * N), capi.DeviceBuffer(2 * (L - 1) * N)
+    gctx.hmult(a, b, L, gkey, m)
+    gctx.rescale(m, L, r)
+    want_m = octx.hmult_relin(x, y, L, key)
+    assert np.array_equal(m.download((2, L, N)), want_m)
+    assert np.array_equal(r.download((2, L - 1, N)), octx.rescale_ct(want_m, L))
+    o = capi.DeviceBuffer(2 * L * N)
+    gctx.rotate(a, L, gkey, 5, o)
+    assert np.array_equal(o.download((2, L, N)), octx.rotate(x, L, key, 5))
+
+
 def test_cfg1_square_and_lower_levels(cfg1):
     prm, octx, gctx, keys = cfg1
     N = prm.N
