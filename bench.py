@@ -315,6 +315,8 @@ def cpu_baseline(prm, budget_s=12.0):
         "kind": "port",
         "sample": "%d HMult+relin+Rescale at N=2^16, limbs drawn U{6..30} (first %s), %.1f s of oracle time; "
                   "own CPU oracle (OpenMP), not OpenFHE" % (done, levels[:4], el),
+        "op_ms": {"median": round(float(np.median(ops)) * 1e3, 2), "p99": round(float(np.percentile(ops, 99)) * 1e3, 2),
+                  "note": "per HMult+relin+Rescale, levels of the draw"},
         "single_thread": {"value": round(len(st) / sum(st), 3), "cores": 1,
                           "sample": "%d of the same operations (the first levels of the draw), %.1f s"
                                     % (len(st), sum(st))},
