@@ -665,6 +665,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-boot", action="store_true")
     ap.add_argument("--boot-only", action="store_true", help="only the bootstrap leg (for profiling)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving leg (for profiling)")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=80.0, help="Poisson arrival rate, requests/s")
@@ -694,6 +695,14 @@ def main():
         boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
         boot.build_plaintexts()
         print(json.dumps({"bootstrap": bootstrap_leg(prm, ctx, w, boot, bench_keys(prm, w, boot))}))
+        return
+    if args.serve_only:
+        from fideslib_amd import bootstrap as BT
+        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
+        boot.build_plaintexts()
+        more = [float(x) for x in args.serve_load_points.split(",") if x.strip()]
+        print(json.dumps({"serve": serve_leg(prm, ctx, w, boot, bench_keys(prm, w, boot), args.serve_requests,
+                                             args.serve_rate, SEED + 5, more)}))
         return
 
     def timed(step):
