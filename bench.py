@@ -423,10 +423,14 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
         # busy = 4 ACTIVE / (1024 SIMDs x GUI / 8); a multiply that needs more
         # than one pass counts once, so this is a lower bound
         names = ("SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_BUSY_CYCLES", "GRBM_GUI_ACTIVE")
-        per = {}
+        per, allk = {}, {}
         for r in sorted(rows(valu_dir, "*counter_collection.csv"), key=lambda r: int(r["Start_Timestamp"])):
             k = r["Kernel_Name"].split("(")[0]
-            if int(r["Grid_Size"]) != roof_threads or k not in ROOF_KERNELS or r["Counter_Name"] not in names:
+            if r["Counter_Name"] not in names:
+                continue
+            a = allk.setdefault(k, {}).setdefault(r["Dispatch_Id"], {})
+            a[r["Counter_Name"]] = a.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            if int(r["Grid_Size"]) != roof_threads or k not in ROOF_KERNELS:
                 continue
             d = per.setdefault(k, {}).setdefault(r["Dispatch_Id"], {})
             d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
@@ -446,6 +450,21 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
                        "valu_busy": round(4 * act / (simds * gui / xcds), 3),
                        "clock_ghz": round(gui / xcds / ns, 3), "launches": len(ds)}
         res["valu"] = vres
+        # every kernel of the counter run: VALU busy over all its dispatches
+        out.append("")
+        out.append("VALU busy per kernel over the whole counter run (weighted by GRBM_GUI_ACTIVE; short "
+                   "dispatches can read slightly above 100%):")
+        out.append("| kernel | dispatches | busy |")
+        out.append("|---|---|---|")
+        tab = []
+        for k, ds in allk.items():
+            ds = [d for d in ds.values() if all(n in d for n in names)]
+            act = sum(d["SQ_ACTIVE_INST_VALU"] for d in ds)
+            gui = sum(d["GRBM_GUI_ACTIVE"] for d in ds)
+            if ds and gui > 0:
+                tab.append((gui, k, len(ds), 4 * act / (simds * gui / xcds)))
+        for gui, k, n, b in sorted(tab, reverse=True)[:14]:
+            out.append("| %s | %d | %.1f%% |" % (k, n, 100 * b))
         out.append("")
         out.append("VALU per roofline pass (busy = 4 SQ_ACTIVE_INST_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8)): " + "; ".join(
             "%s %.2f VALU instructions per butterfly, busy %.1f%% at %.2f GHz" % (
