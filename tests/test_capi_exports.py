@@ -60,3 +60,17 @@ def test_context_rejects_bad_parameters(lib):
     # K = 0
     prm = capi.Params(10, 1, 0, 1, q.ctypes.data_as(u64p), p.ctypes.data_as(u64p), None)
     assert lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)) == -1
+
+
+def test_tuning_null_arguments_and_dict_round_trip(lib):
+    """Tuning entry points refuse null handles without touching a GPU; the
+    binding's dict form round-trips and needs one entry per size class."""
+    t = capi.Tuning()
+    assert lib.fides_tuning_get(None, ctypes.byref(t)) == -1
+    assert lib.fides_tuning_set(None, ctypes.byref(t)) == -1
+    lv = (ctypes.c_uint32 * 1)(30)
+    assert lib.fides_autotune(None, 1, lv, 2, ctypes.byref(t)) == -1
+    d = {"conv_chunk": [8, 16, 32], "group_by_prime": [0, 1, 1], "fuse_row_dot": [1, 0, 1]}
+    assert capi.Tuning.from_dict(d).as_dict() == d
+    with pytest.raises(capi.FidesError):
+        capi.Tuning.from_dict(dict(d, conv_chunk=[8, 16]))
