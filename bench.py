@@ -677,7 +677,7 @@ def main():
         return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--mode", default="batched", choices=["plain", "batched"])
     ap.add_argument("--no-cpu", action="store_true")
