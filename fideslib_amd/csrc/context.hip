@@ -1562,7 +1562,8 @@ extern "C" int fides_tuning_get(const fides_context* c, fides_tuning* t) {
 extern "C" int fides_tuning_set(fides_context* c, const fides_tuning* t) {
   if (!c || !t) return fail(FIDES_ERR_ARG, "null argument");
   for (u32 b = 0; b < 3; ++b) {
-    if (t->conv_chunk[b] < 1 || t->conv_chunk[b] > 32) return fail(FIDES_ERR_ARG, "conv_chunk must be in 1..32");
+    if (t->conv_chunk[b] < 1 || t->conv_chunk[b] > CONV_MAXT)
+      return fail(FIDES_ERR_ARG, "conv_chunk must be in 1..96");
     if (t->group_by_prime[b] > 1 || t->fuse_row_dot[b] > 1) return fail(FIDES_ERR_ARG, "flags must be 0 or 1");
   }
   c->tune = *t;
@@ -1690,7 +1691,7 @@ extern "C" int fides_autotune(fides_context* c, uint32_t n, const uint32_t* l, u
     rc = trial(t);
   }
   for (u32 cl : classes)
-    for (u32 v : {4u, 8u, 16u, 24u, 32u}) {
+    for (u32 v : {4u, 8u, 16u, 24u, 32u, 48u, 64u}) {
       if (rc != FIDES_OK) break;
       fides_tuning t = best;
       t.conv_chunk[cl] = v;

@@ -299,7 +299,7 @@ int fides_graph_destroy(fides_graph* graph);
  * Each setting is kept per size class c of a key-switch batch: a conversion
  * (ModUp of all digits, or ModDown) with < 512 (c = 0), < 1024 (1) or more
  * (2) target limbs in total; the flags follow the class of the ModUp.
- *   conv_chunk[c]      targets per basis-conversion task, 1..32
+ *   conv_chunk[c]      targets per basis-conversion task, 1..96
  *                      (defaults 8, 16, 32)
  *   group_by_prime[c]  order each NTT task list by prime so one prime's
  *                      twiddle tables stay in L2 (1, default) or keep

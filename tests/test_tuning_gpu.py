@@ -69,6 +69,7 @@ SETTINGS = [
     {"conv_chunk": [1, 1, 1], "group_by_prime": [1, 0, 1], "fuse_row_dot": [1, 0, 0]},
     {"conv_chunk": [32, 32, 32], "group_by_prime": [0, 0, 0], "fuse_row_dot": [0, 0, 0]},
     {"conv_chunk": [5, 7, 3], "group_by_prime": [0, 1, 0], "fuse_row_dot": [0, 1, 1]},
+    {"conv_chunk": [96, 64, 48], "group_by_prime": [1, 1, 1], "fuse_row_dot": [1, 1, 1]},
 ]
 
 
@@ -93,7 +94,7 @@ def test_every_setting_gives_the_same_bits(setup, n):
 def test_invalid_settings_refused(setup):
     gctx = setup[1]
     before = gctx.tuning
-    for bad in (dict(DEFAULT, conv_chunk=[0, 16, 32]), dict(DEFAULT, conv_chunk=[8, 33, 32]),
+    for bad in (dict(DEFAULT, conv_chunk=[0, 16, 32]), dict(DEFAULT, conv_chunk=[8, 97, 32]),
                 dict(DEFAULT, group_by_prime=[1, 2, 1]), dict(DEFAULT, fuse_row_dot=[1, 1, 7])):
         with pytest.raises(capi.FidesError):
             gctx.tuning = bad
@@ -108,7 +109,7 @@ def test_autotune_keeps_a_valid_setting(setup):
     chosen = gctx.autotune(setup[3], reps=2)
     say("autotune chose %s" % chosen)
     assert gctx.tuning == chosen
-    assert all(1 <= v <= 32 for v in chosen["conv_chunk"])
+    assert all(1 <= v <= 96 for v in chosen["conv_chunk"])
     assert all(v in (0, 1) for v in chosen["group_by_prime"] + chosen["fuse_row_dot"])
     # the batch reaches classes 1 (ModDown) and 2 (ModUp): class 0 is untouched
     assert chosen["conv_chunk"][0] == 8 and chosen["group_by_prime"][0] == 1 and chosen["fuse_row_dot"][0] == 1
