@@ -186,6 +186,27 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
     }
 
 
+# ----------------------------------------------------------------- batch size
+def batch_size_leg(ctx, w, ev, sizes=(1, 2, 4, 8, 16, 32, 64), reps=5):
+    """Key-switch ops/s of the batched step on the first n ciphertexts of the
+    workload (n HMult+relin+Rescale, then n Rotate), for each n."""
+    out = []
+    for n in sizes:
+        def step():
+            ctx.hmult_batch(w.a[:n], w.b[:n], w.levels[:n], w.rlk, w.r[:n], rescale=True)
+            ctx.rotate_batch(w.r[:n], [l - 1 for l in w.levels[:n]], w.rot_keys[:n], w.kidx[:n], w.o[:n])
+        step()
+        ctx.sync()
+        e0, e1 = ev.event(), ev.event()
+        ev.record(e0)
+        for _ in range(reps):
+            step()
+        ev.record(e1)
+        ms = ev.elapsed_ms(e0, e1) / reps
+        out.append({"ciphertexts": n, "ms_per_step": round(ms, 3), "ops_per_s": round(2 * n / (ms * 1e-3), 1)})
+    return out
+
+
 # ----------------------------------------------------------------- single primitives
 def primitive_inputs():
     """Seeded inputs of the primitive timings, shared by the GPU leg and the
@@ -778,6 +799,8 @@ def main():
     }
     out["config"]["hoisted_rotations"] = hoisted_leg(prm, ctx, w, ev)
     print("bench: hoisted %s" % out["config"]["hoisted_rotations"], file=sys.stderr, flush=True)
+    out["config"]["batch_sizes"] = batch_size_leg(ctx, w, ev)
+    print("bench: batch sizes %s" % out["config"]["batch_sizes"], file=sys.stderr, flush=True)
     out["config"]["primitives"] = primitives_leg(ev)
     print("bench: primitives %s" % out["config"]["primitives"], file=sys.stderr, flush=True)
     if not (args.no_serve and args.no_boot):
