@@ -1642,8 +1642,9 @@ extern "C" int fides_autotune(fides_context* c, uint32_t n, const uint32_t* l, u
   std::vector<u64> h;
   fill_residues(h, c, 2 * c->dnum * LK, LK, seed);
   RCHK(fides_key_upload(c, h.data(), &B.key));
-  // one random ciphertext per distinct limb count, copied to every input of that count
-  fill_residues(h, c, 2 * c->L, c->L, seed);  // component c, limb m at (c * L + m) -> prime m
+  // one random ciphertext at L limbs (component c, limb m at c * L + m, prime m); input i
+  // takes the first l[i] limbs of each component
+  fill_residues(h, c, 2 * c->L, c->L, seed);
   for (u32 i = 0; i < n; ++i) {
     u64 *pi = nullptr, *po = nullptr;
     const size_t S = (size_t)l[i] * N;
