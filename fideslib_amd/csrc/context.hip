@@ -215,6 +215,7 @@ struct fides_context {
   int ring_next = 0;
   hipStream_t copy_stream = nullptr;  // task-list uploads, overlapped with the previous operation's kernels
   bool conv_split = false;            // every prime < 2^60: carry-free basis conversion (k_conv_batch)
+  bool wide = false;                  // some prime >= 2^61: at most 16 products per 128-bit accumulation
   Capture* cap = nullptr;             // graph capture in progress
   fides_tuning tune{{8, 16, 32}, {1, 1, 1}, {1, 1, 1}};  // launch shapes per size class (fides_tuning_set)
   std::vector<void*> allocs;
@@ -329,6 +330,11 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
     for (size_t j = 0; j < i; ++j)
       if (pr[j] == pr[i]) return fail(FIDES_ERR_ARG, "duplicate prime");
   }
+  // The inner product accumulates one 128-bit product per active digit (up to
+  // dnum); the column sums of Acc128 hold 16 products of values below 2^62
+  // (64 below 2^61).
+  const bool wide = std::any_of(pr.begin(), pr.end(), [](u64 q) { return q >= (1ull << 61); });
+  if (wide && P->dnum > 16) return fail(FIDES_ERR_UNSUPPORTED, "dnum > 16 needs every prime below 2^61");
   fides_context* c = new fides_context();
   c->logN = P->logN;
   c->N = 1u << P->logN;
@@ -338,6 +344,7 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   c->alpha = alpha;
   c->primes = pr;
   c->conv_split = std::all_of(pr.begin(), pr.end(), [](u64 q) { return q < (1ull << 60); });
+  c->wide = wide;
   const u32 NP = P->L + P->K, N = c->N, L = c->L, K = c->K;
   c->roots.resize(NP);
   for (u32 i = 0; i < NP; ++i) {
@@ -1287,6 +1294,9 @@ extern "C" void fides_key_destroy(fides_key* k) {
 // ======================================================================
 // ciphertext operations (single = batch of one)
 // ======================================================================
+// [a, a + na) and [b, b + nb) share a word.
+static bool overlaps(const u64* a, size_t na, const u64* b, size_t nb) { return a < b + nb && b < a + na; }
+
 static int check_key(fides_context* c, const fides_key* key, u32 l) {
   if (!key) return fail(FIDES_ERR_ARG, "null key");
   if (key->ctx != c) return fail(FIDES_ERR_ARG, "key belongs to another context");
@@ -1326,6 +1336,23 @@ static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* 
     RCHK(check_key(c, rlk, lv[i]));
     if (!a[i] || (!square && !b[i]) || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     if (rescale && lv[i] < 2) return fail(FIDES_ERR_ARG, "rescale needs l >= 2");
+  }
+  // Without rescale the tensor product writes out[i] in the launch that reads
+  // every job's inputs: out[i] may only coincide with its own a[i] or b[i]
+  // (each element is read before the same thread writes it).  With rescale
+  // the outputs are written after the last read of any input.  Outputs never
+  // overlap each other.
+  for (u32 i = 0; i < n; ++i) {
+    const size_t so = (size_t)2 * (rescale ? lv[i] - 1 : lv[i]) * N;
+    for (u32 j = 0; j < n; ++j) {
+      const size_t sj = (size_t)2 * lv[j] * N, sjo = (size_t)2 * (rescale ? lv[j] - 1 : lv[j]) * N;
+      if (j < i && overlaps(out[i], so, out[j], sjo)) return fail(FIDES_ERR_ARG, "outputs overlap");
+      if (rescale) continue;
+      for (const u64* in : {a[j], square ? nullptr : b[j]}) {
+        if (!in || (j == i && in == out[i])) continue;
+        if (overlaps(out[i], so, in, sj)) return fail(FIDES_ERR_ARG, "output overlaps an input of the batch");
+      }
+    }
   }
   // arena: [key-switch scratch | per job: c1d1 (l) | tensor product [2][l] if rescaling]
   size_t ksw = 0, extra = 0;
@@ -1456,8 +1483,15 @@ static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32
   for (u32 i = 0; i < n; ++i) {
     RCHK(check_key(c, keys[i], lv[i]));
     if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
-    if (in[i] == out[i]) return fail(FIDES_ERR_ARG, "rotate output must not alias its input");
     if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
+    // the ModDown store scatters out[i] while other workgroups still read
+    // every job's c0: no output may overlap any input or another output
+    const size_t si = (size_t)2 * lv[i] * N;
+    for (u32 j = 0; j < n; ++j) {
+      if (overlaps(out[i], si, in[j], (size_t)2 * lv[j] * N))
+        return fail(FIDES_ERR_ARG, "rotate output overlaps an input of the batch");
+      if (j < i && overlaps(out[i], si, out[j], (size_t)2 * lv[j] * N)) return fail(FIDES_ERR_ARG, "outputs overlap");
+    }
     for (u32 j = 0; j < i; ++j)
       if (in[j] == in[i] && lv[j] == lv[i] && share[j] < 0) {
         share[i] = (int)j;
@@ -1995,6 +2029,7 @@ extern "C" int fides_lincomb_batch(fides_context* c, uint32_t n, const uint32_t*
   size_t o = 0;
   for (u32 i = 0; i < n; ++i) {
     if (cnt[i] == 0 || cnt[i] > LINCOMB_MAXT) return fail(FIDES_ERR_ARG, "term count must be 1..64");
+    if (c->wide && cnt[i] > 16) return fail(FIDES_ERR_ARG, "term count must be 1..16 when a prime is >= 2^61");
     if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
     if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
     const uint64_t* a = add_res ? add_res[i] : nullptr;

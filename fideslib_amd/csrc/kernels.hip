@@ -104,12 +104,14 @@ __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__
     PrimeC P;
     P.q = prm[d->dst_pid[t]].q;
     P.qneg_inv = prm[d->dst_pid[t]].qneg_inv;
+    P.one_sh = prm[d->dst_pid[t]].one_sh;
     Acc128 acc;
 #pragma unroll
     for (int i = 0; i < NS; ++i) acc_mac(acc, xs[i], d->mat[t][i]);
     u64 hi, lo;
     acc_value(acc, hi, lo);
-    out[((u64)d->dst_off[t] << logN) + k] = mont_redc(hi, lo, P);
+    // up to 16 products of values below 2^62: the high word can exceed 4q
+    out[((u64)d->dst_off[t] << logN) + k] = mont_redc_any(hi, lo, P);
   }
 }
 
@@ -133,9 +135,9 @@ __global__ __launch_bounds__(256) void k_dot_batch(const DotTask* __restrict__ t
   }
   u64 h, l;
   acc_value(a0, h, l);
-  t.o0[k] = mont_redc(h, l, P);
+  t.o0[k] = mont_redc_any(h, l, P);
   acc_value(a1, h, l);
-  t.o1[k] = mont_redc(h, l, P);
+  t.o1[k] = mont_redc_any(h, l, P);
 }
 
 // ---------------------------------------------------------------- key conversion
@@ -207,9 +209,9 @@ __global__ __launch_bounds__(256) void k_ptdot_batch(const PtDotTask* __restrict
   }
   u64 h, l;
   acc_value(a0, h, l);
-  t.out0[k] = mont_redc(h, l, P);
+  t.out0[k] = mont_redc_any(h, l, P);
   acc_value(a1, h, l);
-  t.out1[k] = mont_redc(h, l, P);
+  t.out1[k] = mont_redc_any(h, l, P);
 }
 
 // ---------------------------------------------------------------- linear combination
@@ -217,8 +219,9 @@ __global__ __launch_bounds__(256) void k_ptdot_batch(const PtDotTask* __restrict
 // (the Chebyshev-series combination; the reference evaluates it with
 // scalar multiplications and additions, src/CKKS/ApproxModEval.cu).  The
 // constants are uniform per task and arrive through scalar loads; one
-// column accumulation and one Montgomery reduction per coefficient
-// (x, m < q < 2^60, 64 terms: the sum stays below 4 q 2^64).
+// column accumulation and one Montgomery reduction per coefficient.  The
+// column sums stay exact for 64 terms below 2^61 and for 16 terms below 2^62
+// (fides_lincomb_batch refuses more); the high word is reduced mod q first.
 __global__ __launch_bounds__(256) void k_lincomb_batch(const LcTask* __restrict__ tasks_,
                                                        const LcTerm* __restrict__ terms_,
                                                        const PrimeC* __restrict__ prm_, u32 logN) {
@@ -229,12 +232,13 @@ __global__ __launch_bounds__(256) void k_lincomb_batch(const LcTask* __restrict_
   PrimeC P;
   P.q = prm[t->pid].q;
   P.qneg_inv = prm[t->pid].qneg_inv;
+  P.one_sh = prm[t->pid].one_sh;
   const u32 first = t->first, cnt = t->cnt;
   Acc128 acc;
   for (u32 j = 0; j < cnt; ++j) acc_mac(acc, terms[first + j].x[k], terms[first + j].m);
   u64 hi, lo;
   acc_value(acc, hi, lo);
-  t->out[k] = add_mod(mont_redc(hi, lo, P), t->add, P.q);
+  t->out[k] = add_mod(mont_redc_any(hi, lo, P), t->add, P.q);
 }
 
 // ---------------------------------------------------------------- launchers

@@ -145,7 +145,8 @@ __device__ __forceinline__ void mac128(u64& hi, u64& lo, u64 a, u64 b) {
   hi = hi + h + c;
 }
 
-// Column accumulator for sums of up to 16 products x * m with x, m < 2^62:
+// Column accumulator for sums of up to 16 products x * m with x, m < 2^62
+// (or up to 64 with x, m < 2^61: H then stays below 64 * 2^58):
 //   T = L + 2^32 M + 2^64 (H + cL) + 2^96 cM
 // where L = sum x0 m0, M = sum (x0 m1 + x1 m0) and H = sum x1 m1 (32-bit
 // halves), each kept modulo 2^64 with the carries out of L and M counted
@@ -206,6 +207,15 @@ __device__ __forceinline__ u64 mont_redc(u64 hi, u64 lo, const PrimeC& P) {
 __device__ __forceinline__ u64 redc_once(u64 hi, u64 lo, const PrimeC& P) {
   const u64 m = lo * P.qneg_inv;
   return csub(hi + mulhi64(m, P.q) + (lo != 0), P.q);
+}
+// Montgomery reduction of any T = (hi, lo) < 2^128: the high word is first
+// reduced mod q (a Shoup product by 1, exact for any 64-bit value), so the
+// reduced T' = (hi mod q) R + lo is below q R and one conditional
+// subtraction finishes.  Used by the accumulations whose high word can reach
+// 4q and more (up to 16 products with q near 2^62, or 64 products below
+// 2^61), where the sum hi + floor(m q / R) of mont_redc would overflow.
+__device__ __forceinline__ u64 mont_redc_any(u64 hi, u64 lo, const PrimeC& P) {
+  return redc_once(shoup(hi, 1, P.one_sh, P.q), lo, P);
 }
 // y R mod q for y < q: REDC(y (R^2 mod q)).
 __device__ __forceinline__ u64 to_mont(u64 y, const PrimeC& P) {

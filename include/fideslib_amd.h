@@ -166,15 +166,19 @@ int fides_rotate(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fid
  * ModDown, Rescale) is one launch over the limbs of all n ciphertexts.
  * With rescale != 0, out[i] is [2][l[i]-1] (HMult + relin + Rescale), else
  * [2][l[i]].  Results are identical to n calls of fides_hmult (+
- * fides_rescale).  The reference batches only inside linear transforms
- * (src/CKKS/LimbPartitionBatch.cu:20-616); this entry point has no single
- * reference counterpart. */
+ * fides_rescale).  Outputs must not overlap each other; without rescale an
+ * output may coincide with its own a[i] or b[i] but must not overlap any
+ * other input of the batch (refused with FIDES_ERR_ARG).  The reference
+ * batches only inside linear transforms (src/CKKS/LimbPartitionBatch.cu:20-616);
+ * this entry point has no single reference counterpart. */
 int fides_hmult_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint64_t* const* d_b,
                       const uint32_t* l, const fides_key* rlk, int rescale, uint64_t* const* d_out, void* stream);
 
 /* n rotations/automorphisms, ciphertext i with its own key and index.
  * Identical to n calls of fides_rotate.  Entries that rotate the same input
- * (same pointer and level) share one ModUp, as fides_rotate_hoisted does. */
+ * (same pointer and level) share one ModUp, as fides_rotate_hoisted does.
+ * No output may overlap any input of the batch or another output (refused
+ * with FIDES_ERR_ARG). */
 int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
                        const fides_key* const* keys, const uint32_t* auto_index, uint64_t* const* d_out,
                        void* stream);

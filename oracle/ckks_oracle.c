@@ -14,8 +14,9 @@
  * is the only concession to run time).
  *
  * Pinning (what fixes each function independently of this file) is listed
- * in the tests under tests/test_oracle_*.py; every function here is checked
- * there against closed forms / Python big-integer arithmetic / brute force.
+ * in tests/test_oracle.py; every function here is checked there against
+ * golden vectors from an independent Python big-integer implementation
+ * (tests/golden/), closed forms and brute force.
  *
  * Conventions
  *   - A limb is N = 2^logN contiguous uint64 residues in [0, q).

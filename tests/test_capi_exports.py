@@ -74,3 +74,17 @@ def test_tuning_null_arguments_and_dict_round_trip(lib):
     assert capi.Tuning.from_dict(d).as_dict() == d
     with pytest.raises(capi.FidesError):
         capi.Tuning.from_dict(dict(d, conv_chunk=[8, 16]))
+
+
+def test_context_refuses_many_digits_with_wide_primes(lib):
+    """More than 16 digits need every prime below 2^61 (the 128-bit column
+    sums of the inner product hold 16 products of 62-bit values); refused
+    before any device call."""
+    from fideslib_amd import params
+    q = np.ascontiguousarray(params.ntt_primes(62, 17, 10), np.uint64)
+    p = np.ascontiguousarray(params.ntt_primes(62, 2, 10, exclude=list(q)), np.uint64)
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    h = ctypes.c_void_p()
+    prm = capi.Params(10, 17, 2, 17, q.ctypes.data_as(u64p), p.ctypes.data_as(u64p), None)
+    assert lib.fides_context_create(ctypes.byref(prm), ctypes.byref(h)) == -3
+    assert b"dnum" in lib.fides_last_error()

@@ -328,6 +328,42 @@ def test_key_switch_wide_primes(bits):
     assert np.array_equal(o.download((2, L, N)), octx.rotate(x, L, key, 5))
 
 
+def test_lincomb_wide_primes_extreme_terms():
+    """16 terms whose products are all (q-1)^2 under 62-bit primes just below
+    2^62: the 128-bit column sum comes within 2^64 of overflowing, so the
+    Montgomery reduction must first reduce its high word (a plain
+    hi + floor(m q / 2^64) wraps).  17 terms are refused for such primes;
+    beyond 16 digits the context refuses the parameters."""
+    logN = 12
+    q = params.ntt_primes(62, 4, logN)
+    p = params.ntt_primes(62, 2, logN, exclude=q)
+    assert min(q) > (1 << 62) - (1 << 40)
+    say("lincomb 62-bit extremes")
+    N, l = 1 << logN, 4
+    octx = O.Context(logN, q, p, 2)
+    gctx = capi.Context(logN, q, p, 2)
+    x = np.stack([np.full(N, qi - 1, np.uint64) for qi in q] * 2)  # [2 l][N]
+    dx = capi.DeviceBuffer.from_array(x)
+    # constant residues m_j with m_j 2^64 = q_j - 1 (mod q_j): the kernel's
+    # Montgomery-form constants are then q_j - 1, the largest possible
+    res = [(qi - 1) * pow(pow(2, 64, qi), -1, qi) % qi for qi in q]
+    Q = 1
+    for qi in q:
+        Q *= qi
+    M = sum(r * (Q // qi) * pow(Q // qi, -1, qi) for r, qi in zip(res, q)) % Q
+    out = capi.DeviceBuffer(2 * l * N)
+    gctx.lincomb_batch([[(dx, l, M)] * 16], [l], [None], [out])
+    pid = list(range(l)) * 2
+    c = np.stack([np.full(N, M % q[j], np.uint64) for j in range(l)] * 2)
+    t = octx.mul_limbs(x, c, pid)
+    want = t
+    for _ in range(15):
+        want = octx.add_limbs(want, t, pid)
+    assert np.array_equal(out.download((2 * l, N)), want)
+    with pytest.raises(capi.FidesError):
+        gctx.lincomb_batch([[(dx, l, M)] * 17], [l], [None], [out])
+
+
 def test_cfg1_square_and_lower_levels(cfg1):
     prm, octx, gctx, keys = cfg1
     N = prm.N
@@ -483,6 +519,45 @@ def test_cfg3_batch_mixed_levels_bit_exact(cfg3):
         gctx.hmult(A[i], B[i], l, g_rlk, m)
         gctx.rescale(m, l, r)
         assert np.array_equal(r.download((2, l - 1, N)), OB[i].download((2, l - 1, N))), (i, l)
+
+
+def test_cfg3_rotate_batch_real_keys_mixed_levels(cfg3):
+    """The headline rotation shape: 16 ciphertexts at limb counts U{6..30}
+    (l = 30 and l = 6 included), each with its own real rotation key (index
+    5^r mod 2N, r ~ U[1, 2^15)), in one fides_rotate_batch call.  Four of them
+    (l = 30, l = 6 and two others) equal the oracle; every one equals a single
+    fides_rotate; the l = 30 one is a real encryption and decrypts to the
+    rotated message."""
+    prm, octx, gctx, keys = cfg3
+    N = prm.N
+    rng = np.random.default_rng(2027)
+    n = 16
+    levels = [int(x) for x in rng.integers(6, prm.L + 1, n)]
+    levels[0], levels[1] = 30, 6
+    rots = [int(x) for x in rng.integers(1, 1 << 15, n)]
+    ks = [pow(5, r, 2 * N) for r in rots]
+    say("cfg3 rotate batch: levels %s" % levels)
+    hkeys = [keys.rotation_key(k) for k in ks]
+    gk = [gctx.upload_key(h) for h in hkeys]
+    z, (c0, _) = _enc_pair(prm, keys, 303, 30)
+    cts = [c0] + [np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]) for l in levels[1:]]
+    IN = [capi.DeviceBuffer.from_array(c) for c in cts]
+    OUT = [capi.DeviceBuffer(2 * l * N) for l in levels]
+    gctx.rotate_batch(IN, levels, gk, ks, OUT)
+    got = [o.download((2, l, N)) for o, l in zip(OUT, levels)]
+    for i in (0, 1, 5, 11):
+        say("  oracle %d (l=%d)" % (i, levels[i]))
+        assert np.array_equal(got[i], octx.rotate(cts[i], levels[i], hkeys[i], ks[i])), (i, levels[i])
+    for i, l in enumerate(levels):
+        single = capi.DeviceBuffer(2 * l * N)
+        gctx.rotate(IN[i], l, gk[i], ks[i], single)
+        assert np.array_equal(single.download((2, l, N)), got[i]), (i, l)
+    dec = C.decode(keys.decrypt_coef(got[0], 30), 2.0 ** prm.scale_bits, N)
+    assert C.precision_bits(dec, np.roll(z[0], -rots[0])) > 30
+    # an output may not overlap any input of the batch (the ModDown store
+    # scatters while other jobs still read their c0)
+    with pytest.raises(capi.FidesError):
+        gctx.rotate_batch(IN[:2], levels[:2], gk[:2], ks[:2], [OUT[0], IN[0]])
 
 
 # ------------------------------------------------------- element-wise batch ops, limb drop, batched rescale
