@@ -89,6 +89,7 @@ SIGNATURES = {
     "fides_key_upload": (_int, [_vp, _u64p, ctypes.POINTER(_vp)]),
     "fides_key_destroy": (None, [_vp]),
     "fides_keyswitch": (_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
+    "fides_keyswitch_noalloc": (_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp]),
     "fides_hmult": (_int, [_vp, _vp, _vp, _u32, _vp, _vp, _vp]),
     "fides_square": (_int, [_vp, _vp, _u32, _vp, _vp, _vp]),
     "fides_rescale": (_int, [_vp, _vp, _u32, _vp, _vp]),
@@ -246,6 +247,11 @@ class Context:
 
     def keyswitch(self, inp, l, key, out0, out1, stream=None):
         check(load().fides_keyswitch(self.h, inp.ptr, l, key.h, out0.ptr, out1.ptr, _sptr(stream)), "fides_keyswitch")
+
+    def keyswitch_noalloc(self, inp, l, key, out0, out1, stream=None):
+        """fides_keyswitch with the library's static device scratch (no allocation)."""
+        check(load().fides_keyswitch_noalloc(self.h, inp.ptr, l, key.h, out0.ptr, out1.ptr, _sptr(stream)),
+              "fides_keyswitch_noalloc")
 
     # ---- ciphertext ops
     def hmult(self, a, b, l, rlk, out, stream=None):

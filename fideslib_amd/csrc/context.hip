@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -196,6 +197,7 @@ struct Capture {
 
 struct fides_context {
   u32 logN, N, L, K, dnum, alpha;
+  int device = 0;  // HIP device current at creation: tables, keys and scratch live there
   std::vector<u64> primes, roots;
   std::vector<PrimeC> hprm;
   // device tables
@@ -223,6 +225,31 @@ struct fides_context {
   u64* limb(u64* base, u64 i) const { return base + (i << logN); }
   const u64* limb(const u64* base, u64 i) const { return base + (i << logN); }
 };
+
+// ----------------------------------------------------------------------
+// Device scratch of the allocation-free key switch (fides_keyswitch_noalloc):
+// a file-scope array that every GPU running this library holds, at its own
+// address, until the process ends.  Host code never uses the array's name as
+// an address: each device's address comes from hipGetSymbolAddress, taken
+// once per device (with that device current) when a context is created there.
+constexpr size_t NOALLOC_SCRATCH_WORDS = (size_t)20 << 20;  // 160 MiB
+static __device__ __align__(256) u64 scratch[NOALLOC_SCRATCH_WORDS];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static u64* g_scratch_ptr[MAX_DEVICES];
+
+// Address of `scratch` on the current device `dev`, resolved on first use.
+static int scratch_init(int dev) {
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(FIDES_ERR_UNSUPPORTED, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return FIDES_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p)
+    return fail(FIDES_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = (u64*)p;
+  return FIDES_OK;
+}
 
 static int dalloc(fides_context* c, void** p, size_t bytes) {
   hipError_t e = hipMalloc(p, bytes);
@@ -336,6 +363,14 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   const bool wide = std::any_of(pr.begin(), pr.end(), [](u64 q) { return q >= (1ull << 61); });
   if (wide && P->dnum > 16) return fail(FIDES_ERR_UNSUPPORTED, "dnum > 16 needs every prime below 2^61");
   fides_context* c = new fides_context();
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return fail(FIDES_ERR_HIP, "hipGetDevice");
+  }
+  if (const int rc = scratch_init(c->device)) {
+    delete c;
+    return rc;
+  }
   c->logN = P->logN;
   c->N = 1u << P->logN;
   c->L = P->L;
@@ -678,7 +713,8 @@ struct Blob {
 
 // Upload a blob through the next ring slot; returns its device base.
 // During a graph capture the blob goes to the graph's task pool instead.
-static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char** dev, int* slot_out) {
+static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char** dev, int* slot_out,
+                       bool may_grow = true) {
   if (c->cap) {
     Capture& g = *c->cap;
     if (st != g.st) return fail(FIDES_ERR_ARG, "operation on another stream during a graph capture");
@@ -697,6 +733,8 @@ static int upload_blob(fides_context* c, const Blob& blob, hipStream_t st, char*
   c->ring_next = (c->ring_next + 1) % RING;
   if (s.used) HIPCHK(hipEventSynchronize(s.done));  // slot's previous operation has finished
   const size_t need = std::max<size_t>(blob.b.size(), 1);
+  if (need > s.cap && !may_grow)
+    return fail(FIDES_ERR_ARG, "task list of " + std::to_string(need) + " bytes exceeds the preallocated task slot");
   if (need > s.cap) {
     if (s.host) HIPCHK(hipHostFree(s.host));
     if (s.dev) HIPCHK(hipFree(s.dev));
@@ -1326,6 +1364,33 @@ extern "C" int fides_keyswitch(fides_context* c, const uint64_t* d, uint32_t l, 
   RCHK(check_key(c, key, l));
   if (!d || !o0 || !o1) return fail(FIDES_ERR_ARG, "null buffer");
   return run_keyswitch(c, {KsJob{d, l, key, o0, o1, nullptr, nullptr}}, (hipStream_t)stream);
+}
+
+// The same key switch with its scratch in the file-scope device array and
+// its task list in a preallocated task slot: no device or pinned allocation.
+extern "C" int fides_keyswitch_noalloc(fides_context* c, const uint64_t* d, uint32_t l, const fides_key* key,
+                                       uint64_t* o0, uint64_t* o1, void* stream) {
+  if (!c) return fail(FIDES_ERR_ARG, "null context");
+  RCHK(check_key(c, key, l));
+  if (!d || !o0 || !o1) return fail(FIDES_ERR_ARG, "null buffer");
+  const size_t words = ks_words(c, l);
+  if (words > NOALLOC_SCRATCH_WORDS)
+    return fail(FIDES_ERR_ARG, "key switch needs " + std::to_string(words * 8) +
+                                   " bytes of scratch; the allocation-free entry point holds " +
+                                   std::to_string(NOALLOC_SCRATCH_WORDS * 8));
+  RCHK(scratch_init(c->device));
+  u64* scr = g_scratch_ptr[c->device];
+  hipStream_t st = (hipStream_t)stream;
+  Blob blob;
+  KsLists K_;
+  ks_build(c, {KsJob{d, l, key, o0, o1, nullptr, nullptr}}, scr, blob, K_);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot, false));
+  RCHK(ks_launch(c, dev, K_, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
 }
 
 // HMult (+ relinearize [+ rescale]) of n ciphertext pairs.

@@ -132,6 +132,19 @@ void fides_key_destroy(fides_key* key);
 int fides_keyswitch(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fides_key* key,
                     uint64_t* d_out0, uint64_t* d_out1, void* stream);
 
+/* Allocation-free variant of fides_keyswitch, with the same result bits.  No
+ * reference counterpart: it is for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  Its scratch is
+ * a file-scope device array of 160 MiB that every GPU running this library
+ * holds until the process ends (a key switch needs
+ * N (l + beta (l + K) + 2 (l + K) + 2 l + 2) words; larger requests return
+ * FIDES_ERR_ARG), its task list goes through a task slot preallocated by
+ * fides_context_create, and it calls no allocator.  All calls on one GPU share
+ * that array: calls that overlap in time on one GPU (other streams, other
+ * contexts, other threads) race and must be serialized by the caller. */
+int fides_keyswitch_noalloc(fides_context* ctx, const uint64_t* d_in, uint32_t l, const fides_key* key,
+                            uint64_t* d_out0, uint64_t* d_out1, void* stream);
+
 /* ---- ciphertext operations ([2][l][N] device buffers) ---------------- */
 
 /* HMult + relinearization (no rescale).  d_out may alias d_a.  Replaces
