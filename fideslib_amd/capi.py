@@ -106,6 +106,13 @@ SIGNATURES = {
     "fides_mod_raise_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_key_download": (_int, [_vp, _u64p]),
     "fides_rescale_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_rotate_hoisted_ext": (_int, [_vp, _vp, _u32, _u32, ctypes.POINTER(_vp), _u32p, _int,
+                                        ctypes.POINTER(_vp), _vp]),
+    "fides_ext_raise_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_ext_add_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, _int,
+                                   ctypes.POINTER(_vp), _vp]),
+    "fides_ext_moddown_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_vp), _vp]),
+    "fides_plaintext_embed": (_int, [_vp, _vp, _u32, _vp, _vp]),
     "fides_lin_batch": (_int, [_vp, _u32, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
                                ctypes.POINTER(_u64p), _u32p, ctypes.POINTER(_vp), _vp]),
     "fides_lincomb_batch": (_int, [_vp, _u32, _u32p, ctypes.POINTER(_vp), _u32p, ctypes.POINTER(_u64p),
@@ -311,6 +318,40 @@ class Context:
         lv = np.ascontiguousarray(levels, np.uint32)
         check(load().fides_ptdot_batch(self.h, n, cnt.ctypes.data_as(_u32p), C, P, lv.ctypes.data_as(_u32p), O,
                                        _sptr(stream)), "fides_ptdot_batch")
+
+    # -- extended basis Q_l u P: [2][l+K][N] buffers
+    def rotate_hoisted_ext(self, inp, level, keys, kidx, out, add_first=True, stream=None):
+        n = len(kidx)
+        K = (_vp * n)(*[None if k is None else k.h for k in keys])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        ki = np.ascontiguousarray(kidx, np.uint32)
+        check(load().fides_rotate_hoisted_ext(self.h, _ptr(inp), level, n, K, ki.ctypes.data_as(_u32p),
+                                              int(bool(add_first)), O, _sptr(stream)), "fides_rotate_hoisted_ext")
+
+    def _ext_unary(self, fn, inp, levels, out, stream, what):
+        n = len(levels)
+        I = (_vp * n)(*[_ptr(x) for x in inp])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(fn(self.h, n, I, lv.ctypes.data_as(_u32p), O, _sptr(stream)), what)
+
+    def ext_raise_batch(self, inp, levels, out, stream=None):
+        self._ext_unary(load().fides_ext_raise_batch, inp, levels, out, stream, "fides_ext_raise_batch")
+
+    def ext_moddown_batch(self, inp, levels, out, stream=None):
+        self._ext_unary(load().fides_ext_moddown_batch, inp, levels, out, stream, "fides_ext_moddown_batch")
+
+    def ext_add_batch(self, a, b, levels, out, subtract=False, stream=None):
+        n = len(levels)
+        A = (_vp * n)(*[_ptr(x) for x in a])
+        B = (_vp * n)(*[_ptr(x) for x in b])
+        O = (_vp * n)(*[_ptr(x) for x in out])
+        lv = np.ascontiguousarray(levels, np.uint32)
+        check(load().fides_ext_add_batch(self.h, n, A, B, lv.ctypes.data_as(_u32p), int(bool(subtract)), O,
+                                         _sptr(stream)), "fides_ext_add_batch")
+
+    def plaintext_embed(self, pt, level, out, stream=None):
+        check(load().fides_plaintext_embed(self.h, _ptr(pt), level, _ptr(out), _sptr(stream)), "fides_plaintext_embed")
 
     def mod_raise_batch(self, inp, l_out, out, stream=None):
         n = len(l_out)

@@ -209,6 +209,7 @@ struct fides_context {
   std::vector<ConvDesc> hmodup, hmoddown;
   // host copies of per-limb scalars
   std::vector<u64> pinv, pinv_sh;                    // [L]
+  std::vector<u64> pmod, pmod_sh;                    // [L] P mod q_i (the ext basis scales by P)
   std::vector<u64> qlinv, qlinv_sh, qlql, qlql_sh;   // [(L+1) * L], row l = dropping q_{l-1}
   // scratch arena (grows on demand) and task ring
   u64* arena = nullptr;
@@ -467,10 +468,14 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   // ---- ModDown and rescale scalars
   c->pinv.resize(L);
   c->pinv_sh.resize(L);
+  c->pmod.resize(L);
+  c->pmod_sh.resize(L);
   for (u32 i = 0; i < L; ++i) {
     const u64 qi = pr[i];
     u64 Pm = 1;
     for (u32 k = 0; k < K; ++k) Pm = mulm(Pm, pr[L + k] % qi, qi);
+    c->pmod[i] = Pm;
+    c->pmod_sh[i] = shoup_of(Pm, qi);
     c->pinv[i] = invm(Pm, qi);
     c->pinv_sh[i] = shoup_of(c->pinv[i], qi);
   }
@@ -838,6 +843,9 @@ struct KsJob {
                          // Rescale(acc + KeySwitch(d)); acc0/acc1 required
   int share = -1;        // index of an earlier job with the same d and l whose ModUp
                          // (decomposed, extended digits) this job reuses (hoisting)
+  bool ext = false;      // no ModDown: out0 receives [2][l+K] limbs in Q_l u P,
+                         // sigma_kidx(T0 + P acc0) and sigma_kidx(T1) (acc0 nullable)
+  u32 kidx = 1;          // automorphism of an ext job
 };
 
 // Arena words needed by the key switch of one job.
@@ -860,7 +868,9 @@ static size_t ks_words(const fides_context* c, u32 l, bool shared_modup = false)
 //   (fused Rescale jobs) top limb: acc + T P^-1 -> INTT -> - conv P^-1, then
 //   one NTT per remaining limb with PRO_MDRS / EPI_MDRS (see ntt.h).
 struct KsLists {
-  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, axpy1, intt_top, axpy2, ntt_mdrs;
+  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, axpy1, intt_top, axpy2, ntt_mdrs,
+      auto_ext;
+  u32 n_auto_ext = 0;
   // dot tasks grouped by digit count for the fused row pass: [start, count) per beta
   u32 dot_start[ROWDOT_MAXB + 1] = {}, dot_count[ROWDOT_MAXB + 1] = {};
   bool fused_dot = false;
@@ -883,6 +893,7 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   std::vector<ConvTask> conv_up[CONV_MAXS + 1], conv_down;
   std::vector<DotTask> dot;
   std::vector<AxpyTask> axpy1, axpy2;
+  std::vector<AutoTask> auto_ext;
   u64* w = scratch;
   std::vector<u64*> ext_of(jobs.size(), nullptr);
   size_t up_targets = 0, down_targets = 0;
@@ -936,9 +947,26 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
       d.jown = t < l ? t / c->alpha : 0xffffffffu;
       d.pid = pid;
       d.estride = ext_stride;
+      d.ext = J.ext ? 1u : 0u;
+      d.pad = 0;
       dot.push_back(d);
     }
     u64* Tp[2] = {T0, T1};
+    if (J.ext) {  // Q_l u P result, permuted, with P c0 added to component 0
+      for (u32 p = 0; p < 2; ++p)
+        for (u32 t = 0; t < l + K; ++t) {
+          AutoTask a;
+          a.in = c->limb(Tp[p], t);
+          a.out = c->limb(J.out0, (size_t)p * (l + K) + t);
+          a.add = (p == 0 && J.acc0 && t < l) ? c->limb(J.acc0, t) : nullptr;
+          a.c = t < l ? c->pmod[t] : 0;
+          a.csh = t < l ? c->pmod_sh[t] : 0;
+          a.kidx = J.kidx;
+          a.pid = t < l ? t : L + (t - l);
+          auto_ext.push_back(a);
+        }
+      continue;
+    }
     u64* outp[2] = {J.out0, J.out1};
     const u64* accp[2] = {J.acc0, J.acc1};
     const ConvDesc& md = c->hmoddown[l];
@@ -1046,6 +1074,8 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   K_.n_axpy2 = (u32)axpy2.size();
   K_.ntt_mdrs = blob.add(ntt_mdrs);
   K_.n_ntt_mdrs = (u32)ntt_mdrs.size();
+  K_.auto_ext = blob.add(auto_ext);
+  K_.n_auto_ext = (u32)auto_ext.size();
 }
 
 static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t st) {
@@ -1092,6 +1122,9 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
     fides_launch_axpy_batch(at<AxpyTask>(dev, K_.axpy2), K_.n_axpy2, c->prm, c->logN, st);
     RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_mdrs), K_.n_ntt_mdrs, PRO_MDRS, EPI_MDRS, st));
   }
+  for (u32 o = 0; o < K_.n_auto_ext; o += 65535)
+    fides_launch_automorph_batch(at<AutoTask>(dev, K_.auto_ext) + o, std::min<u32>(65535, K_.n_auto_ext - o), c->prm,
+                                 c->logN, st);
   return FIDES_OK;
 }
 
@@ -1523,13 +1556,13 @@ extern "C" int fides_automorph(fides_context* c, const uint64_t* in, uint32_t n,
   hipStream_t st = (hipStream_t)stream;
   Blob blob;
   std::vector<AutoTask> at_;
-  for (u32 t = 0; t < n; ++t) at_.push_back(AutoTask{c->limb(in, t), c->limb(out, t), k, 0});
+  for (u32 t = 0; t < n; ++t) at_.push_back(AutoTask{c->limb(in, t), c->limb(out, t), nullptr, 0, 0, k, 0});
   const size_t o = blob.add(at_);
   char* dev;
   int slot;
   RCHK(upload_blob(c, blob, st, &dev, &slot));
   for (u32 x = 0; x < n; x += 65535)
-    fides_launch_automorph_batch(at<AutoTask>(dev, o) + x, std::min<u32>(65535, n - x), c->logN, st);
+    fides_launch_automorph_batch(at<AutoTask>(dev, o) + x, std::min<u32>(65535, n - x), c->prm, c->logN, st);
   RCHK(finish_slot(c, slot, st));
   HIPCHK(hipGetLastError());
   return FIDES_OK;
@@ -1647,6 +1680,182 @@ extern "C" int fides_rotate_batch(fides_context* c, uint32_t n, const uint64_t* 
   if (!c || (n && (!in || !l || !keys || !k || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return rotate_batch(c, n, in, l, keys, k, out, (hipStream_t)stream);
+}
+
+// ======================================================================
+// extended basis Q_l u P (rotations whose ModDown is hoisted)
+// ======================================================================
+// An ext ciphertext is [2][l+K][N]: limbs 0..l-1 under q_0..q_{l-1}, then the
+// K special primes; it holds P times a ciphertext (OpenFHE's
+// EvalFastRotationExt / the reference's Ciphertext::rotate_hoisted with
+// ext = true, src/CKKS/Ciphertext.cpp:940-1114).  Sums of ext ciphertexts
+// take one ModDown (fides_ext_moddown_batch) instead of one per rotation.
+static u32 ext_pid(const fides_context* c, u32 l, u32 t) { return t < l ? t : c->L + (t - l); }
+static int run_lin(fides_context* c, const std::vector<LinTask>& tasks, hipStream_t st);
+
+extern "C" int fides_rotate_hoisted_ext(fides_context* c, const uint64_t* in, uint32_t l, uint32_t n,
+                                        const fides_key* const* keys, const uint32_t* kidx, int add_first,
+                                        uint64_t* const* out, void* stream) {
+  if (!c || !in || (n && (!keys || !kidx || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+  hipStream_t st = (hipStream_t)stream;
+  const u32 N = c->N, K = c->K;
+  const size_t S = (size_t)l * N, SE = (size_t)2 * (l + K) * N;
+  for (u32 i = 0; i < n; ++i) {
+    if (!out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if ((kidx[i] & 1) == 0 || kidx[i] >= 2 * N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
+    if (overlaps(out[i], SE, in, 2 * S)) return fail(FIDES_ERR_ARG, "rotate output must not overlap its input");
+    for (u32 j = 0; j < i; ++j)
+      if (overlaps(out[i], SE, out[j], SE)) return fail(FIDES_ERR_ARG, "outputs overlap");
+    if (kidx[i] != 1) RCHK(check_key(c, keys[i], l));
+  }
+  std::vector<LinTask> ident;  // index 1: (P c0 or 0, P c1)
+  std::vector<KsJob> jobs;
+  size_t ksw = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (kidx[i] == 1) {
+      for (u32 p = 0; p < 2; ++p)
+        for (u32 t = 0; t < l + K; ++t) {
+          LinTask x;
+          memset(&x, 0, sizeof(x));
+          x.a = c->limb(in, (size_t)p * l + (t < l ? t : 0));
+          x.out = c->limb(out[i], (size_t)p * (l + K) + t);
+          x.pid = ext_pid(c, l, t);
+          x.flags = LIN_MUL;
+          x.m = (t < l && (p == 1 || add_first)) ? c->pmod[t] : 0;
+          x.msh = (t < l && (p == 1 || add_first)) ? c->pmod_sh[t] : 0;
+          ident.push_back(x);
+        }
+      continue;
+    }
+    KsJob J{in + S, l, keys[i], out[i], nullptr, add_first ? in : nullptr, nullptr};
+    J.ext = true;
+    J.kidx = kidx[i];
+    J.share = jobs.empty() ? -1 : 0;
+    ksw += ks_words(c, l, !jobs.empty());
+    jobs.push_back(J);
+  }
+  if (!ident.empty()) RCHK(run_lin(c, ident, st));
+  if (jobs.empty()) return FIDES_OK;
+  RCHK(ensure_arena(c, ksw, st));
+  Blob blob;
+  KsLists K_;
+  ks_build(c, jobs, c->arena, blob, K_);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(ks_launch(c, dev, K_, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
+}
+
+extern "C" int fides_ext_raise_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
+                                     uint64_t* const* out, void* stream) {
+  if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  const u32 K = c->K;
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    if (overlaps(out[i], (size_t)2 * (l[i] + K) * c->N, in[i], (size_t)2 * l[i] * c->N))
+      return fail(FIDES_ERR_ARG, "output must not overlap its input");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 t = 0; t < l[i] + K; ++t) {
+        LinTask x;
+        memset(&x, 0, sizeof(x));
+        x.a = c->limb(in[i], (size_t)p * l[i] + (t < l[i] ? t : 0));
+        x.out = c->limb(out[i], (size_t)p * (l[i] + K) + t);
+        x.pid = ext_pid(c, l[i], t);
+        x.flags = LIN_MUL;
+        x.m = t < l[i] ? c->pmod[t] : 0;  // P = 0 mod every special prime
+        x.msh = t < l[i] ? c->pmod_sh[t] : 0;
+        tasks.push_back(x);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+extern "C" int fides_ext_add_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
+                                   const uint32_t* l, int subtract, uint64_t* const* out, void* stream) {
+  if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  const u32 K = c->K;
+  std::vector<LinTask> tasks;
+  for (u32 i = 0; i < n; ++i) {
+    if (!a[i] || !b[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+    for (u32 p = 0; p < 2; ++p)
+      for (u32 t = 0; t < l[i] + K; ++t) {
+        const size_t off = (size_t)p * (l[i] + K) + t;
+        LinTask x;
+        memset(&x, 0, sizeof(x));
+        x.a = c->limb(a[i], off);
+        x.b = c->limb(b[i], off);
+        x.out = c->limb(out[i], off);
+        x.pid = ext_pid(c, l[i], t);
+        x.flags = LIN_B | (subtract ? LIN_SUB : 0);
+        tasks.push_back(x);
+      }
+  }
+  return run_lin(c, tasks, (hipStream_t)stream);
+}
+
+// ModDown of both components (src/CKKS/Conv.cu:26-125 ModDown2 + the
+// NTT_MODDOWN epilogue, as in fides_moddown) for n ext ciphertexts.
+extern "C" int fides_ext_moddown_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
+                                       uint64_t* const* out, void* stream) {
+  if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
+  if (n == 0) return FIDES_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const u32 N = c->N, K = c->K, L = c->L;
+  size_t words = 0;
+  for (u32 i = 0; i < n; ++i) {
+    if (!in[i] || !out[i]) return fail(FIDES_ERR_ARG, "null buffer");
+    if (l[i] == 0 || l[i] > L) return fail(FIDES_ERR_ARG, "level out of range");
+    if (overlaps(out[i], (size_t)2 * l[i] * N, in[i], (size_t)2 * (l[i] + K) * N))
+      return fail(FIDES_ERR_ARG, "output must not overlap its input");
+    words += (size_t)2 * (K + l[i]) * N;
+  }
+  RCHK(ensure_arena(c, words, st));
+  std::vector<NttTask> intt, ntt;
+  std::vector<ConvTask> conv;
+  u64* w = c->arena;
+  for (u32 i = 0; i < n; ++i)
+    for (u32 p = 0; p < 2; ++p) {
+      const u32 li = l[i];
+      const u64* src = in[i] + (size_t)p * (li + K) * N;
+      u64* sp = w;
+      u64* Y = w + (size_t)K * N;
+      w = Y + (size_t)li * N;
+      for (u32 k = 0; k < K; ++k) {
+        NttTask t = ntask(c->limb(sp, k), L + k);
+        t.src = c->limb(src, li + k);
+        intt.push_back(t);
+      }
+      const ConvDesc& md = c->hmoddown[li];
+      for (u32 t0 = 0; t0 < md.nt; t0 += CONV_TCHUNK)
+        conv.push_back(ConvTask{sp, Y, c->moddown + li, t0, std::min(md.nt, t0 + CONV_TCHUNK)});
+      for (u32 j = 0; j < li; ++j) {
+        NttTask t = ntask(c->limb(Y, j), j);
+        t.x = c->limb(src, j);
+        t.out = c->limb(out[i], (size_t)p * li + j);
+        t.c0 = c->pinv[j];
+        t.c0sh = c->pinv_sh[j];
+        ntt.push_back(t);
+      }
+    }
+  std::stable_sort(ntt.begin(), ntt.end(), [](const NttTask& a, const NttTask& b) { return a.pid < b.pid; });
+  Blob blob;
+  const size_t o_intt = blob.add(intt), o_conv = blob.add(conv), o_ntt = blob.add(ntt);
+  char* dev;
+  int slot;
+  RCHK(upload_blob(c, blob, st, &dev, &slot));
+  RCHK(launch_intt_list(c, at<NttTask>(dev, o_intt), (u32)intt.size(), st));
+  RCHK(launch_conv_list(c, at<ConvTask>(dev, o_conv), (u32)conv.size(), K, st));
+  RCHK(launch_ntt_list(c, at<NttTask>(dev, o_ntt), (u32)ntt.size(), PRO_NONE, EPI_MODDOWN, st));
+  RCHK(finish_slot(c, slot, st));
+  HIPCHK(hipGetLastError());
+  return FIDES_OK;
 }
 
 // ======================================================================
@@ -1980,6 +2189,20 @@ extern "C" int fides_plaintext_prepare(fides_context* c, uint64_t* pt, uint32_t 
   fides_launch_to_mont(pt, c->prm, c->logN, l, l, st);
   HIPCHK(hipGetLastError());
   return FIDES_OK;
+}
+
+extern "C" int fides_plaintext_embed(fides_context* c, const uint64_t* pt, uint32_t l, uint64_t* ct, void* stream) {
+  if (!c || !pt || !ct) return fail(FIDES_ERR_ARG, "null argument");
+  if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
+  if (overlaps(ct, (size_t)2 * l * c->N, pt, (size_t)l * c->N)) return fail(FIDES_ERR_ARG, "output overlaps the plaintext");
+  std::vector<LinTask> tasks;
+  for (u32 p = 0; p < 2; ++p)
+    for (u32 j = 0; j < l; ++j) {
+      LinTask t = lin0(c->limb(pt, j), c->limb(ct, (size_t)p * l + j), j);
+      t.flags = p == 0 ? LIN_AMONT : LIN_MUL;  // c0 = pt in standard form, c1 = pt * 0
+      tasks.push_back(t);
+    }
+  return run_lin(c, tasks, (hipStream_t)stream);
 }
 
 extern "C" int fides_ptdot_batch(fides_context* c, uint32_t n, const uint32_t* cnt, const uint64_t* const* ct,

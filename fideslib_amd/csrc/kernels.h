@@ -42,6 +42,9 @@ struct DotTask {
   u32 jown;         // digit owning limb t (Q limbs), 0xffffffff for special limbs
   u32 pid;
   u32 estride;      // limbs between consecutive digits' ModUp outputs
+  u32 ext;          // 1: the result stays in Q u P (no ModDown follows): a special
+                    // limb is stored in evaluation form, not row-inverted
+  u32 pad;
 };
 
 // One limb of the tensor product (a0,a1) x (b0,b1).
@@ -99,7 +102,8 @@ struct LcTask {
 };
 constexpr u32 LINCOMB_MAXT = 64;
 
-// One limb of out = (a * m or a) (+ or - b) + add  (mod q_pid).
+// One limb of out = (a * m or a) (+ or - b) + add  (mod q_pid); with
+// LIN_AMONT, a is read in Montgomery form (a R mod q) and converted first.
 struct LinTask {
   const u64* a;
   const u64* b;   // used when flags & LIN_B
@@ -109,14 +113,17 @@ struct LinTask {
   u32 pid;
   u32 flags;
 };
-constexpr u32 LIN_B = 1, LIN_SUB = 2, LIN_MUL = 4;
+constexpr u32 LIN_B = 1, LIN_SUB = 2, LIN_MUL = 4, LIN_AMONT = 8;
 
-// One limb of an evaluation-domain automorphism.
+// One limb of an evaluation-domain automorphism, optionally with a scaled
+// addend permuted alongside: out = sigma_k(in + add * c)  (mod q_pid).
 struct AutoTask {
   const u64* in;
   u64* out;
+  const u64* add;  // null: plain permutation
+  u64 c, csh;      // constant of the addend and its Shoup companion
   u32 kidx;
-  u32 pad;
+  u32 pid;
 };
 
 int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, bool split, const PrimeC* prm, u32 logN,
@@ -130,7 +137,7 @@ void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 n
 void fides_launch_lincomb_batch(const LcTask* tasks, const LcTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
                                 hipStream_t st);
 void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
-void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st);
+void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st);
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st);

@@ -155,13 +155,20 @@ __global__ __launch_bounds__(256) void k_to_mont(u64* key, const PrimeC* prm, u3
 // out[brev(j)] = in[brev(((2j+1) k mod 2N) >> 1)], evaluation form
 // (OpenFHE PrecomputeAutoMap / AutomorphismTransform; the reference's
 // scatter src/Rotation.cuh:14-26 with k^-1 is the same permutation).
-__global__ __launch_bounds__(256) void k_automorph_batch(const AutoTask* __restrict__ tasks, u32 logN) {
+__global__ __launch_bounds__(256) void k_automorph_batch(const AutoTask* __restrict__ tasks,
+                                                         const PrimeC* __restrict__ prm, u32 logN) {
   const AutoTask t = tasks[blockIdx.y];
   const u32 i = blockIdx.x * 256u + threadIdx.x;
   const u32 j = __brev(i) >> (32 - logN);
   const u32 mask = (2u << logN) - 1;
   const u32 s = (((2u * j + 1u) * t.kidx) & mask) >> 1;
-  t.out[i] = t.in[__brev(s) >> (32 - logN)];
+  const u32 src = __brev(s) >> (32 - logN);
+  u64 v = t.in[src];
+  if (t.add) {  // uniform per task
+    const u64 q = prm[t.pid].q;
+    v = add_mod(v, shoup(t.add[src], t.c, t.csh, q), q);
+  }
+  t.out[i] = v;
 }
 
 // ---------------------------------------------------------------- axpy
@@ -183,6 +190,12 @@ __global__ __launch_bounds__(256) void k_lin_batch(const LinTask* __restrict__ t
   const u32 k = blockIdx.x * 256u + threadIdx.x;
   const u64 q = prm[t.pid].q;
   u64 v = t.a[k];
+  if (t.flags & LIN_AMONT) {  // v R^-1 mod q: Montgomery reduction of (0, v)
+    PrimeC P;
+    P.q = q;
+    P.qneg_inv = prm[t.pid].qneg_inv;
+    v = redc_once(0, v, P);
+  }
   if (t.flags & LIN_MUL) v = shoup(v, t.m, t.msh, q);
   if (t.flags & LIN_B) v = (t.flags & LIN_SUB) ? sub_mod(v, t.b[k], q) : add_mod(v, t.b[k], q);
   t.out[k] = add_mod(v, t.add, q);
@@ -294,9 +307,9 @@ void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC
   k_tensor_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
 }
 
-void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, u32 logN, hipStream_t st) {
+void fides_launch_automorph_batch(const AutoTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
   if (ntasks == 0) return;
-  k_automorph_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, logN);
+  k_automorph_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
 }
 
 void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32 LK, hipStream_t st) {

@@ -466,7 +466,7 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
     acc_value(a1, h, l);
     e1[k] = mont_redc(h, l, P);
   }
-  if (tk.jown != 0xffffffffu) {
+  if (tk.jown != 0xffffffffu || tk.ext) {  // uniform per workgroup
 #pragma unroll
     for (int k = 0; k < C::E; ++k) {
       const u64 i = rbase + r0 + C::TPL * k;

@@ -17,6 +17,7 @@
 #include <stdexcept>
 
 #include "fideslib_amd.h"
+#include "program.hpp"
 
 namespace fideslib {
 
@@ -24,9 +25,11 @@ static void chk(int rc, const char* what) {
   if (rc != FIDES_OK) throw std::runtime_error(std::string(what) + ": " + fides_last_error());
 }
 
-CiphertextImpl::CiphertextImpl(std::shared_ptr<CryptoContextImpl> c, uint32_t l) : limbs(l), cc(std::move(c)) {
+CiphertextImpl::CiphertextImpl(std::shared_ptr<CryptoContextImpl> c, uint32_t l, bool e)
+    : limbs(l), cc(std::move(c)), ext(e) {
   void* p = nullptr;
-  chk(fides_device_alloc((size_t)2 * l * cc->GetRingDimension() * 8, &p), "ciphertext allocation");
+  const size_t nl = e ? l + cc->GetNumSpecialPrimes() : l;
+  chk(fides_device_alloc((size_t)2 * nl * cc->GetRingDimension() * 8, &p), "ciphertext allocation");
   d = (uint64_t*)p;
 }
 
@@ -190,8 +193,53 @@ const fides_key* CryptoContextImpl::RotationKey(uint32_t a) const {
   return it->second;
 }
 
-Ciphertext CryptoContextImpl::NewCiphertext(uint32_t limbs) {
-  return Ciphertext(new CiphertextImpl(shared_from_this(), limbs));
+Ciphertext CryptoContextImpl::NewCiphertext(uint32_t limbs, bool ext) {
+  return Ciphertext(new CiphertextImpl(shared_from_this(), limbs, ext));
+}
+
+static void copy_meta(CiphertextImpl& o, const CiphertextImpl& a) {
+  o.NoiseLevel = a.NoiseLevel;
+  o.NoiseFactor = a.NoiseFactor;
+  o.slots = a.slots;
+  o.keyid = a.keyid;
+}
+
+Ciphertext CryptoContextImpl::Regular(const Ciphertext& a) {
+  if (!a->ext) return a;
+  Ciphertext o = NewCiphertext(a->limbs);
+  const uint64_t* in = a->d;
+  uint64_t* out = o->d;
+  const uint32_t l = a->limbs;
+  chk(fides_ext_moddown_batch(ctx, 1, &in, &l, &out, nullptr), "ModDown");
+  copy_meta(*o, *a);
+  return o;
+}
+
+Ciphertext CryptoContextImpl::Extended(const Ciphertext& a) {
+  if (a->ext) return a;
+  Ciphertext o = NewCiphertext(a->limbs, true);
+  const uint64_t* in = a->d;
+  uint64_t* out = o->d;
+  const uint32_t l = a->limbs;
+  chk(fides_ext_raise_batch(ctx, 1, &in, &l, &out, nullptr), "scale by P");
+  copy_meta(*o, *a);
+  return o;
+}
+
+// Sum in Q_l u P of two ciphertexts at one level and scale degree, at least
+// one of them extended (the other is scaled by P first, as Ciphertext::add
+// extends its operand in the reference's Accumulate).
+Ciphertext CryptoContextImpl::AddSubExt(const Ciphertext& a0, const Ciphertext& b0, bool sub) {
+  const Ciphertext a = Extended(a0), b = Extended(b0);
+  Ciphertext o = NewCiphertext(a->limbs, true);
+  const uint64_t* pa = a->d;
+  const uint64_t* pb = b->d;
+  uint64_t* po = o->d;
+  const uint32_t l = a->limbs;
+  chk(fides_ext_add_batch(ctx, 1, &pa, &pb, &l, sub ? 1 : 0, &po, nullptr), sub ? "EvalSub" : "EvalAdd");
+  copy_meta(*o, *a);
+  o->slots = std::max(a->slots, b->slots);
+  return o;
 }
 
 Ciphertext CryptoContextImpl::LoadCiphertext(const RawCipherText& raw) {
@@ -242,7 +290,7 @@ Plaintext CryptoContextImpl::LoadPlaintext(const RawPlainText& raw) {
 }
 
 Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Plaintext& p) {
-  Ciphertext a = a0;
+  Ciphertext a = Regular(a0);
   if (technique != FIXEDMANUAL && a->NoiseLevel == 2) a = Rescale(a);
   if (p->limbs < a->limbs) {
     if (technique == FIXEDMANUAL || Flexible())
@@ -262,7 +310,8 @@ Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Plaintext& p)
   return o;
 }
 
-RawCipherText CryptoContextImpl::StoreCiphertext(const Ciphertext& ct) {
+RawCipherText CryptoContextImpl::StoreCiphertext(const Ciphertext& ct0) {
+  const Ciphertext ct = Regular(ct0);
   const uint32_t l = ct->limbs;
   std::vector<uint64_t> flat((size_t)2 * l * N);
   chk(fides_copy_to_host(flat.data(), ct->d, flat.size() * 8, nullptr), "ciphertext download");
@@ -289,7 +338,8 @@ void CryptoContextImpl::MultMetadata(CiphertextImpl& o, const CiphertextImpl& a,
   o.keyid = a.keyid;
 }
 
-Ciphertext CryptoContextImpl::DropToLimbs(const Ciphertext& a, uint32_t limbs) {
+Ciphertext CryptoContextImpl::DropToLimbs(const Ciphertext& a0, uint32_t limbs) {
+  const Ciphertext a = Regular(a0);
   if (limbs == 0 || limbs > a->limbs) throw std::runtime_error("DropToLimbs: bad limb count");
   Ciphertext o = NewCiphertext(limbs);
   const size_t bytes = (size_t)limbs * N * 8;
@@ -336,9 +386,10 @@ std::vector<uint64_t> CryptoContextImpl::ScalarResidues(uint32_t limbs, double o
 }
 
 // The constant of an addition: round(|c| * sf) (64-bit, the reference's
-// ElemForEvalAddOrSub, src/CKKS/Context.cu:468-560), times round(sf) for
-// every scale degree above one, negated for c < 0 (Ciphertext::addScalar,
-// src/CKKS/Ciphertext.cpp:780-798).
+// ElemForEvalAddOrSub, src/CKKS/Context.cu:468-560, with OpenFHE's
+// MAX_BITS_IN_WORD = 61 (recalled; OpenFHE is not available here)), times
+// round(sf) for every scale degree above one, negated for c < 0
+// (Ciphertext::addScalar, src/CKKS/Ciphertext.cpp:780-798).
 std::vector<uint64_t> CryptoContextImpl::AddScalarResidues(uint32_t limbs, double c, int noiseDeg) const {
   const uint32_t level = limbs - 1;
   const bool extTop = technique == FLEXIBLEAUTOEXT && level == L - 1;
@@ -347,7 +398,7 @@ std::vector<uint64_t> CryptoContextImpl::AddScalarResidues(uint32_t limbs, doubl
   const double mag = std::fabs(c) * sf;
   if (mag > 0) {
     const int logSF = (int)std::ceil(std::log2(mag));
-    logApprox = logSF - std::min(logSF, 60);
+    logApprox = logSF - std::min(logSF, 61);
   }
   const uint64_t base = (uint64_t)(std::fabs(c) * sf / std::ldexp(1.0, logApprox) + 0.5);
   const uint64_t isf = (uint64_t)(sf + 0.5);
@@ -363,7 +414,8 @@ std::vector<uint64_t> CryptoContextImpl::AddScalarResidues(uint32_t limbs, doubl
   return r;
 }
 
-Ciphertext CryptoContextImpl::MulConst(const Ciphertext& a, const std::vector<uint64_t>& res) {
+Ciphertext CryptoContextImpl::MulConst(const Ciphertext& a0, const std::vector<uint64_t>& res) {
+  const Ciphertext a = Regular(a0);
   Ciphertext o = NewCiphertext(a->limbs);
   const uint64_t* in = a->d;
   uint64_t* out = o->d;
@@ -388,7 +440,7 @@ Ciphertext CryptoContextImpl::MultScalarNoPrecheck(const Ciphertext& a, double c
 }
 
 Ciphertext CryptoContextImpl::MultScalar(const Ciphertext& a0, double c) {
-  Ciphertext a = a0;
+  Ciphertext a = Regular(a0);
   if (technique != FIXEDMANUAL && a->NoiseLevel == 2) a = Rescale(a);
   if (a->NoiseLevel != 1) throw std::runtime_error("EvalMult(scalar): operand at scale degree > 1 (FIXEDMANUAL)");
   return MultScalarNoPrecheck(a, c);
@@ -477,7 +529,9 @@ bool CryptoContextImpl::AdjustForMult(Ciphertext& a, const Ciphertext& b) {
 }
 
 Ciphertext CryptoContextImpl::AddSub(const Ciphertext& a0, const Ciphertext& b0, bool sub) {
-  Ciphertext a = a0, b = b0;
+  if ((a0->ext || b0->ext) && a0->limbs == b0->limbs && a0->NoiseLevel == b0->NoiseLevel)
+    return AddSubExt(a0, b0, sub);
+  Ciphertext a = Regular(a0), b = Regular(b0);
   if (technique != FIXEDMANUAL) {
     if (!AdjustForAddOrSub(a, b)) {
       if (!AdjustForAddOrSub(b, a)) throw std::runtime_error("EvalAdd/EvalSub: cannot align operands");
@@ -503,7 +557,8 @@ Ciphertext CryptoContextImpl::AddSub(const Ciphertext& a0, const Ciphertext& b0,
 Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, const Ciphertext& b) { return AddSub(a, b, false); }
 Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, const Ciphertext& b) { return AddSub(a, b, true); }
 
-Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double c) {
+Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a0, double c) {
+  const Ciphertext a = Regular(a0);
   Ciphertext o = NewCiphertext(a->limbs);
   const std::vector<uint64_t> res = AddScalarResidues(a->limbs, c, a->NoiseLevel);
   const uint64_t* in = a->d;
@@ -520,11 +575,11 @@ Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, double c) {
 
 Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, double c) { return EvalAdd(a, -c); }
 
-Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a, double c) {
-  Ciphertext o = MultScalar(a, c);
-  if (technique == FIXEDAUTO) o = Rescale(o);
-  return o;
-}
+// Ciphertext::multScalar(c) with its default rescale = false
+// (src/CKKS/Ciphertext.cpp:764-777, api/CryptoContext.cpp:1244-1266): the
+// product stays at scale degree 2 under every technique; automatic
+// techniques rescale it before its next multiplication.
+Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a, double c) { return MultScalar(a, c); }
 
 Ciphertext CryptoContextImpl::EvalNegate(const Ciphertext& a) {
   std::vector<uint64_t> res(a->limbs);
@@ -537,26 +592,27 @@ void CryptoContextImpl::PrepareMult(Ciphertext& a, Ciphertext& b) {
     if (a->limbs != b->limbs) throw std::runtime_error("EvalMult: operands at different levels (FIXEDMANUAL)");
     return;
   }
+  // Ciphertext::mult's alignment (src/CKKS/Ciphertext.cpp:458-477): adjust a
+  // to b, or else b to the adjusted a
+  if (!AdjustForMult(a, b)) {
+    if (!AdjustForMult(b, a)) throw std::runtime_error("EvalMult: cannot align operands");
+  }
+  if (a->NoiseLevel == 2) a = Rescale(a);
+  if (b->NoiseLevel == 2) b = Rescale(b);
   if (Flexible()) {
-    // Ciphertext::mult's alignment (src/CKKS/Ciphertext.cpp:458-477)
-    if (!AdjustForMult(a, b)) {
-      if (!AdjustForMult(b, a)) throw std::runtime_error("EvalMult: cannot align operands");
-    }
-    if (a->NoiseLevel == 2) a = Rescale(a);
-    if (b->NoiseLevel == 2) b = Rescale(b);
     if (a->limbs != b->limbs) throw std::runtime_error("EvalMult: alignment left different levels");
     return;
   }
-  // FIXEDAUTO: rescale degree-2 operands, then align limb counts
-  if (a->NoiseLevel == 2) a = Rescale(a);
-  if (b->NoiseLevel == 2) b = Rescale(b);
+  // FIXEDAUTO's rules leave the levels apart: the reference multiplies at the
+  // first operand's level (reading the other's first limbs), OpenFHE drops
+  // the higher operand; the latter, which agrees wherever the former is defined
   if (a->limbs > b->limbs) a = DropToLimbs(a, b->limbs);
   if (b->limbs > a->limbs) b = DropToLimbs(b, a->limbs);
 }
 
 Ciphertext CryptoContextImpl::EvalMult(const Ciphertext& a0, const Ciphertext& b0) {
   if (!evalMultKey) throw std::runtime_error("EvalMult: relinearization key not loaded");
-  Ciphertext a = a0, b = b0;
+  Ciphertext a = Regular(a0), b = Regular(b0);
   PrepareMult(a, b);
   Ciphertext o = NewCiphertext(a->limbs);
   chk(fides_hmult(ctx, a->d, b->d, a->limbs, evalMultKey, o->d, nullptr), "EvalMult");
@@ -568,7 +624,7 @@ void CryptoContextImpl::EvalMultInPlace(Ciphertext& a, const Ciphertext& b) { a 
 
 Ciphertext CryptoContextImpl::EvalSquare(const Ciphertext& a0) {
   if (!evalMultKey) throw std::runtime_error("EvalSquare: relinearization key not loaded");
-  Ciphertext a = a0;
+  Ciphertext a = Regular(a0);
   if (technique != FIXEDMANUAL && a->NoiseLevel == 2) a = Rescale(a);
   Ciphertext o = NewCiphertext(a->limbs);
   chk(fides_square(ctx, a->d, a->limbs, evalMultKey, o->d, nullptr), "EvalSquare");
@@ -578,7 +634,8 @@ Ciphertext CryptoContextImpl::EvalSquare(const Ciphertext& a0) {
 
 void CryptoContextImpl::EvalSquareInPlace(Ciphertext& a) { a = EvalSquare(a); }
 
-Ciphertext CryptoContextImpl::Rescale(const Ciphertext& a) {
+Ciphertext CryptoContextImpl::Rescale(const Ciphertext& a0) {
+  const Ciphertext a = Regular(a0);
   if (a->limbs < 2) throw std::runtime_error("Rescale: no limb left to drop");
   Ciphertext o = NewCiphertext(a->limbs - 1);
   chk(fides_rescale(ctx, a->d, a->limbs, o->d, nullptr), "Rescale");
@@ -591,7 +648,8 @@ Ciphertext CryptoContextImpl::Rescale(const Ciphertext& a) {
 
 void CryptoContextImpl::RescaleInPlace(Ciphertext& a) { a = Rescale(a); }
 
-Ciphertext CryptoContextImpl::EvalRotate(const Ciphertext& a, int32_t index) {
+Ciphertext CryptoContextImpl::EvalRotate(const Ciphertext& a0, int32_t index) {
+  const Ciphertext a = Regular(a0);
   const uint32_t k = AutoIndex(index);
   if (k == 1) {
     Ciphertext o = DropToLimbs(a, a->limbs);
@@ -608,7 +666,8 @@ Ciphertext CryptoContextImpl::EvalRotate(const Ciphertext& a, int32_t index) {
 
 void CryptoContextImpl::EvalRotateInPlace(Ciphertext& a, int32_t index) { a = EvalRotate(a, index); }
 
-Ciphertext CryptoContextImpl::EvalConjugate(const Ciphertext& a) {
+Ciphertext CryptoContextImpl::EvalConjugate(const Ciphertext& a0) {
+  const Ciphertext a = Regular(a0);
   const uint32_t k = 2 * N - 1;
   Ciphertext o = NewCiphertext(a->limbs);
   chk(fides_rotate(ctx, a->d, a->limbs, RotationKey(k), k, o->d, nullptr), "EvalConjugate");
@@ -623,7 +682,9 @@ std::vector<Ciphertext> CryptoContextImpl::EvalMultMany(const std::vector<Cipher
                                                         const std::vector<Ciphertext>& b, bool rescale) {
   if (!evalMultKey) throw std::runtime_error("EvalMultMany: relinearization key not loaded");
   if (a.size() != b.size()) throw std::runtime_error("EvalMultMany: operand counts differ");
-  std::vector<Ciphertext> A(a), B(b), out;
+  std::vector<Ciphertext> A, B, out;
+  for (const auto& x : a) A.push_back(Regular(x));
+  for (const auto& x : b) B.push_back(Regular(x));
   std::vector<const uint64_t*> pa, pb;
   std::vector<uint64_t*> po;
   std::vector<uint32_t> lv;
@@ -654,8 +715,9 @@ std::shared_ptr<void> CryptoContextImpl::EvalFastRotationPrecompute(const Cipher
   return nullptr;
 }
 
-std::vector<Ciphertext> CryptoContextImpl::EvalFastRotation(const Ciphertext& a, const std::vector<int32_t>& indices,
+std::vector<Ciphertext> CryptoContextImpl::EvalFastRotation(const Ciphertext& a0, const std::vector<int32_t>& indices,
                                                             uint32_t, const std::shared_ptr<void>&) {
+  const Ciphertext a = Regular(a0);
   std::vector<uint64_t*> po;
   std::vector<uint32_t> kv;
   std::vector<const fides_key*> keys;
@@ -682,8 +744,10 @@ Ciphertext CryptoContextImpl::EvalFastRotation(const Ciphertext& a, int32_t inde
   return EvalFastRotation(a, std::vector<int32_t>{index}, m, precomp)[0];
 }
 
-std::vector<Ciphertext> CryptoContextImpl::EvalRotateMany(const std::vector<Ciphertext>& a,
+std::vector<Ciphertext> CryptoContextImpl::EvalRotateMany(const std::vector<Ciphertext>& a0,
                                                           const std::vector<int32_t>& idx) {
+  std::vector<Ciphertext> a;
+  for (const auto& x : a0) a.push_back(Regular(x));
   if (a.size() != idx.size()) throw std::runtime_error("EvalRotateMany: counts differ");
   std::vector<const uint64_t*> pin;
   std::vector<uint64_t*> po;
@@ -742,6 +806,253 @@ CryptoContextImpl::Tuning CryptoContextImpl::Autotune(const std::vector<uint32_t
   fides_tuning t;
   chk(fides_autotune(ctx, (uint32_t)limbs.size(), limbs.data(), reps, &t), "Autotune");
   return from_c(t);
+}
+
+// ---------------------------------------------------------------- plaintext addition
+Ciphertext CryptoContextImpl::EmbedPlaintext(const Plaintext& p) {
+  Ciphertext o = NewCiphertext(p->limbs);
+  chk(fides_plaintext_embed(ctx, p->d, p->limbs, o->d, nullptr), "plaintext embedding");
+  o->NoiseLevel = p->NoiseLevel;
+  o->NoiseFactor = p->NoiseFactor;
+  o->slots = p->slots;
+  return o;
+}
+
+// Ciphertext::addPt / subPt (src/CKKS/Ciphertext.cpp:242-300) with
+// Plaintext::adjustPlaintextToCiphertext (src/CKKS/Plaintext.cu:195-226): the
+// plaintext is handled as the ciphertext (pt, 0), so its rescale and scalar
+// multiplication are the ciphertext kernels and c1 gets + 0.
+Ciphertext CryptoContextImpl::AddSubPt(const Ciphertext& a0, const Plaintext& p, bool sub) {
+  Ciphertext a = Regular(a0);
+  Ciphertext pc = EmbedPlaintext(p);
+  if (technique != FIXEDMANUAL) {
+    if (pc->NoiseLevel == 1 && a->NoiseLevel == 2 && pc->limbs + 1 == a->limbs) a = Rescale(a);
+    if (pc->limbs != a->limbs) {
+      if (technique == FIXEDAUTO) {
+        const int pl = (int)pc->limbs - 1 - pc->NoiseLevel, cl = (int)a->limbs - 1 - a->NoiseLevel;
+        if (pl > cl) {
+          if (a->NoiseLevel == 1 && pc->NoiseLevel == 2) {
+            pc = Rescale(DropToLimbs(pc, a->limbs + 1));
+          } else {
+            pc = DropToLimbs(pc, a->limbs);
+          }
+        } else if (a->NoiseLevel == 1 && pc->NoiseLevel == 2) {
+          pc = Rescale(pc);
+        } else if (pc->NoiseLevel == 1 && a->NoiseLevel == 2) {
+          throw std::runtime_error("EvalAdd(plaintext): cannot align the plaintext");
+        }
+      } else {
+        if (pc->limbs < a->limbs || !AdjustScaleAndLevel(pc, a->NoiseLevel, a->limbs - 1, a->NoiseFactor))
+          throw std::runtime_error("EvalAdd(plaintext): plaintext below the ciphertext level");
+      }
+    }
+  }
+  if (pc->NoiseLevel != a->NoiseLevel)
+    throw std::runtime_error("EvalAdd(plaintext): plaintext and ciphertext at different scale degrees");
+  if (pc->limbs < a->limbs) throw std::runtime_error("EvalAdd(plaintext): plaintext below the ciphertext level");
+  if (pc->limbs > a->limbs) pc = DropToLimbs(pc, a->limbs);
+  Ciphertext o = NewCiphertext(a->limbs);
+  const uint64_t* pa = a->d;
+  const uint64_t* pb = pc->d;
+  uint64_t* po = o->d;
+  const uint32_t l = a->limbs;
+  chk(fides_add_batch(ctx, 1, &pa, &pb, &l, sub ? 1 : 0, &po, nullptr), sub ? "EvalSub(plaintext)" : "EvalAdd(plaintext)");
+  copy_meta(*o, *a);
+  return o;
+}
+
+Ciphertext CryptoContextImpl::EvalAdd(const Ciphertext& a, const Plaintext& p) { return AddSubPt(a, p, false); }
+Ciphertext CryptoContextImpl::EvalSub(const Ciphertext& a, const Plaintext& p) { return AddSubPt(a, p, true); }
+
+// ---------------------------------------------------------------- sums of many
+Ciphertext CryptoContextImpl::EvalAddMany(const std::vector<Ciphertext>& cts) {
+  if (cts.empty()) throw std::runtime_error("EvalAddMany: input ciphertext vector is empty");
+  if (cts.size() == 1) return DropToLimbs(cts[0], cts[0]->limbs);
+  bool uniform = true;
+  for (const auto& c : cts)
+    uniform = uniform && !c->ext && c->limbs == cts[0]->limbs && c->NoiseLevel == cts[0]->NoiseLevel &&
+              c->NoiseFactor == cts[0]->NoiseFactor;
+  if (!uniform) {  // the reference's pairwise tree (api/CryptoContext.cpp:909-926)
+    const size_t n = cts.size();
+    std::vector<Ciphertext> sums(n - 1);
+    size_t k = 0;
+    for (size_t i = 0; i < 2 * n - 2; i += 2)
+      sums[k++] = EvalAdd(i < n ? cts[i] : sums[i - n], i + 1 < n ? cts[i + 1] : sums[i + 1 - n]);
+    return sums.back();
+  }
+  // one level and scale: sums of up to 16 terms per fused launch
+  const uint32_t l = cts[0]->limbs;
+  const std::vector<uint64_t> one(l, 1);
+  std::vector<Ciphertext> cur(cts.begin(), cts.end());
+  while (cur.size() > 1) {
+    std::vector<Ciphertext> next;
+    for (size_t i = 0; i < cur.size(); i += 16) {
+      const size_t m = std::min<size_t>(16, cur.size() - i);
+      if (m == 1) {
+        next.push_back(cur[i]);
+        continue;
+      }
+      Ciphertext o = NewCiphertext(l);
+      std::vector<const uint64_t*> xs, ms;
+      std::vector<uint32_t> xl(m, l);
+      for (size_t j = 0; j < m; ++j) {
+        xs.push_back(cur[i + j]->d);
+        ms.push_back(one.data());
+      }
+      const uint32_t cnt = (uint32_t)m;
+      const uint64_t* add = nullptr;
+      uint64_t* po = o->d;
+      chk(fides_lincomb_batch(ctx, 1, &cnt, xs.data(), xl.data(), ms.data(), &add, &l, &po, nullptr), "EvalAddMany");
+      copy_meta(*o, *cur[i]);
+      next.push_back(o);
+    }
+    cur.swap(next);
+  }
+  int slots = 0;
+  for (const auto& c : cts) slots = std::max(slots, c->slots);
+  cur[0]->slots = slots;
+  return cur[0];
+}
+
+void CryptoContextImpl::EvalAddManyInPlace(std::vector<Ciphertext>& cts) { cts.at(0) = EvalAddMany(cts); }
+
+// ---------------------------------------------------------------- extended rotations, accumulation
+std::vector<Ciphertext> CryptoContextImpl::EvalFastRotationExt(const Ciphertext& a0,
+                                                               const std::vector<int32_t>& indices,
+                                                               const std::shared_ptr<void>&, bool addFirst) {
+  const Ciphertext a = Regular(a0);
+  std::vector<uint64_t*> po;
+  std::vector<uint32_t> kv;
+  std::vector<const fides_key*> keys;
+  std::vector<Ciphertext> out;
+  for (int32_t idx : indices) {
+    const uint32_t k = AutoIndex(idx);
+    Ciphertext o = NewCiphertext(a->limbs, true);
+    copy_meta(*o, *a);
+    po.push_back(o->d);
+    kv.push_back(k);
+    keys.push_back(k == 1 ? nullptr : RotationKey(k));
+    out.push_back(o);
+  }
+  chk(fides_rotate_hoisted_ext(ctx, a->d, a->limbs, (uint32_t)indices.size(), keys.data(), kv.data(),
+                               addFirst ? 1 : 0, po.data(), nullptr),
+      "EvalFastRotationExt");
+  return out;
+}
+
+Ciphertext CryptoContextImpl::EvalFastRotationExt(const Ciphertext& a, int32_t index,
+                                                  const std::shared_ptr<void>& digits, bool addFirst) {
+  return EvalFastRotationExt(a, std::vector<int32_t>{index}, digits, addFirst)[0];
+}
+
+// Rounds s = start, 4 start, 16 start, ...: rotations by stride*s*j (j = 1..3,
+// below stride*size) hoisted and left in Q_l u P, summed with P * a, one
+// ModDown (src/CKKS/AccumulateBroadcast.cu:9-42 and :79-106, bStep = 4).
+void CryptoContextImpl::AccumulateImpl(Ciphertext& a, int stride, int size, int start) {
+  if (stride <= 0 || size <= 0 || start <= 0) throw std::runtime_error("AccumulateSum: stride, slots and start must be > 0");
+  Ciphertext x = Regular(a);
+  for (int64_t s = start; s < size; s <<= 2) {
+    std::vector<int32_t> idx;
+    for (int64_t i = stride * s; i < (int64_t)stride * size && i < 4 * stride * s; i += stride * s)
+      idx.push_back((int32_t)i);
+    if (idx.empty()) continue;
+    const auto rots = EvalFastRotationExt(x, idx, nullptr, true);
+    Ciphertext acc = Extended(x);
+    const uint32_t l = acc->limbs;
+    for (const auto& r : rots) {
+      const uint64_t* pa = acc->d;
+      const uint64_t* pb = r->d;
+      uint64_t* po = acc->d;
+      chk(fides_ext_add_batch(ctx, 1, &pa, &pb, &l, 0, &po, nullptr), "AccumulateSum");
+    }
+    x = Regular(acc);
+  }
+  if ((int64_t)size * stride == x->slots) x->slots = stride * start;
+  a = x;
+}
+
+Ciphertext CryptoContextImpl::AccumulateSum(const Ciphertext& a, int slots, int stride) {
+  Ciphertext x = a;
+  AccumulateImpl(x, stride, slots, 1);
+  return x;
+}
+void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& a, int slots, int stride) { AccumulateImpl(a, stride, slots, 1); }
+void CryptoContextImpl::AccumulateSumInPlace(Ciphertext& a, int slots, int stride, int start) {
+  AccumulateImpl(a, stride, slots, start);
+}
+
+// ---------------------------------------------------------------- levels
+Ciphertext CryptoContextImpl::SetLevelImpl(const Ciphertext& a0, size_t level) {
+  const Ciphertext a = Regular(a0);
+  if (level >= L) throw std::runtime_error("SetLevel: level beyond the modulus chain");
+  const uint32_t target = L - (uint32_t)level;
+  if (target >= a->limbs) return a;  // dropToLevel only drops
+  if (Flexible()) {
+    if (a->NoiseLevel != 1 && a->NoiseLevel != 2) throw std::runtime_error("SetLevel: scale degree must be 1 or 2");
+    Ciphertext x = a;
+    const uint32_t lev = target - 1;
+    AdjustScaleAndLevel(x, x->NoiseLevel, lev, x->NoiseLevel == 1 ? sfReal.at(lev) : sfBig.at(lev));
+    return x;
+  }
+  return DropToLimbs(a, target);
+}
+
+void CryptoContextImpl::SetLevel(Ciphertext& a, size_t level) { a = a->cc->SetLevelImpl(a, level); }
+
+// ---------------------------------------------------------------- Chebyshev series
+std::vector<double> CryptoContextImpl::GetChebyshevCoefficients(std::function<double(double)>& func, double a,
+                                                                double b, size_t degree) {
+  if (degree == 0) throw std::runtime_error("GetChebyshevCoefficients: the degree can not be zero");
+  const size_t n = degree + 1;
+  const double bma = 0.5 * (b - a), bpa = 0.5 * (b + a), pd = M_PI / (double)n;
+  std::vector<double> fx(n), c(n, 0.0);
+  for (size_t i = 0; i < n; ++i) fx[i] = func(std::cos(pd * ((double)i + 0.5)) * bma + bpa);
+  const double f = 2.0 / (double)n;
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t j = 0; j < n; ++j) c[i] += fx[j] * std::cos(pd * (double)i * ((double)j + 0.5));
+    c[i] *= f;
+  }
+  return c;
+}
+
+Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& a0, std::vector<double>& coeffs, double lo,
+                                                  double hi) {
+  if (!evalMultKey) throw std::runtime_error("EvalChebyshevSeries: relinearization key not loaded");
+  if (coeffs.size() < 2) throw std::runtime_error("EvalChebyshevSeries: needs a degree of at least 1");
+  if (!(hi > lo)) throw std::runtime_error("EvalChebyshevSeries: needs lo < hi");
+  Ciphertext a = Regular(a0);
+  if (a->NoiseLevel == 2) a = Rescale(a);
+  if (a->NoiseLevel != 1) throw std::runtime_error("EvalChebyshevSeries: scale degree must be 1 or 2");
+  detail::Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
+  detail::Reg x;
+  x.buf = std::make_shared<detail::DevBuf>(a->d, false);
+  x.level = a->limbs;
+  x.scale = a->NoiseFactor;
+  // y = (2x - lo - hi) / (hi - lo): multiply by round(sa * delta) and Rescale,
+  // then add round(sb * scale) (fideslib_amd/serve.py chebyshev_program)
+  const double sa = 2.0 / (hi - lo), sb = -(hi + lo) / (hi - lo);
+  detail::Reg y = x;
+  if (!(sa == 1.0 && sb == 0.0)) {
+    y = X.mul_real(x, sa);
+    if (sb != 0.0) {
+      const detail::i128 add = detail::Exec::round_i128(sb * y.scale);
+      y = X.lin(y, nullptr, nullptr, false, &add, y.scale);
+    }
+  }
+  std::vector<double> c = coeffs;
+  c[0] *= 0.5;  // OpenFHE's series is c_0 / 2 + sum_k c_k T_k
+  const detail::Reg r = detail::chebyshev_eval(X, y, c);
+  Ciphertext o = NewCiphertext(r.level);
+  chk(fides_copy_device(o->d, r.buf->d, (size_t)2 * r.level * N * 8, nullptr), "EvalChebyshevSeries");
+  o->NoiseLevel = 1;
+  o->NoiseFactor = r.scale;
+  o->slots = a->slots;
+  o->keyid = a->keyid;
+  return o;
+}
+
+void CryptoContextImpl::EvalChebyshevSeriesInPlace(Ciphertext& a, std::vector<double>& coeffs, double lo, double hi) {
+  a = EvalChebyshevSeries(a, coeffs, lo, hi);
 }
 
 }  // namespace fideslib

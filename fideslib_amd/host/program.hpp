@@ -1,0 +1,373 @@
+// program.hpp -- internal to the host C++ API (not installed): a straight-
+// line program executor over device ciphertext registers and the Chebyshev
+// series evaluator built on it.  It mirrors the Python program builder
+// (fideslib_amd/serve.py Program, chebyshev_eval): every operation is one
+// call of the batched C ABI with a batch of one, levels are tracked in Q
+// limbs and scales nominally (a rescaled scale is scale / q_top), and
+// integer constants are rounded half to even exactly as Python's round().
+#ifndef FIDESLIB_AMD_HOST_PROGRAM_HPP
+#define FIDESLIB_AMD_HOST_PROGRAM_HPP
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fideslib/fideslib.hpp"
+#include "fideslib_amd.h"
+
+namespace fideslib {
+namespace detail {
+
+using i128 = __int128;
+
+inline void chk(int rc, const char* what) {
+  if (rc != 0) {
+    const char* m = fides_last_error();
+    throw std::runtime_error(std::string(what) + ": " + (m ? m : ""));
+  }
+}
+
+inline uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
+  unsigned __int128 r = 1, x = b % m;
+  while (e) {
+    if (e & 1) r = r * x % m;
+    x = x * x % m;
+    e >>= 1;
+  }
+  return (uint64_t)r;
+}
+
+
+// A device buffer owned by a register (own == false: a view of caller memory).
+struct DevBuf {
+  uint64_t* d = nullptr;
+  bool own = true;
+  DevBuf(uint64_t* view, bool) : d(view), own(false) {}
+  explicit DevBuf(size_t words) {
+    void* p = nullptr;
+    chk(fides_device_alloc(words * 8, &p), "device buffer");
+    d = (uint64_t*)p;
+  }
+  ~DevBuf() {
+    if (d && own) fides_device_free(d);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+};
+
+struct Reg {
+  std::shared_ptr<DevBuf> buf;
+  uint32_t level = 0;  // limbs
+  double scale = 0;
+};
+
+
+// A prepared device plaintext (evaluation form, Montgomery representation).
+struct DevPt {
+  std::shared_ptr<DevBuf> buf;
+  uint32_t level;
+  double scale;
+};
+
+// The program builder/executor over one ciphertext (mirrors serve.Program).
+struct Exec {
+  CryptoContextImpl& cc;
+  fides_context* ctx;
+  const std::vector<uint64_t>& q;
+  uint32_t N;
+  double delta;
+  const fides_key* rlk;
+
+  Reg fresh(uint32_t level, double scale) {
+    if (level < 1) throw std::runtime_error("out of levels");
+    Reg r;
+    r.buf = std::make_shared<DevBuf>((size_t)2 * level * N);
+    r.level = level;
+    r.scale = scale;
+    return r;
+  }
+  std::vector<uint64_t> residues(i128 x, uint32_t l) const {
+    std::vector<uint64_t> r(l);
+    for (uint32_t j = 0; j < l; ++j) {
+      i128 v = x % (i128)q[j];
+      if (v < 0) v += q[j];
+      r[j] = (uint64_t)v;
+    }
+    return r;
+  }
+  static i128 round_i128(double x) { return (i128)std::nearbyint(x); }
+
+  Reg drop(const Reg& a, uint32_t level) {
+    if (level == a.level) return a;
+    Reg o = fresh(level, a.scale);
+    const uint64_t* in = a.buf->d;
+    uint64_t* out = o.buf->d;
+    chk(fides_drop_limbs_batch(ctx, 1, &in, &a.level, &level, &out, nullptr), "drop");
+    return o;
+  }
+  Reg hmul(Reg a, Reg b) {  // with Rescale
+    const uint32_t l = std::min(a.level, b.level);
+    a = drop(a, l);
+    b = drop(b, l);
+    if (l < 2) throw std::runtime_error("HMult+Rescale needs 2 limbs");
+    Reg o = fresh(l - 1, a.scale * b.scale / (double)q[l - 1]);
+    const uint64_t* pa = a.buf->d;
+    const uint64_t* pb = b.buf->d;
+    uint64_t* po = o.buf->d;
+    chk(fides_hmult_batch(ctx, 1, &pa, &pb, &l, rlk, 1, &po, nullptr), "HMult");
+    return o;
+  }
+  Reg automorph(const Reg& a, uint32_t k) {
+    Reg o = fresh(a.level, a.scale);
+    const uint64_t* in = a.buf->d;
+    uint64_t* out = o.buf->d;
+    const fides_key* key = cc.RotationKeyFor(k);
+    chk(fides_rotate_batch(ctx, 1, &in, &a.level, &key, &k, &out, nullptr), "rotate");
+    return o;
+  }
+  Reg rotate(const Reg& a, int64_t r) {
+    const int64_t half = N / 2;
+    const int64_t rr = ((r % half) + half) % half;
+    if (rr == 0) return a;
+    return automorph(a, (uint32_t)powmod(5, (uint64_t)rr, 2ull * N));
+  }
+  Reg conjugate(const Reg& a) { return automorph(a, 2 * N - 1); }
+  // out = a*mul (+/-) b + add (integers)
+  Reg lin(Reg a, const i128* mul, const Reg* b0, bool sub, const i128* add, double scale) {
+    Reg b;
+    if (b0) {
+      const uint32_t l = std::min(a.level, b0->level);
+      a = drop(a, l);
+      b = drop(*b0, l);
+    }
+    Reg o = fresh(a.level, scale);
+    const uint32_t l = a.level;
+    std::vector<uint64_t> mr, ar;
+    if (mul) mr = residues(*mul, l);
+    if (add) ar = residues(*add, l);
+    const uint64_t* pa = a.buf->d;
+    const uint64_t* pb = b0 ? b.buf->d : nullptr;
+    const uint64_t* pm = mul ? mr.data() : nullptr;
+    const uint64_t* pd = add ? ar.data() : nullptr;
+    const uint32_t s = sub ? 1 : 0;
+    uint64_t* po = o.buf->d;
+    chk(fides_lin_batch(ctx, 1, &pa, &pb, &l, &pm, &pd, &s, &po, nullptr), "linear");
+    return o;
+  }
+  Reg add(const Reg& a, const Reg& b) { return lin(a, nullptr, &b, false, nullptr, a.scale); }
+  Reg sub(const Reg& a, const Reg& b) { return lin(a, nullptr, &b, true, nullptr, a.scale); }
+  Reg rescale(const Reg& a) {
+    if (a.level < 2) throw std::runtime_error("Rescale needs 2 limbs");
+    Reg o = fresh(a.level - 1, a.scale / (double)q[a.level - 1]);
+    const uint64_t* in = a.buf->d;
+    uint64_t* out = o.buf->d;
+    chk(fides_rescale_batch(ctx, 1, &in, &a.level, &out, nullptr), "Rescale");
+    return o;
+  }
+  Reg mul_real(const Reg& a, double x) {
+    const i128 m = round_i128(x * delta);
+    return rescale(lin(a, &m, nullptr, false, nullptr, a.scale * delta));
+  }
+  Reg match_scale(const Reg& a, double scale, uint32_t level) {
+    if (a.level <= level) throw std::runtime_error("match_scale needs a spare level");
+    const double qq = (double)q[a.level - 1];
+    const i128 m = round_i128(scale * qq / a.scale);
+    Reg t = lin(a, &m, nullptr, false, nullptr, a.scale * (double)m);
+    return drop(rescale(t), level);
+  }
+  Reg lincomb(const std::vector<std::pair<Reg, i128>>& terms, const i128* add, double scale) {
+    uint32_t l = ~0u;
+    for (const auto& t : terms) l = std::min(l, t.first.level);
+    Reg o = fresh(l, scale);
+    const uint32_t cnt = (uint32_t)terms.size();
+    std::vector<const uint64_t*> xs;
+    std::vector<uint32_t> xl;
+    std::vector<std::vector<uint64_t>> mres;
+    std::vector<const uint64_t*> mp;
+    for (const auto& t : terms) {
+      xs.push_back(t.first.buf->d);
+      xl.push_back(t.first.level);
+      mres.push_back(residues(t.second, l));
+    }
+    for (auto& v : mres) mp.push_back(v.data());
+    std::vector<uint64_t> ar;
+    if (add) ar = residues(*add, l);
+    const uint64_t* pa = add ? ar.data() : nullptr;
+    uint64_t* po = o.buf->d;
+    chk(fides_lincomb_batch(ctx, 1, &cnt, xs.data(), xl.data(), mp.data(), &pa, &l, &po, nullptr), "lincomb");
+    return o;
+  }
+  Reg ptdot(const std::vector<std::pair<Reg, const DevPt*>>& terms) {
+    uint32_t l = ~0u;
+    for (const auto& t : terms) l = std::min(l, t.first.level);
+    std::vector<Reg> regs;
+    for (const auto& t : terms) {
+      regs.push_back(drop(t.first, l));
+      if (t.second->level < l) throw std::runtime_error("plaintext below the ciphertext level");
+    }
+    Reg o = fresh(l, regs[0].scale * terms[0].second->scale);
+    const uint32_t cnt = (uint32_t)terms.size();
+    std::vector<const uint64_t*> cs, ps;
+    for (size_t i = 0; i < terms.size(); ++i) {
+      cs.push_back(regs[i].buf->d);
+      ps.push_back(terms[i].second->buf->d);
+    }
+    uint64_t* po = o.buf->d;
+    chk(fides_ptdot_batch(ctx, 1, &cnt, cs.data(), ps.data(), &l, &po, nullptr), "plaintext product");
+    return o;
+  }
+  Reg mod_raise(Reg a, uint32_t level, double scale) {
+    a = drop(a, 1);
+    Reg o = fresh(level, scale);
+    const uint64_t* in = a.buf->d;
+    uint64_t* out = o.buf->d;
+    chk(fides_mod_raise_batch(ctx, 1, &in, &level, &out, nullptr), "ModRaise");
+    return o;
+  }
+};
+
+// ---- Chebyshev series by baby steps and giant steps (serve.chebyshev_eval)
+struct Cheb {
+  Exec* X;  // null: structural dry run (levels only)
+  std::map<uint32_t, Reg> T;
+  std::map<uint32_t, uint32_t> TL;  // levels (dry run and real)
+  uint32_t m = 2;
+
+  uint32_t lvl(uint32_t k) const { return TL.at(k); }
+};
+
+inline std::vector<uint32_t> leaf_terms(const std::vector<double>& cs) {
+  std::vector<uint32_t> ks;
+  for (uint32_t k = 1; k < cs.size(); ++k)
+    if (cs[k] != 0.0) ks.push_back(k);
+  if (ks.empty()) ks.push_back(1);
+  return ks;
+}
+
+inline void split(const std::vector<double>& cs, uint32_t Mg, std::vector<double>& qc, std::vector<double>& rc) {
+  qc.assign(1, cs[Mg]);
+  for (size_t k = 1; k < cs.size() - Mg; ++k) qc.push_back(2.0 * cs[Mg + k]);
+  rc.assign(Mg, 0.0);
+  for (uint32_t j = 0; j < Mg; ++j) rc[j] = cs[j] - (2 * Mg - j < cs.size() ? cs[2 * Mg - j] : 0.0);
+}
+
+inline uint32_t giant_of(const Cheb& C, size_t n) {
+  uint32_t g = C.m;
+  while (2 * (size_t)g < n) g *= 2;
+  return g;
+}
+
+inline uint32_t node_level(const Cheb& C, const std::vector<double>& cs) {
+  if (cs.size() <= C.m) {
+    uint32_t l = ~0u;
+    for (uint32_t k : leaf_terms(cs)) l = std::min(l, C.lvl(k));
+    return l - 1;
+  }
+  const uint32_t Mg = giant_of(C, cs.size());
+  std::vector<double> qc, rc;
+  split(cs, Mg, qc, rc);
+  return std::min(std::min(node_level(C, qc), C.lvl(Mg)) - 1, node_level(C, rc));
+}
+
+inline Reg node_build(Cheb& C, const std::vector<double>& cs, double S) {
+  Exec& X = *C.X;
+  if (cs.size() <= C.m) {
+    const auto ks = leaf_terms(cs);
+    uint32_t l = ~0u;
+    for (uint32_t k : ks) l = std::min(l, C.lvl(k));
+    const double target = S * (double)X.q[l - 1];
+    std::vector<std::pair<Reg, i128>> terms;
+    for (uint32_t k : ks)
+      terms.emplace_back(C.T.at(k), k < cs.size() ? Exec::round_i128(cs[k] * target / C.T.at(k).scale) : 0);
+    const i128 add = Exec::round_i128(cs[0] * target);
+    return X.rescale(X.lincomb(terms, &add, target));
+  }
+  const uint32_t Mg = giant_of(C, cs.size());
+  std::vector<double> qc, rc;
+  split(cs, Mg, qc, rc);
+  const uint32_t lp = std::min(node_level(C, qc), C.lvl(Mg));
+  Reg qr = node_build(C, qc, S * (double)X.q[lp - 1] / C.T.at(Mg).scale);
+  Reg prod = X.hmul(qr, C.T.at(Mg));
+  return X.add(prod, node_build(C, rc, S));
+}
+
+// Levels of the basis T_1..T_m and giants (the product rule and doubling)
+// without a device: the structure of Cheb's basis construction.
+inline void basis_levels(Cheb& C, uint32_t y_level, uint32_t deg) {
+  C.TL.clear();
+  C.TL[1] = y_level;
+  const uint32_t upto = std::max(1u, std::min(C.m, deg));
+  uint32_t m = 1;
+  while (m < upto) {
+    for (uint32_t k = m + 1; k <= std::min(2 * m, upto); ++k) {
+      const uint32_t n = k - m;
+      const uint32_t pl = std::min(C.TL[m], C.TL[n]) - 1;  // product level
+      C.TL[k] = pl;  // the subtraction operand is aligned down to it
+    }
+    m *= 2;
+  }
+  uint32_t M = C.m;
+  while (2 * M <= deg) {
+    C.TL[2 * M] = C.TL[M] - 1;
+    M *= 2;
+  }
+}
+
+inline Reg chebyshev_eval(Exec& X, const Reg& y, std::vector<double> c) {
+  size_t deg = c.size() - 1;
+  while (deg > 0 && c[deg] == 0.0) --deg;
+  c.resize(deg + 1);
+  if (deg < 1) throw std::runtime_error("Chebyshev degree must be >= 1");
+  Cheb C;
+  C.X = &X;
+  while ((size_t)C.m * C.m < deg + 1) C.m *= 2;
+  const uint32_t upto = std::max<uint32_t>(1, std::min<uint32_t>(C.m, (uint32_t)deg));
+  C.T[1] = y;
+  uint32_t m = 1;
+  while (m < upto) {
+    for (uint32_t k = m + 1; k <= std::min(2 * m, upto); ++k) {
+      const uint32_t n = k - m;
+      Reg prod = X.hmul(C.T[m], C.T[n]);
+      if (n == m) {
+        const i128 two = 2, add = -Exec::round_i128(prod.scale);
+        C.T[k] = X.lin(prod, &two, nullptr, false, &add, prod.scale);
+      } else {
+        Reg b = C.T[m - n];
+        if (b.scale != prod.scale && b.level > prod.level) b = X.match_scale(b, prod.scale, prod.level);
+        const i128 two = 2;
+        C.T[k] = X.lin(prod, &two, &b, true, nullptr, prod.scale);
+      }
+    }
+    m *= 2;
+  }
+  uint32_t M = C.m;
+  while (2 * (size_t)M <= deg) {
+    Reg sq = X.hmul(C.T[M], C.T[M]);
+    const i128 two = 2, add = -Exec::round_i128(sq.scale);
+    C.T[2 * M] = X.lin(sq, &two, nullptr, false, &add, sq.scale);
+    M *= 2;
+  }
+  for (auto& kv : C.T) C.TL[kv.first] = kv.second.level;
+  return node_build(C, c, X.delta);
+}
+
+inline uint32_t chebyshev_eval_depth(const std::vector<double>& c0) {
+  std::vector<double> c = c0;
+  size_t deg = c.size() - 1;
+  while (deg > 0 && c[deg] == 0.0) --deg;
+  c.resize(deg + 1);
+  Cheb C;
+  C.X = nullptr;
+  while ((size_t)C.m * C.m < deg + 1) C.m *= 2;
+  basis_levels(C, 64, (uint32_t)deg);
+  return 64 - node_level(C, c);
+}
+
+}  // namespace detail
+}  // namespace fideslib
+
+#endif

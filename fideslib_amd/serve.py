@@ -289,12 +289,14 @@ def chebyshev_program(prm, level: int, coeffs, lo: float, hi: float) -> Program:
     (see chebyshev_eval)."""
     P = Program(prm.q, prm.logN, prm.scale_bits, "chebyshev%d" % (len(coeffs) - 1))
     x = P.input(level)
-    # y = a*x + b with a = 2/(hi-lo), b = -(hi+lo)/(hi-lo)
+    # y = a*x + b with a = 2/(hi-lo), b = -(hi+lo)/(hi-lo); none on [-1, 1]
     a = 2.0 / (hi - lo)
     b = -(hi + lo) / (hi - lo)
-    y = P.mul_real(x, a)
-    if b != 0.0:
-        y = P.add_real(y, b)
+    y = x
+    if not (a == 1.0 and b == 0.0):
+        y = P.mul_real(x, a)
+        if b != 0.0:
+            y = P.add_real(y, b)
     P.set_output(chebyshev_eval(P, y, coeffs))
     return P
 

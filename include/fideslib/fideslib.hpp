@@ -17,6 +17,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -114,6 +115,12 @@ class CiphertextImpl {
   double GetScalingFactor() const { return NoiseFactor; }
   int GetSlots() const { return slots; }
   uint64_t* DeviceData() const { return d; }
+  // In the extended basis Q_l u P (an EvalFastRotationExt result or a sum of
+  // them): [2][limbs + K][N] holding P times the ciphertext.  Every operation
+  // other than addition and subtraction takes it back to Q_l first (one
+  // ModDown per component), as the reference does lazily
+  // (src/CKKS/Ciphertext.cpp:940-1114, isModUp()).
+  bool IsExtended() const { return ext; }
 
   uint32_t limbs = 0;
   int NoiseLevel = 1;
@@ -123,8 +130,9 @@ class CiphertextImpl {
 
  private:
   friend class CryptoContextImpl;
-  explicit CiphertextImpl(std::shared_ptr<CryptoContextImpl> cc, uint32_t limbs);
+  CiphertextImpl(std::shared_ptr<CryptoContextImpl> cc, uint32_t limbs, bool ext = false);
   std::shared_ptr<CryptoContextImpl> cc;
+  bool ext = false;
   uint64_t* d = nullptr;
 };
 using Ciphertext = std::shared_ptr<CiphertextImpl>;
@@ -163,6 +171,7 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   ~CryptoContextImpl();
 
   uint32_t GetRingDimension() const { return N; }
+  uint32_t GetNumSpecialPrimes() const { return K; }
   uint32_t GetCyclotomicOrder() const { return 2 * N; }
   ScalingTechnique GetScalingTechnique() const { return technique; }
 
@@ -190,6 +199,13 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   void EvalMultInPlace(Ciphertext& a, const Plaintext& p) { a = EvalMult(a, p); }
   Ciphertext EvalSquare(const Ciphertext& a);
   void EvalSquareInPlace(Ciphertext& a);
+  // The reference's "Mutable" forms are aliases of the plain ones
+  // (api/CryptoContext.cpp:864-883, 1321-1347, 1401-1404).
+  Ciphertext EvalMultMutable(Ciphertext& a, Ciphertext& b) { return EvalMult(a, b); }
+  Ciphertext EvalMultMutable(Ciphertext& a, Plaintext& p) { return EvalMult(a, p); }
+  Ciphertext EvalMultMutable(Plaintext& p, Ciphertext& a) { return EvalMult(a, p); }
+  void EvalMultMutableInPlace(Ciphertext& a, Ciphertext& b) { a = EvalMult(a, b); }
+  Ciphertext EvalSquareMutable(Ciphertext& a) { return EvalSquare(a); }
   Ciphertext EvalRotate(const Ciphertext& a, int32_t index);
   void EvalRotateInPlace(Ciphertext& a, int32_t index);
   Ciphertext EvalConjugate(const Ciphertext& a);
@@ -199,6 +215,43 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   Ciphertext EvalFastRotation(const Ciphertext& a, int32_t index, uint32_t m, const std::shared_ptr<void>& precomp);
   std::vector<Ciphertext> EvalFastRotation(const Ciphertext& a, const std::vector<int32_t>& indices, uint32_t m,
                                            const std::shared_ptr<void>& precomp);
+  // Hoisted rotations left in the extended basis (api/CryptoContext.hpp:171,
+  // 174): sigma_k(P c0 + KS0), sigma_k(KS1) without the ModDown (addFirst ==
+  // false leaves P c0 out, OpenFHE's meaning; the reference's GPU path always
+  // adds it).  Sums of the results take one ModDown when used.  `digits` is
+  // unused: the device path raises the digits itself (one ModUp per call).
+  Ciphertext EvalFastRotationExt(const Ciphertext& a, int32_t index, const std::shared_ptr<void>& digits,
+                                 bool addFirst);
+  std::vector<Ciphertext> EvalFastRotationExt(const Ciphertext& a, const std::vector<int32_t>& indices,
+                                              const std::shared_ptr<void>& digits, bool addFirst);
+  // sum_{j < slots} Rotate(a, j stride) (api/CryptoContext.hpp:188-190): rounds
+  // of three hoisted extended rotations whose sum takes one ModDown (the
+  // reference GPU Accumulate with bStep 4, src/CKKS/AccumulateBroadcast.cu:
+  // 79-106); with `start`, the rounds begin at rotation stride*start
+  // (AccumulateCascadeImpl, :9-42).  Rotation keys for stride*s*j
+  // (j = 1..3, s = 1, 4, 16, ...) must be loaded.
+  Ciphertext AccumulateSum(const Ciphertext& a, int slots, int stride = 1);
+  void AccumulateSumInPlace(Ciphertext& a, int slots, int stride = 1);
+  void AccumulateSumInPlace(Ciphertext& a, int slots, int stride, int start);
+  // Chebyshev series c_0 / 2 + sum_{k>=1} c_k T_k(y), y = (2x - a - b)/(b - a)
+  // (api/CryptoContext.hpp:176-178, OpenFHE's coefficient convention; the
+  // reference evalChebyshevSeries, src/CKKS/ApproxModEval.cu:371): the map to
+  // [-1, 1] (skipped for [-1, 1]), baby steps T_1..T_m by the product rule,
+  // giant steps by doubling and a recursive Chebyshev division into fused
+  // linear combinations.  The result is at scale degree 1 with the nominal
+  // scale 2^(bit length of q_1); it needs ceil(log2(deg)) + 2 levels.
+  Ciphertext EvalChebyshevSeries(const Ciphertext& a, std::vector<double>& coeffs, double lo, double hi);
+  void EvalChebyshevSeriesInPlace(Ciphertext& a, std::vector<double>& coeffs, double lo, double hi);
+  // Interpolation coefficients of func on [a, b] at the degree + 1 Chebyshev
+  // nodes, c_0 not halved (reference get_chebyshev_coefficients,
+  // src/PolyApprox.cu:11-32).
+  static std::vector<double> GetChebyshevCoefficients(std::function<double(double)>& func, double a, double b,
+                                                      size_t degree);
+  // OpenFHE's level convention (levels consumed): keep L - level limbs
+  // (api/CryptoContext.cpp:1714-1717 -> Ciphertext::dropToLevel); with
+  // FLEXIBLEAUTO(EXT) the scale follows the target level's factor
+  // (adjustScaleAndLevel), otherwise limbs are dropped.
+  static void SetLevel(Ciphertext& a, size_t level);
   Ciphertext Rescale(const Ciphertext& a);
   void RescaleInPlace(Ciphertext& a);
   // Drop limbs above `limbs` (OpenFHE LevelReduce / FIDESlib dropToLevel).
@@ -212,10 +265,48 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   Ciphertext EvalAdd(const Ciphertext& a, double c);
   Ciphertext EvalSub(const Ciphertext& a, double c);
   Ciphertext EvalMult(const Ciphertext& a, double c);
+  Ciphertext EvalAdd(double c, const Ciphertext& a) { return EvalAdd(a, c); }
+  Ciphertext EvalSub(double c, const Ciphertext& a) { return EvalAdd(EvalNegate(a), c); }
+  Ciphertext EvalMult(double c, const Ciphertext& a) { return EvalMult(a, c); }
   Ciphertext EvalNegate(const Ciphertext& a);
+  void EvalNegateInPlace(Ciphertext& a) { a = EvalNegate(a); }
   void EvalAddInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalAdd(a, b); }
   void EvalSubInPlace(Ciphertext& a, const Ciphertext& b) { a = EvalSub(a, b); }
+  void EvalAddInPlace(Ciphertext& a, double c) { a = EvalAdd(a, c); }
+  void EvalSubInPlace(Ciphertext& a, double c) { a = EvalSub(a, c); }
+  void EvalAddInPlace(double c, Ciphertext& a) { a = EvalAdd(a, c); }
+  void EvalSubInPlace(double c, Ciphertext& a) { a = EvalSub(c, a); }
   void EvalMultInPlace(Ciphertext& a, double c) { a = EvalMult(a, c); }
+  void EvalMultInPlace(double c, Ciphertext& a) { a = EvalMult(a, c); }
+  Ciphertext EvalAddMutable(Ciphertext& a, Ciphertext& b) { return EvalAdd(a, b); }
+  Ciphertext EvalSubMutable(Ciphertext& a, Ciphertext& b) { return EvalSub(a, b); }
+  void EvalAddMutableInPlace(Ciphertext& a, Ciphertext& b) { a = EvalAdd(a, b); }
+  void EvalSubMutableInPlace(Ciphertext& a, Ciphertext& b) { a = EvalSub(a, b); }
+  // Plaintext addition and subtraction (api/CryptoContext.hpp:119-146;
+  // Ciphertext::addPt / subPt, src/CKKS/Ciphertext.cpp:242-300): with
+  // automatic rescaling a degree-2 ciphertext one level above the plaintext
+  // is rescaled, and a plaintext at another level is brought to the
+  // ciphertext's (dropped for FIXEDAUTO, adjustScaleAndLevel for
+  // FLEXIBLEAUTO(EXT); it must not be below the ciphertext).  FIXEDMANUAL
+  // needs equal scale degrees and uses the plaintext's first limbs.
+  Ciphertext EvalAdd(const Ciphertext& a, const Plaintext& p);
+  Ciphertext EvalAdd(const Plaintext& p, const Ciphertext& a) { return EvalAdd(a, p); }
+  Ciphertext EvalSub(const Ciphertext& a, const Plaintext& p);
+  Ciphertext EvalSub(const Plaintext& p, const Ciphertext& a) { return EvalAdd(EvalNegate(a), p); }
+  void EvalAddInPlace(Ciphertext& a, const Plaintext& p) { a = EvalAdd(a, p); }
+  void EvalAddInPlace(const Plaintext& p, Ciphertext& a) { a = EvalAdd(a, p); }
+  void EvalSubInPlace(Ciphertext& a, const Plaintext& p) { a = EvalSub(a, p); }
+  Ciphertext EvalAddMutable(Ciphertext& a, Plaintext& p) { return EvalAdd(a, p); }
+  Ciphertext EvalAddMutable(Plaintext& p, Ciphertext& a) { return EvalAdd(a, p); }
+  Ciphertext EvalSubMutable(Ciphertext& a, Plaintext& p) { return EvalSub(a, p); }
+  Ciphertext EvalSubMutable(Plaintext& p, Ciphertext& a) { return EvalSub(p, a); }
+  // Sum of n ciphertexts (api/CryptoContext.hpp:133-134; the reference adds
+  // them pairwise in a tree, api/CryptoContext.cpp:884-927).  Inputs at one
+  // level and scale are summed in fused linear-combination launches (the same
+  // bits: modular addition is exact); otherwise the pairwise tree of EvalAdd
+  // with its level/scale alignment runs.
+  Ciphertext EvalAddMany(const std::vector<Ciphertext>& cts);
+  void EvalAddManyInPlace(std::vector<Ciphertext>& cts);
 
   double ScalingFactorReal(uint32_t level) const { return sfReal.at(level); }
   double ScalingFactorRealBig(uint32_t level) const { return sfBig.at(level); }
@@ -261,7 +352,14 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
 
  private:
   CryptoContextImpl() = default;
-  Ciphertext NewCiphertext(uint32_t limbs);
+  Ciphertext NewCiphertext(uint32_t limbs, bool ext = false);
+  Ciphertext Regular(const Ciphertext& a);          // ModDown of an extended ciphertext (else a)
+  Ciphertext Extended(const Ciphertext& a);         // (P c0, P c1) in Q_l u P (a if extended)
+  Ciphertext AddSubExt(const Ciphertext& a, const Ciphertext& b, bool sub);
+  Ciphertext EmbedPlaintext(const Plaintext& p);    // (pt, 0) as a ciphertext
+  Ciphertext AddSubPt(const Ciphertext& a, const Plaintext& p, bool sub);
+  void AccumulateImpl(Ciphertext& a, int stride, int size, int start);
+  Ciphertext SetLevelImpl(const Ciphertext& a, size_t level);
   uint32_t AutoIndex(int32_t index) const;
   const fides_key* RotationKey(uint32_t autoIndex) const;
   void PrepareMult(Ciphertext& a, Ciphertext& b);  // FIXEDAUTO / FLEXIBLEAUTO level/scale alignment

@@ -207,6 +207,33 @@ int fides_rotate_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_
 int fides_rotate_hoisted(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint32_t n,
                          const fides_key* const* keys, const uint32_t* kidx, uint64_t* const* d_out, void* stream);
 
+/* ---- the extended basis Q_l u P (ModDown hoisted out of rotations) ----
+ * An ext ciphertext is [2][l+K][N] uint64, evaluation form: limbs 0..l-1
+ * under q_0..q_{l-1}, then the K special primes.  It holds P times a
+ * ciphertext, so sums of rotations need one ModDown instead of one each
+ * (reference Ciphertext::rotate_hoisted with ext = true,
+ * src/CKKS/Ciphertext.cpp:940-1114, behind CryptoContext::EvalFastRotationExt,
+ * api/CryptoContext.hpp:171,174; Accumulate, src/CKKS/AccumulateBroadcast.cu:79-106). */
+
+/* Hoisted rotations of one ciphertext [2][l][N] left in Q_l u P: out[i] =
+ * sigma_k(P c0 + KS_ext0(c1)), sigma_k(KS_ext1(c1)) with k = kidx[i] and the
+ * key switch's inner product without its ModDown (with add_first == 0 the
+ * P c0 term is left out, OpenFHE's addFirst).  kidx[i] == 1 gives
+ * (P c0 or 0, P c1) (its key may be NULL).  One ModUp serves all n.  Outputs
+ * must not overlap the input or each other. */
+int fides_rotate_hoisted_ext(fides_context* ctx, const uint64_t* d_in, uint32_t l, uint32_t n,
+                             const fides_key* const* keys, const uint32_t* kidx, int add_first, uint64_t* const* d_out,
+                             void* stream);
+/* out[i] = (P c0, P c1) in Q_l u P (zero special limbs) for [2][l[i]][N] inputs. */
+int fides_ext_raise_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
+                          uint64_t* const* d_out, void* stream);
+/* out[i] = a[i] + b[i] (or - with subtract != 0) over [2][l[i]+K][N]; out may alias a or b. */
+int fides_ext_add_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_a, const uint64_t* const* d_b,
+                        const uint32_t* l, int subtract, uint64_t* const* d_out, void* stream);
+/* ModDown of both components: [2][l[i]+K][N] -> [2][l[i]][N] (as fides_moddown). */
+int fides_ext_moddown_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
+                            uint64_t* const* d_out, void* stream);
+
 /* Rescale of n ciphertexts ([2][l[i]] -> [2][l[i]-1]) in one pass. */
 int fides_rescale_batch(fides_context* ctx, uint32_t n, const uint64_t* const* d_in, const uint32_t* l,
                         uint64_t* const* d_out, void* stream);
@@ -265,6 +292,13 @@ int fides_drop_limbs_batch(fides_context* ctx, uint32_t n, const uint64_t* const
  * form fides_ptdot_batch consumes (the reference keeps its linear-transform
  * plaintexts on the device in evaluation form, src/CKKS/Plaintext.cu). */
 int fides_plaintext_prepare(fides_context* ctx, uint64_t* d_pt, uint32_t l, void* stream);
+
+/* A prepared plaintext [l][N] as the ciphertext (pt, 0) [2][l][N] in standard
+ * evaluation form (a trivial encryption), so plaintext additions and their
+ * level/scale alignment run as ciphertext operations (reference
+ * Ciphertext::addPt / subPt, src/CKKS/Ciphertext.cpp:242-300, which add the
+ * plaintext limbs to c0). */
+int fides_plaintext_embed(fides_context* ctx, const uint64_t* d_pt, uint32_t l, uint64_t* d_ct, void* stream);
 
 /* Plaintext dot products: out[i] = sum_{j < cnt[i]} ct_ij (*) pt_ij, both
  * ciphertext components multiplied element-wise by the same plaintext

@@ -273,3 +273,30 @@ def test_keyswitch_decrypts_to_product_with_old_key():
     diff = ctx.intt_limbs(ctx.sub_limbs(lhs, rhs, pids), pids)
     err = C.crt_centered(diff, prm.q[:l])
     assert max(abs(e) for e in err) < 2 ** 24
+
+
+def test_ext_composition_pins_on_the_oracle_keyswitch():
+    """The extended-basis pieces composed in tests/ext_oracle.py: ModDown of
+    the key switch without ModDown is the oracle's key switch; ModDown of the
+    P-scaled embedding is the ciphertext itself; ModDown of an extended
+    rotation decrypts to the rotated message (its automorphism is applied
+    before the ModDown, so the bits may differ from a plain rotation)."""
+    import ext_oracle as X
+    from fideslib_amd import params
+    from oracle import ckks_codec as CC
+    prm = params.ckks_params(11, 60, 40, 5, dnum=3)
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = CC.Keys(octx, 11)
+    N = prm.N
+    rng = np.random.default_rng(5)
+    for l in (prm.L, 3):
+        z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+        ct = keys.encrypt(CC.encode(z, 2.0 ** 40, N), l)
+        key = keys.rotation_key(pow(5, 3, 2 * N))
+        k0, k1 = X.keyswitch_ext(octx, ct[1], l, key)
+        w0, w1 = octx.keyswitch(ct[1], l, key)
+        assert np.array_equal(octx.moddown(k0, l), w0) and np.array_equal(octx.moddown(k1, l), w1)
+        assert np.array_equal(X.moddown_ext(octx, X.raise_ext(octx, ct, l), l), ct)
+        r = X.moddown_ext(octx, X.rotate_ext(octx, ct, l, key, pow(5, 3, 2 * N)), l)
+        dec = CC.decode(keys.decrypt_coef(r, l), 2.0 ** 40, N)
+        assert CC.precision_bits(dec, np.roll(z, -3)) > 20
