@@ -6,7 +6,9 @@ mathematics and computed on the host):
 
   1. prescale   x <- x * 2^-c, Rescale, keep one limb (emulates a larger q_0
                 so the sine below is near-linear; undone by an exact integer
-                multiply by 2^c at the end, as the reference's correction factor)
+                multiply by 2^c at the end, as the reference's correction factor;
+                under FLEXIBLEAUTO the same multiply also brings the input
+                level's scaling factor to delta)
   2. ModRaise   one limb at q_0 -> L limbs: the ciphertext now encrypts
                 t = m + q_0 I(X) with a small integer polynomial I.  With
                 sparse-secret encapsulation (reference Bootstrap.cu:254-260,
@@ -191,13 +193,15 @@ class BootConfig:
     K: float = 32.0             # |I| + |m/q_0| bound of the argument
     correction: int = 10        # prescale 2^-c
     encaps_h: int | None = None  # sparse-secret encapsulation: Hamming weight of s' (None: off)
+    flexible: bool = False      # FLEXIBLEAUTO: the result carries its level's FLEXIBLEAUTO
+                                # scaling factor (folded into the first SlotToCoeff group)
 
     @staticmethod
-    def encapsulated(h: int = 32) -> "BootConfig":
+    def encapsulated(h: int = 32, flexible: bool = False) -> "BootConfig":
         """Sparse-secret encapsulation with a weight-h secret: |I| stays
         below ~8 with overwhelming probability for h = 32 (I is a sum of h + 1
         terms uniform in [-1/2, 1/2]), so K' = 16."""
-        return BootConfig(K=16.0, encaps_h=h)
+        return BootConfig(K=16.0, encaps_h=h, flexible=flexible)
 
 
 class BootstrapPlan:
@@ -238,6 +242,15 @@ class BootstrapPlan:
             raise ValueError("not enough levels for bootstrapping: need %d, have %d" % (self.depth + 1, self.L))
         self._pt_cache = {}
         self._pts = None
+        self.out_scale = None
+        if cfg.flexible:
+            # the nominal output scale does not depend on the input (ModRaise
+            # declares delta): fold target / nominal into SlotToCoeff's factor
+            from .params import flexible_scaling_factors
+            s_nom = self._nominal_output_scale()
+            self.out_scale = flexible_scaling_factors(self.q)[self.output_level() - 1]
+            self.f_stc *= self.out_scale / s_nom
+            self.stc_plans = [plan_group(D, n, self.f_stc if i == 0 else 1.0) for i, D in enumerate(stc)]
 
     # -- rotation indices every program uses (slot rotations, mod n); conjugation separately
     def rotations(self) -> list:
@@ -284,27 +297,58 @@ class BootstrapPlan:
         c[p] = sign
         return self._upload(codec.to_rns(c, self.q[:level]), level, 1.0)
 
-    def build_plaintexts(self) -> None:
-        """Encode every diagonal of every group at the level it is used."""
+    def build_plaintexts(self, scale_only: bool = False) -> None:
+        """Encode every diagonal of every group at the level it is used
+        (scale_only: placeholders with level and scale, for structure)."""
         if self._pts is not None:
             return
         pts = {"cts": [], "stc": []}
         l = self.l_raise
         for P in self.cts_plans:
-            pts["cts"].append(self._group_pts(P, l))
+            pts["cts"].append(self._group_pts(P, l, scale_only))
             l -= 1
         l_split = l
-        self._minus_i = self._monomial(3 * self.N // 2, l_split)
+        mono = (lambda p, lv: Plaintext(0, lv, 1.0)) if scale_only else self._monomial
+        self._minus_i = mono(3 * self.N // 2, l_split)
         l -= self.evalmod_depth
-        self._plus_i = self._monomial(self.N // 2, l)
+        self._plus_i = mono(self.N // 2, l)
         for P in self.stc_plans:
-            pts["stc"].append(self._group_pts(P, l))
+            pts["stc"].append(self._group_pts(P, l, scale_only))
             l -= 1
         self._pts = pts
 
-    def _group_pts(self, P: GroupPlan, level: int) -> dict:
+    def _group_pts(self, P: GroupPlan, level: int, scale_only: bool = False) -> dict:
         scale = float(self.q[level - 1])      # Rescale after the products divides it out
+        if scale_only:
+            return {k: [(b, Plaintext(0, level, scale)) for b, _ in terms] for k, terms in P.giants.items()}
         return {k: [(b, self._encode(v, level, scale)) for b, v in terms] for k, terms in P.giants.items()}
+
+    def _nominal_output_scale(self) -> float:
+        """The tracked scale of program()'s result (structure only)."""
+        self.build_plaintexts(scale_only=True)
+        try:
+            return self.program(self.L).output.scale
+        finally:
+            self._pts = None
+
+    def export_precomputation(self) -> dict:
+        """The plan's tables for the C++ API's EvalBootstrapSetup(const
+        BootstrapPrecomputation&): per group (CoeffToSlot then SlotToCoeff,
+        application order) the stride, baby-step count, babies and the terms
+        (giant k, baby b, level, scale, coefficient-form residues), the
+        EvalMod series and the output scale.  Needs keep_host=True."""
+        self.build_plaintexts()
+
+        def group(P, pts):
+            terms = []
+            for k in sorted(pts):
+                for b, pt in pts[k]:
+                    assert pt.host is not None, "export_precomputation needs keep_host=True"
+                    terms.append((k, b, pt.level, pt.scale, np.ascontiguousarray(pt.host, np.uint64)))
+            return dict(stride=P.stride, g=P.g, babies=list(P.babies), terms=terms)
+        return dict(cts=[group(P, x) for P, x in zip(self.cts_plans, self._pts["cts"])],
+                    stc=[group(P, x) for P, x in zip(self.stc_plans, self._pts["stc"])],
+                    cheb=[float(c) for c in self.cheb], out_scale=self.out_scale or 0.0)
 
     # -- program pieces
     def _linear(self, Pg: Program, x: Reg, plan: GroupPlan, pts: dict) -> Reg:
@@ -332,7 +376,14 @@ class BootstrapPlan:
         cfg = self.cfg
         Pg = Program(self.q, self.prm.logN, self.prm.scale_bits, "bootstrap")
         x = Pg.input(level_in, scale_in)
-        x = Pg.mul_real(x, 2.0 ** -cfg.correction)
+        # prescale by 2^-c; with `flexible` also by q_{l-1} / s_in, so that
+        # after the Rescale the message sits at delta whatever the input
+        # level's factor (the reference's adjustment factor,
+        # src/CKKS/Bootstrap.cu:441-470)
+        pre = 2.0 ** -cfg.correction
+        if cfg.flexible:
+            pre = pre * float(self.q[level_in - 1]) / x.scale
+        x = Pg.mul_real(x, pre)
         x = Pg.drop(x, 1)
         if cfg.encaps_h:
             x = Pg.switch_key(x, self.ENCAPS_DOWN)
@@ -349,8 +400,9 @@ class BootstrapPlan:
         w = Pg.add(re, Pg.ptdot([(im, self._plus_i)]))
         for plan, pts in zip(self.stc_plans, self._pts["stc"]):
             w = self._linear(Pg, w, plan, pts)
-        out = Pg.lin(w, mul=1 << cfg.correction)
         # value = the input message: the nominal scale of the result is the
-        # tracked one (the constants above are folded relative to delta)
+        # tracked one (the constants above are folded relative to delta), or
+        # with `flexible` the level's FLEXIBLEAUTO factor folded into SlotToCoeff
+        out = Pg.lin(w, mul=1 << cfg.correction, scale=self.out_scale)
         Pg.set_output(out)
         return Pg

@@ -223,6 +223,8 @@ struct BootPlan {
   BootstrapConfig cfg;
   uint32_t n = 0, depth = 0, evalmod_depth = 0, l_raise = 0;
   double f_cts = 0, f_stc = 0;
+  double out_scale = 0;  // nominal scale of the result (0: the tracked one)
+  bool flexible = false; // FLEXIBLEAUTO: the prescale also brings the input's scale to delta
   std::vector<GroupPlan> cts, stc;
   std::vector<double> cheb;
   // device plaintexts: per group, per giant k, (baby b, prepared plaintext)
@@ -233,117 +235,55 @@ struct BootPlan {
 };
 
 // ---------------------------------------------------------------- plan + API
-std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapConfig& cfg) {
-  auto P = std::make_shared<BootPlan>();
-  P->cfg = cfg;
-  P->n = N / 2;
-  const uint32_t n = P->n;
-  const double q0 = (double)q[0];
-  // CoeffToSlot output y = t / (q_0 K'), the 1/2 of the Re/Im split folded in;
-  // SlotToCoeff input sin(2 pi t / q_0) ~ 2 pi m / q_0 -> m / delta
-  P->f_cts = boot_delta() / (2.0 * q0 * cfg.K);
-  P->f_stc = q0 / (2.0 * M_PI * boot_delta());
-  const auto cts = transform_groups(n, N, cfg.budgetCtS, true);
-  const auto stc = transform_groups(n, N, cfg.budgetStC, false);
-  for (size_t i = 0; i < cts.size(); ++i) P->cts.push_back(plan_group(cts[i], n, i == 0 ? P->f_cts : 1.0));
-  for (size_t i = 0; i < stc.size(); ++i) P->stc.push_back(plan_group(stc[i], n, i == 0 ? P->f_stc : 1.0));
-  const double fa[2] = {cfg.K, std::ldexp(1.0, (int)cfg.doubleAngle)};
-  P->cheb = chebyshev_coefficients(evalmod_fn, fa, cfg.degree);
-  P->l_raise = L;
-  P->evalmod_depth = chebyshev_eval_depth(P->cheb) + cfg.doubleAngle;
-  P->depth = cfg.budgetCtS + cfg.budgetStC + P->evalmod_depth;
-  if (P->l_raise <= P->depth) throw std::runtime_error("EvalBootstrapSetup: not enough levels for bootstrapping");
-  // rotations
+namespace {
+
+using GroupPts = std::map<int64_t, std::vector<std::pair<int64_t, BootPlan::Pt>>>;
+
+// Rotation indices (slot rotations mod n, without 0) of the groups' baby
+// and giant steps.
+std::vector<int32_t> plan_rotations(const BootPlan& P) {
+  const int64_t n = P.n;
   std::vector<int32_t> rots;
-  for (const auto* groups : {&P->cts, &P->stc})
-    for (const GroupPlan& g : *groups) {
-      for (int64_t b : g.babies)
-        if (b) rots.push_back((int32_t)((b * (int64_t)g.stride) % n));
-      for (const auto& kv : g.giants)
-        if (kv.first) rots.push_back((int32_t)((((kv.first * (int64_t)g.g * (int64_t)g.stride) % n) + n) % n));
-    }
+  auto add = [&](const GroupPlan& g, const GroupPts& pts) {
+    for (int64_t b : g.babies)
+      if (b) rots.push_back((int32_t)(((b * (int64_t)g.stride) % n + n) % n));
+    for (const auto& kv : pts)
+      if (kv.first) rots.push_back((int32_t)((((kv.first * (int64_t)g.g * (int64_t)g.stride) % n) + n) % n));
+  };
+  for (size_t i = 0; i < P.cts.size(); ++i) add(P.cts[i], P.cts_pts[i]);
+  for (size_t i = 0; i < P.stc.size(); ++i) add(P.stc[i], P.stc_pts[i]);
   std::sort(rots.begin(), rots.end());
   rots.erase(std::unique(rots.begin(), rots.end()), rots.end());
   rots.erase(std::remove(rots.begin(), rots.end(), 0), rots.end());
-  P->rotations = rots;
-  // plaintexts: encode v at `level` limbs with scale `scale`
-  const std::vector<uint64_t> slot_t = [&] {
-    std::vector<uint64_t> t(n);
-    uint64_t g = 1;
-    for (uint32_t j = 0; j < n; ++j) {
-      t[j] = (g - 1) / 2;
-      g = g * 5 % (2ull * N);
-    }
-    return t;
-  }();
-  auto upload = [&](const std::vector<long double>& coef, uint32_t level, double scale) {
-    BootPlan::Pt pt;
-    pt.level = level;
-    pt.scale = scale;
-    pt.buf = std::make_shared<DevBuf>((size_t)level * N);
-    std::vector<uint64_t> host((size_t)level * N);
-    for (uint32_t j = 0; j < level; ++j)
-      for (uint32_t i = 0; i < N; ++i) {
-        long double r = std::fmod(coef[i], (long double)q[j]);
-        if (r < 0) r += (long double)q[j];
-        host[(size_t)j * N + i] = (uint64_t)r;
-      }
-    chk(fides_copy_to_device(pt.buf->d, host.data(), host.size() * 8, nullptr), "plaintext upload");
-    chk(fides_plaintext_prepare(ctx, pt.buf->d, level, nullptr), "plaintext preparation");
-    return pt;
-  };
-  auto encode = [&](const Vec& z, uint32_t level, double scale) {
-    std::vector<cd> v(N);
-    for (uint32_t j = 0; j < n; ++j) {
-      v[slot_t[j]] = z[j];
-      v[N - 1 - slot_t[j]] = std::conj(z[j]);
-    }
-    fft(v);
-    std::vector<long double> coef(N);
-    for (uint32_t i = 0; i < N; ++i) {
-      const double a = -M_PI * (double)i / (double)N;
-      const cd m = v[i] / (double)N * cd(std::cos(a), std::sin(a));
-      coef[i] = std::nearbyint((long double)(m.real() * scale));
-    }
-    return upload(coef, level, scale);
-  };
-  auto monomial = [&](uint64_t power, uint32_t level) {
-    std::vector<long double> c(N, 0.0L);
-    uint64_t p = power % (2ull * N);
-    long double sign = 1;
-    if (p >= N) {
-      p -= N;
-      sign = -1;
-    }
-    c[p] = sign;
-    return upload(c, level, 1.0);
-  };
-  auto group_pts = [&](const GroupPlan& G, uint32_t level) {
-    std::map<int64_t, std::vector<std::pair<int64_t, BootPlan::Pt>>> out;
-    const double scale = (double)q[level - 1];
-    for (const auto& kv : G.giants)
-      for (const auto& bv : kv.second) out[kv.first].emplace_back(bv.first, encode(bv.second, level, scale));
-    return out;
-  };
-  uint32_t l = P->l_raise;
-  for (const GroupPlan& G : P->cts) P->cts_pts.push_back(group_pts(G, l--));
-  P->minus_i = monomial(3ull * N / 2, l);
-  l -= P->evalmod_depth;
-  P->plus_i = monomial(N / 2, l);
-  for (const GroupPlan& G : P->stc) P->stc_pts.push_back(group_pts(G, l--));
-  boot = P;
   return rots;
 }
 
-Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0) {
-  if (!boot) throw std::runtime_error("EvalBootstrap: call EvalBootstrapSetup first");
-  const Ciphertext ct = Regular(ct0);
-  if (!evalMultKey) throw std::runtime_error("EvalBootstrap: relinearization key not loaded");
-  const BootPlan& P = *boot;
-  Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
-  auto linear = [&](Reg x, const GroupPlan& G, const std::map<int64_t, std::vector<std::pair<int64_t, BootPlan::Pt>>>& pts) {
+// Plaintexts without data (levels and scales), for a dry run.
+void placeholder_pts(BootPlan& P, const std::vector<uint64_t>& q) {
+  auto group = [&](const GroupPlan& G, uint32_t level) {
+    GroupPts out;
+    for (const auto& kv : G.giants)
+      for (const auto& bv : kv.second) out[kv.first].emplace_back(bv.first, BootPlan::Pt{nullptr, level, (double)q[level - 1]});
+    return out;
+  };
+  P.cts_pts.clear();
+  P.stc_pts.clear();
+  uint32_t l = P.l_raise;
+  for (const GroupPlan& G : P.cts) P.cts_pts.push_back(group(G, l--));
+  P.minus_i = BootPlan::Pt{nullptr, l, 1.0};
+  l -= P.evalmod_depth;
+  P.plus_i = BootPlan::Pt{nullptr, l, 1.0};
+  for (const GroupPlan& G : P.stc) P.stc_pts.push_back(group(G, l--));
+}
+
+// The bootstrap program (fideslib_amd/bootstrap.py BootstrapPlan.program):
+// prescale by 2^-c, one limb, [switch to the sparse secret], ModRaise,
+// [switch back], CoeffToSlot, Re/Im split, EvalMod, recombine, SlotToCoeff,
+// x 2^c.
+Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fides_key* up) {
+  auto linear = [&](Reg v, const GroupPlan& G, const GroupPts& pts) {
     std::map<int64_t, Reg> babies;
-    for (int64_t b : G.babies) babies[b] = b == 0 ? x : X.rotate(x, b * (int64_t)G.stride);
+    for (int64_t b : G.babies) babies[b] = b == 0 ? v : X.rotate(v, b * (int64_t)G.stride);
     std::vector<Reg> parts;
     for (const auto& kv : pts) {
       std::vector<std::pair<Reg, const BootPlan::Pt*>> terms;
@@ -365,15 +305,16 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0) {
     }
     return c;
   };
-  // the input as a register (its nominal scale is the tracked scaling factor)
-  Reg x;
-  x.buf = std::make_shared<DevBuf>((size_t)2 * ct->limbs * N);
-  chk(fides_copy_device(x.buf->d, ct->DeviceData(), (size_t)2 * ct->limbs * N * 8, nullptr), "copy");
-  x.level = ct->limbs;
-  x.scale = ct->NoiseFactor > 0 ? ct->NoiseFactor : boot_delta();
-  if (x.level < 2) throw std::runtime_error("EvalBootstrap: the input needs at least 2 limbs");
-  x = X.mul_real(x, std::ldexp(1.0, -(int)P.cfg.correction));
-  Reg r = X.mod_raise(x, P.l_raise, boot_delta());
+  // prescale by 2^-c; under FLEXIBLEAUTO also by q_{l-1} / s_in, so that after
+  // the Rescale the message sits at delta whatever the input level's factor
+  // (the reference's adjustment factor, src/CKKS/Bootstrap.cu:441-470)
+  double pre = std::ldexp(1.0, -(int)P.cfg.correction);
+  if (P.flexible) pre = pre * (double)X.q[x.level - 1] / x.scale;
+  x = X.mul_real(x, pre);
+  x = X.drop(x, 1);
+  if (P.cfg.encapsulation) x = X.switch_key(x, down);
+  Reg r = X.mod_raise(x, P.l_raise, X.delta);
+  if (P.cfg.encapsulation) r = X.switch_key(r, up);
   for (size_t i = 0; i < P.cts.size(); ++i) r = linear(r, P.cts[i], P.cts_pts[i]);
   Reg cj = X.conjugate(r);
   Reg re = X.add(r, cj);
@@ -384,7 +325,183 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0) {
   Reg w = X.add(re, X.ptdot({{im, &P.plus_i}}));
   for (size_t i = 0; i < P.stc.size(); ++i) w = linear(w, P.stc[i], P.stc_pts[i]);
   const i128 corr = (i128)1 << P.cfg.correction;
-  Reg out = X.lin(w, &corr, nullptr, false, nullptr, w.scale);
+  return X.lin(w, &corr, nullptr, false, nullptr, P.out_scale > 0 ? P.out_scale : w.scale);
+}
+
+}  // namespace
+
+// Plaintext upload in coefficient form ([level][N] residues) and preparation.
+static BootPlan::Pt upload_pt(fides_context* ctx, uint32_t N, const std::vector<uint64_t>& host, uint32_t level,
+                              double scale) {
+  BootPlan::Pt pt;
+  pt.level = level;
+  pt.scale = scale;
+  pt.buf = std::make_shared<DevBuf>((size_t)level * N);
+  chk(fides_copy_to_device(pt.buf->d, host.data(), (size_t)level * N * 8, nullptr), "plaintext upload");
+  chk(fides_plaintext_prepare(ctx, pt.buf->d, level, nullptr), "plaintext preparation");
+  return pt;
+}
+
+static BootPlan::Pt monomial_pt(fides_context* ctx, uint32_t N, const std::vector<uint64_t>& q, uint64_t power,
+                                uint32_t level) {
+  std::vector<uint64_t> host((size_t)level * N, 0);
+  uint64_t p = power % (2ull * N);
+  const bool neg = p >= N;
+  if (neg) p -= N;
+  for (uint32_t j = 0; j < level; ++j) host[(size_t)j * N + p] = neg ? q[j] - 1 : 1;
+  return upload_pt(ctx, N, host, level, 1.0);
+}
+
+std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapConfig& cfg) {
+  auto P = std::make_shared<BootPlan>();
+  P->cfg = cfg;
+  P->n = N / 2;
+  const uint32_t n = P->n;
+  const double q0 = (double)q[0];
+  // CoeffToSlot output y = t / (q_0 K'), the 1/2 of the Re/Im split folded in;
+  // SlotToCoeff input sin(2 pi t / q_0) ~ 2 pi m / q_0 -> m / delta
+  P->f_cts = boot_delta() / (2.0 * q0 * cfg.K);
+  P->f_stc = q0 / (2.0 * M_PI * boot_delta());
+  const auto cts = transform_groups(n, N, cfg.budgetCtS, true);
+  const auto stc = transform_groups(n, N, cfg.budgetStC, false);
+  for (size_t i = 0; i < cts.size(); ++i) P->cts.push_back(plan_group(cts[i], n, i == 0 ? P->f_cts : 1.0));
+  for (size_t i = 0; i < stc.size(); ++i) P->stc.push_back(plan_group(stc[i], n, i == 0 ? P->f_stc : 1.0));
+  const double fa[2] = {cfg.K, std::ldexp(1.0, (int)cfg.doubleAngle)};
+  P->cheb = chebyshev_coefficients(evalmod_fn, fa, cfg.degree);
+  P->l_raise = L;
+  P->evalmod_depth = chebyshev_eval_depth(P->cheb) + cfg.doubleAngle;
+  P->depth = (uint32_t)(P->cts.size() + P->stc.size()) + P->evalmod_depth;
+  if (P->l_raise <= P->depth) throw std::runtime_error("EvalBootstrapSetup: not enough levels for bootstrapping");
+  P->flexible = Flexible();
+  if (Flexible()) {
+    // the nominal output scale (independent of the input: ModRaise declares
+    // delta) from a dry run; fold the level's FLEXIBLEAUTO factor into it
+    placeholder_pts(*P, q);
+    Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
+    X.dry = true;
+    Reg x;
+    x.level = L;
+    x.scale = boot_delta();
+    const Reg out = boot_run(X, *P, x, nullptr, nullptr);
+    P->out_scale = sfReal.at(out.level - 1);
+    P->f_stc *= P->out_scale / out.scale;
+    P->stc.clear();
+    for (size_t i = 0; i < stc.size(); ++i) P->stc.push_back(plan_group(stc[i], n, i == 0 ? P->f_stc : 1.0));
+  }
+  // plaintexts: encode v at `level` limbs with scale `scale`
+  const std::vector<uint64_t> slot_t = [&] {
+    std::vector<uint64_t> t(n);
+    uint64_t g = 1;
+    for (uint32_t j = 0; j < n; ++j) {
+      t[j] = (g - 1) / 2;
+      g = g * 5 % (2ull * N);
+    }
+    return t;
+  }();
+  auto encode = [&](const Vec& z, uint32_t level, double scale) {
+    std::vector<cd> v(N);
+    for (uint32_t j = 0; j < n; ++j) {
+      v[slot_t[j]] = z[j];
+      v[N - 1 - slot_t[j]] = std::conj(z[j]);
+    }
+    fft(v);
+    std::vector<uint64_t> host((size_t)level * N);
+    for (uint32_t i = 0; i < N; ++i) {
+      const double a = -M_PI * (double)i / (double)N;
+      const cd m = v[i] / (double)N * cd(std::cos(a), std::sin(a));
+      const long double c = std::nearbyint((long double)(m.real() * scale));
+      for (uint32_t j = 0; j < level; ++j) {
+        long double r = std::fmod(c, (long double)q[j]);
+        if (r < 0) r += (long double)q[j];
+        host[(size_t)j * N + i] = (uint64_t)r;
+      }
+    }
+    return upload_pt(ctx, N, host, level, scale);
+  };
+  auto group_pts = [&](const GroupPlan& G, uint32_t level) {
+    GroupPts out;
+    const double scale = (double)q[level - 1];
+    for (const auto& kv : G.giants)
+      for (const auto& bv : kv.second) out[kv.first].emplace_back(bv.first, encode(bv.second, level, scale));
+    return out;
+  };
+  P->cts_pts.clear();
+  P->stc_pts.clear();
+  uint32_t l = P->l_raise;
+  for (const GroupPlan& G : P->cts) P->cts_pts.push_back(group_pts(G, l--));
+  P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l);
+  l -= P->evalmod_depth;
+  P->plus_i = monomial_pt(ctx, N, q, N / 2, l);
+  for (const GroupPlan& G : P->stc) P->stc_pts.push_back(group_pts(G, l--));
+  P->rotations = plan_rotations(*P);
+  boot = P;
+  return P->rotations;
+}
+
+std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapPrecomputation& pre) {
+  auto P = std::make_shared<BootPlan>();
+  P->cfg = pre.cfg;
+  P->cfg.budgetCtS = (uint32_t)pre.coeffToSlot.size();
+  P->cfg.budgetStC = (uint32_t)pre.slotToCoeff.size();
+  P->n = N / 2;
+  P->cheb = pre.chebyshev;
+  if (P->cheb.size() < 2) throw std::runtime_error("EvalBootstrapSetup: the EvalMod series needs degree >= 1");
+  P->l_raise = L;
+  P->evalmod_depth = chebyshev_eval_depth(P->cheb) + P->cfg.doubleAngle;
+  P->depth = P->cfg.budgetCtS + P->cfg.budgetStC + P->evalmod_depth;
+  if (P->l_raise <= P->depth) throw std::runtime_error("EvalBootstrapSetup: not enough levels for bootstrapping");
+  P->out_scale = pre.outScale;
+  P->flexible = pre.outScale > 0;
+  auto load = [&](const std::vector<BootstrapPrecomputation::Group>& groups, uint32_t level0,
+                  std::vector<GroupPlan>& plans, std::vector<GroupPts>& pts) {
+    uint32_t level = level0;
+    for (const auto& G : groups) {
+      GroupPlan g;
+      g.stride = (uint32_t)G.stride;
+      g.g = (uint32_t)G.g;
+      g.babies = G.babies;
+      g.n = P->n;
+      GroupPts gp;
+      for (const auto& t : G.terms) {
+        if (t.level != level || t.coef.size() != (size_t)level * N)
+          throw std::runtime_error("EvalBootstrapSetup: plaintext level or size does not match its group");
+        for (uint32_t j = 0; j < level; ++j)
+          for (uint32_t i = 0; i < N; ++i)
+            if (t.coef[(size_t)j * N + i] >= q[j]) throw std::runtime_error("EvalBootstrapSetup: residue not reduced");
+        gp[t.giant].emplace_back(t.baby, upload_pt(ctx, N, t.coef, level, t.scale));
+      }
+      plans.push_back(g);
+      pts.push_back(std::move(gp));
+      --level;
+    }
+  };
+  load(pre.coeffToSlot, P->l_raise, P->cts, P->cts_pts);
+  const uint32_t l_split = P->l_raise - P->cfg.budgetCtS;
+  P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l_split);
+  const uint32_t l_stc = l_split - P->evalmod_depth;
+  P->plus_i = monomial_pt(ctx, N, q, N / 2, l_stc);
+  load(pre.slotToCoeff, l_stc, P->stc, P->stc_pts);
+  P->rotations = plan_rotations(*P);
+  boot = P;
+  return P->rotations;
+}
+
+Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t, uint32_t) {
+  if (!boot) throw std::runtime_error("EvalBootstrap: call EvalBootstrapSetup first");
+  const Ciphertext ct = Regular(ct0);
+  if (!evalMultKey) throw std::runtime_error("EvalBootstrap: relinearization key not loaded");
+  const BootPlan& P = *boot;
+  if (P.cfg.encapsulation && (!encapDown || !encapUp))
+    throw std::runtime_error("EvalBootstrap: sparse-secret encapsulation keys not loaded (AddEncapsulationKeys)");
+  Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
+  // the input as a register (its nominal scale is the tracked scaling factor)
+  Reg x;
+  x.buf = std::make_shared<DevBuf>((size_t)2 * ct->limbs * N);
+  chk(fides_copy_device(x.buf->d, ct->DeviceData(), (size_t)2 * ct->limbs * N * 8, nullptr), "copy");
+  x.level = ct->limbs;
+  x.scale = ct->NoiseFactor > 0 ? ct->NoiseFactor : boot_delta();
+  if (x.level < 2) throw std::runtime_error("EvalBootstrap: the input needs at least 2 limbs");
+  const Reg out = boot_run(X, P, x, encapDown, encapUp);
   Ciphertext o = NewCiphertext(out.level);
   chk(fides_copy_device(o->DeviceData(), out.buf->d, (size_t)2 * out.level * N * 8, nullptr), "copy");
   chk(fides_stream_sync(nullptr), "sync");

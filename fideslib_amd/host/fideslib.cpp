@@ -92,6 +92,8 @@ CryptoContext CryptoContextImpl::Create(const RawParams& P) {
 
 CryptoContextImpl::~CryptoContextImpl() {
   if (evalMultKey) fides_key_destroy(evalMultKey);
+  if (encapDown) fides_key_destroy(encapDown);
+  if (encapUp) fides_key_destroy(encapUp);
   for (auto& kv : rotationKeys) fides_key_destroy(kv.second);
   if (ctx) fides_context_destroy(ctx);
 }
@@ -148,6 +150,21 @@ void CryptoContextImpl::AddAutomorphismKey(uint32_t a, const RawKeySwitchKey& k)
   auto it = rotationKeys.find(a);
   if (it != rotationKeys.end()) fides_key_destroy(it->second);
   rotationKeys[a] = upload(ctx, k, dnum, L + K, N);
+}
+
+void CryptoContextImpl::AddEncapsulationKeys(const RawKeySwitchKey& down, const RawKeySwitchKey& up) {
+  fides_key* d = upload(ctx, down, dnum, L + K, N);
+  fides_key* u = nullptr;
+  try {
+    u = upload(ctx, up, dnum, L + K, N);
+  } catch (...) {
+    fides_key_destroy(d);
+    throw;
+  }
+  if (encapDown) fides_key_destroy(encapDown);
+  if (encapUp) fides_key_destroy(encapUp);
+  encapDown = d;
+  encapUp = u;
 }
 
 void CryptoContextImpl::AddEvalMultKey(const RawKeySwitchKey& k) {

@@ -74,6 +74,8 @@ struct DevPt {
 };
 
 // The program builder/executor over one ciphertext (mirrors serve.Program).
+// With `dry` set no device work is issued: registers carry levels and
+// nominal scales only (the structure of a program, e.g. its output scale).
 struct Exec {
   CryptoContextImpl& cc;
   fides_context* ctx;
@@ -81,11 +83,12 @@ struct Exec {
   uint32_t N;
   double delta;
   const fides_key* rlk;
+  bool dry = false;
 
   Reg fresh(uint32_t level, double scale) {
     if (level < 1) throw std::runtime_error("out of levels");
     Reg r;
-    r.buf = std::make_shared<DevBuf>((size_t)2 * level * N);
+    if (!dry) r.buf = std::make_shared<DevBuf>((size_t)2 * level * N);
     r.level = level;
     r.scale = scale;
     return r;
@@ -104,6 +107,7 @@ struct Exec {
   Reg drop(const Reg& a, uint32_t level) {
     if (level == a.level) return a;
     Reg o = fresh(level, a.scale);
+    if (dry) return o;
     const uint64_t* in = a.buf->d;
     uint64_t* out = o.buf->d;
     chk(fides_drop_limbs_batch(ctx, 1, &in, &a.level, &level, &out, nullptr), "drop");
@@ -115,17 +119,21 @@ struct Exec {
     b = drop(b, l);
     if (l < 2) throw std::runtime_error("HMult+Rescale needs 2 limbs");
     Reg o = fresh(l - 1, a.scale * b.scale / (double)q[l - 1]);
+    if (dry) return o;
     const uint64_t* pa = a.buf->d;
     const uint64_t* pb = b.buf->d;
     uint64_t* po = o.buf->d;
     chk(fides_hmult_batch(ctx, 1, &pa, &pb, &l, rlk, 1, &po, nullptr), "HMult");
     return o;
   }
-  Reg automorph(const Reg& a, uint32_t k) {
+  // automorphism k with `key` (the rotation key of k by default; k = 1 with
+  // a switching key: a key switch to another secret)
+  Reg automorph(const Reg& a, uint32_t k, const fides_key* key = nullptr) {
     Reg o = fresh(a.level, a.scale);
+    if (dry) return o;
     const uint64_t* in = a.buf->d;
     uint64_t* out = o.buf->d;
-    const fides_key* key = cc.RotationKeyFor(k);
+    if (!key) key = cc.RotationKeyFor(k);
     chk(fides_rotate_batch(ctx, 1, &in, &a.level, &key, &k, &out, nullptr), "rotate");
     return o;
   }
@@ -136,6 +144,10 @@ struct Exec {
     return automorph(a, (uint32_t)powmod(5, (uint64_t)rr, 2ull * N));
   }
   Reg conjugate(const Reg& a) { return automorph(a, 2 * N - 1); }
+  Reg switch_key(const Reg& a, const fides_key* key) {
+    if (!key && !dry) throw std::runtime_error("switching key not loaded");
+    return automorph(a, 1, key);
+  }
   // out = a*mul (+/-) b + add (integers)
   Reg lin(Reg a, const i128* mul, const Reg* b0, bool sub, const i128* add, double scale) {
     Reg b;
@@ -145,6 +157,7 @@ struct Exec {
       b = drop(*b0, l);
     }
     Reg o = fresh(a.level, scale);
+    if (dry) return o;
     const uint32_t l = a.level;
     std::vector<uint64_t> mr, ar;
     if (mul) mr = residues(*mul, l);
@@ -163,6 +176,7 @@ struct Exec {
   Reg rescale(const Reg& a) {
     if (a.level < 2) throw std::runtime_error("Rescale needs 2 limbs");
     Reg o = fresh(a.level - 1, a.scale / (double)q[a.level - 1]);
+    if (dry) return o;
     const uint64_t* in = a.buf->d;
     uint64_t* out = o.buf->d;
     chk(fides_rescale_batch(ctx, 1, &in, &a.level, &out, nullptr), "Rescale");
@@ -183,6 +197,7 @@ struct Exec {
     uint32_t l = ~0u;
     for (const auto& t : terms) l = std::min(l, t.first.level);
     Reg o = fresh(l, scale);
+    if (dry) return o;
     const uint32_t cnt = (uint32_t)terms.size();
     std::vector<const uint64_t*> xs;
     std::vector<uint32_t> xl;
@@ -210,6 +225,7 @@ struct Exec {
       if (t.second->level < l) throw std::runtime_error("plaintext below the ciphertext level");
     }
     Reg o = fresh(l, regs[0].scale * terms[0].second->scale);
+    if (dry) return o;
     const uint32_t cnt = (uint32_t)terms.size();
     std::vector<const uint64_t*> cs, ps;
     for (size_t i = 0; i < terms.size(); ++i) {
@@ -223,6 +239,7 @@ struct Exec {
   Reg mod_raise(Reg a, uint32_t level, double scale) {
     a = drop(a, 1);
     Reg o = fresh(level, scale);
+    if (dry) return o;
     const uint64_t* in = a.buf->d;
     uint64_t* out = o.buf->d;
     chk(fides_mod_raise_batch(ctx, 1, &in, &level, &out, nullptr), "ModRaise");

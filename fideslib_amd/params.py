@@ -154,3 +154,15 @@ def boot_small() -> ParamSet:
     """N=2^12 with the headline prime sizes (60 + 21x59 bits, dnum 3): deep
     enough for a full bootstrap at small N (bootstrap tests)."""
     return ckks_params(12, 60, 59, 21, dnum=3, name="boot_N2^12_L22_dnum3")
+
+
+def flexible_scaling_factors(q) -> list:
+    """OpenFHE's FLEXIBLEAUTO scaling factor per level (index = limbs - 1), as
+    the reference copies them into RawParams (RawCiphertext.cu:260-272):
+    sf[L-1] = q[L-1], sf[l-1] = sf[l]^2 / q[l]."""
+    L = len(q)
+    sf = [0.0] * L
+    sf[L - 1] = float(q[L - 1])
+    for l in range(L - 1, 0, -1):
+        sf[l - 1] = sf[l] * sf[l] / float(q[l])
+    return sf

@@ -98,6 +98,34 @@ struct BootstrapConfig {
   uint32_t doubleAngle = 3;
   double K = 32.0;
   uint32_t correction = 10;
+  // Sparse-secret encapsulation (reference Bootstrap.cu:254-260, 592-600):
+  // the one-limb ciphertext is switched to a sparser secret before ModRaise
+  // and back right after it, with the keys of AddEncapsulationKeys; K can
+  // then be 16 for a weight-32 sparse secret.
+  bool encapsulation = false;
+};
+
+// Bootstrapping tables computed elsewhere (the reference takes them from
+// OpenFHE, src/CKKS/BootstrapPrecomputation.cuh fed by RawCiphertext.cu:
+// 488-631): for each linear-transform group, in application order, its
+// baby-step / giant-step shape and plaintexts; the EvalMod Chebyshev series
+// sum_k c_k T_k of the cosine; and the output's nominal scale.
+struct BootstrapPrecomputation {
+  BootstrapConfig cfg;  // budgets (= group counts), doubleAngle, correction, encapsulation
+  struct Term {
+    int64_t giant = 0, baby = 0;  // rotation k g stride of the product of baby b
+    uint32_t level = 0;           // limbs of the plaintext
+    double scale = 0;             // its encoding scale (q_top of the level: the Rescale divides it out)
+    std::vector<uint64_t> coef;   // [level][N] coefficient-form residues mod q_0..q_{level-1}
+  };
+  struct Group {
+    int64_t stride = 1, g = 1;
+    std::vector<int64_t> babies;  // b with baby rotation b stride
+    std::vector<Term> terms;      // giant ascending, then baby order
+  };
+  std::vector<Group> coeffToSlot, slotToCoeff;
+  std::vector<double> chebyshev;
+  double outScale = 0;  // nominal scale of the result (0: the tracked one)
 };
 struct BootPlan;
 
@@ -337,8 +365,22 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   // plaintexts and the modular-reduction series and returns the rotation
   // indices whose keys EvalBootstrap needs (plus the conjugation key,
   // automorphism index 2N-1).  The result has limbs - depth limbs.
+  // With FLEXIBLEAUTO(EXT) the result carries the FLEXIBLEAUTO scaling factor
+  // of its level (folded into the first SlotToCoeff group), so it adds to
+  // fresh ciphertexts of that level without a scale correction.
   std::vector<int32_t> EvalBootstrapSetup(const BootstrapConfig& cfg = BootstrapConfig());
-  Ciphertext EvalBootstrap(const Ciphertext& ct);
+  // The same from precomputed tables (the reference's path from OpenFHE's
+  // precomputation); returns the rotation indices the groups need.
+  std::vector<int32_t> EvalBootstrapSetup(const BootstrapPrecomputation& pre);
+  // Switching keys of the sparse-secret encapsulation: s -> s' and s' -> s
+  // (GetSecretSwitchingKey, reference Bootstrap.cu:259, 596).
+  void AddEncapsulationKeys(const RawKeySwitchKey& toSparse, const RawKeySwitchKey& fromSparse);
+  // numIterations and precision are accepted as the reference's GPU path
+  // accepts them (api/CryptoContext.cpp:1719-1745): one iteration runs.
+  Ciphertext EvalBootstrap(const Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0);
+  void EvalBootstrapInPlace(Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0) {
+    ct = EvalBootstrap(ct, numIterations, precision);
+  }
   // The key-switching key loaded for an automorphism index (throws if absent).
   const fides_key* RotationKeyFor(uint32_t autoIndex) const { return RotationKey(autoIndex); }
 
@@ -383,6 +425,8 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   std::vector<double> modReduceFactor;
   std::vector<double> sfReal, sfBig;  // by level = limbs - 1
   fides_key* evalMultKey = nullptr;
+  fides_key* encapDown = nullptr;  // s -> s' (sparse-secret encapsulation)
+  fides_key* encapUp = nullptr;    // s' -> s
   std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
   std::shared_ptr<BootPlan> boot;                // EvalBootstrapSetup
   // the nominal scale of a fresh ciphertext: 2^(bit length of q_1, rounded)

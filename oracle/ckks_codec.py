@@ -98,7 +98,8 @@ def _slot_index(N: int) -> np.ndarray:
 
 def encode(z: np.ndarray, scale: float, N: int) -> np.ndarray:
     """Slots (N/2 complex) -> rounded integer coefficients (int64) of
-    m(X) with m(zeta^(5^j)) ~= scale * z_j, zeta = exp(i pi / N)."""
+    m(X) with m(zeta^(5^j)) ~= scale * z_j, zeta = exp(i pi / N).  Past
+    2^62 the coefficients come back as Python ints (object array)."""
     z = np.asarray(z, np.complex128)
     n = N // 2
     full = np.zeros(n, np.complex128)
@@ -114,8 +115,10 @@ def encode(z: np.ndarray, scale: float, N: int) -> np.ndarray:
     w = np.fft.fft(v) / N
     i = np.arange(N)
     zeta_inv = np.exp(-1j * np.pi * i / N)
-    m = (w * zeta_inv).real * scale
-    return np.rint(m).astype(np.int64)
+    m = np.rint((w * zeta_inv).real * scale)
+    if np.max(np.abs(m)) >= 2.0 ** 62:  # big scaling factors (FLEXIBLEAUTOEXT top level)
+        return np.array([int(x) for x in m], dtype=object)
+    return m.astype(np.int64)
 
 
 def decode(coeffs, scale: float, N: int, slots: int | None = None) -> np.ndarray:
@@ -183,7 +186,11 @@ class Keys:
         pids = np.arange(l, dtype=np.uint32)
         a = sample_uniform(self.rng, qs, ctx.N)
         e = sample_error(self.rng, ctx.N)
-        me = ctx.ntt_limbs(to_rns(m_coef.astype(np.int64) + e, qs), pids)
+        if m_coef.dtype == object:
+            coef = to_rns_big([int(x) + int(y) for x, y in zip(m_coef, e)], qs)
+        else:
+            coef = to_rns(m_coef.astype(np.int64) + e, qs)
+        me = ctx.ntt_limbs(coef, pids)
         as_ = ctx.mul_limbs(a, self.s_eval[:l], pids)
         c0 = ctx.sub_limbs(me, as_, pids)
         return np.stack([c0, a])

@@ -20,6 +20,11 @@
 //   cheb <out> <a> <lo> <hi> <c0> <c1> ...
 //   setlevel <out> <a> <level>
 //   chebcoef <name> <exp|sigmoid|sin> <a> <b> <degree>  -> name.txt (GetChebyshevCoefficients)
+//   encapkeys <fileDown> <fileUp>       sparse-secret encapsulation keys
+//   bootsetup <ctsBudget> <stcBudget> <degree> <doubleAngle> <K> <correction> <encaps>
+//   bootsetup_pre <file>                EvalBootstrapSetup from a precomputation file
+//                                       (both write "rots ..." to meta.txt)
+//   boot <out> <a>                      EvalBootstrap
 //   store <name>                        -> name.bin [2][limbs][N], line in meta.txt:
 //                                          name limbs deg noiseFactor slots extended
 //   expect_throw <command...>           the command must throw (line "throws" in meta.txt)
@@ -28,6 +33,7 @@
 #include <cstdio>
 #include <functional>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <iomanip>
 #include <map>
@@ -207,6 +213,65 @@ struct State {
       std::ofstream o(g_dir + "/" + w.at(1) + ".txt");
       o << std::setprecision(17);
       for (double v : c) o << v << "\n";
+    } else if (op == "encapkeys") {
+      cc->AddEncapsulationKeys(key(w.at(1)), key(w.at(2)));
+    } else if (op == "bootsetup") {
+      BootstrapConfig c;
+      c.budgetCtS = (uint32_t)I(1);
+      c.budgetStC = (uint32_t)I(2);
+      c.degree = (uint32_t)I(3);
+      c.doubleAngle = (uint32_t)I(4);
+      c.K = D(5);
+      c.correction = (uint32_t)I(6);
+      c.encapsulation = I(7) != 0;
+      const auto rots = cc->EvalBootstrapSetup(c);
+      meta << "rots";
+      for (int32_t r : rots) meta << " " << r;
+      meta << "\n";
+    } else if (op == "bootsetup_pre") {
+      // [ncts nstc ncheb correction doubleAngle encapsulation outScale] cheb[ncheb]
+      // then per group: stride g nbabies babies[] nterms, per term: giant baby level scale coef[level N]
+      const auto b = read_u64(w.at(1));
+      size_t o = 0;
+      auto u = [&]() { return b.at(o++); };
+      auto d = [&]() {
+        double x;
+        const uint64_t v = b.at(o++);
+        std::memcpy(&x, &v, 8);
+        return x;
+      };
+      BootstrapPrecomputation pre;
+      const uint64_t ncts = u(), nstc = u(), ncheb = u();
+      pre.cfg.correction = (uint32_t)u();
+      pre.cfg.doubleAngle = (uint32_t)u();
+      pre.cfg.encapsulation = u() != 0;
+      pre.outScale = d();
+      for (uint64_t i = 0; i < ncheb; ++i) pre.chebyshev.push_back(d());
+      for (uint64_t gi = 0; gi < ncts + nstc; ++gi) {
+        BootstrapPrecomputation::Group g;
+        g.stride = (int64_t)u();
+        g.g = (int64_t)u();
+        const uint64_t nb = u();
+        for (uint64_t i = 0; i < nb; ++i) g.babies.push_back((int64_t)u());
+        const uint64_t nt = u();
+        for (uint64_t i = 0; i < nt; ++i) {
+          BootstrapPrecomputation::Term t;
+          t.giant = (int64_t)u();
+          t.baby = (int64_t)u();
+          t.level = (uint32_t)u();
+          t.scale = d();
+          t.coef.assign(b.begin() + o, b.begin() + o + (size_t)t.level * N);
+          o += (size_t)t.level * N;
+          g.terms.push_back(std::move(t));
+        }
+        (gi < ncts ? pre.coeffToSlot : pre.slotToCoeff).push_back(std::move(g));
+      }
+      const auto rots = cc->EvalBootstrapSetup(pre);
+      meta << "rots";
+      for (int32_t r : rots) meta << " " << r;
+      meta << "\n";
+    } else if (op == "boot") {
+      ct[w.at(1)] = cc->EvalBootstrap(C(w.at(2)));
     } else if (op == "setlevel") {
       Ciphertext a = C(w.at(2));
       CryptoContextImpl::SetLevel(a, (size_t)I(3));

@@ -40,6 +40,23 @@ class Script:
         self.lines.append("pt %s %s %d %d %d %r" % (name, self._file(limbs_data), limbs_data.shape[0], fmt, deg,
                                                     float(sf)))
 
+    def encapkeys(self, down, up):
+        self.lines.append("encapkeys %s %s" % (self._file(down), self._file(up)))
+
+    def bootsetup_pre(self, pre, cfg):
+        """Write BootstrapPlan.export_precomputation() `pre` (config `cfg`)."""
+        words = [len(pre["cts"]), len(pre["stc"]), len(pre["cheb"]), cfg.correction, cfg.double_angle,
+                 1 if cfg.encaps_h else 0]
+        f64 = lambda x: int(np.array([x], np.float64).view(np.uint64)[0])
+        parts = [np.asarray(words + [f64(pre["out_scale"])] + [f64(c) for c in pre["cheb"]], np.uint64)]
+        for g in pre["cts"] + pre["stc"]:
+            parts.append(np.asarray([g["stride"], g["g"], len(g["babies"])] +
+                                    [b % (1 << 64) for b in g["babies"]] + [len(g["terms"])], np.uint64))
+            for k, b, level, scale, coef in g["terms"]:
+                parts.append(np.asarray([k % (1 << 64), b % (1 << 64), level, f64(scale)], np.uint64))
+                parts.append(np.ascontiguousarray(coef, np.uint64).ravel())
+        self.lines.append("bootsetup_pre " + self._file(np.concatenate(parts)))
+
     def op(self, *words):
         self.lines.append(" ".join(repr(w) if isinstance(w, float) else str(w) for w in words))
 
@@ -59,6 +76,8 @@ class Script:
             w = line.split()
             if w[0] == "sf":
                 out["sf"] = [float(x) for x in w[1:]]
+            elif w[0] == "rots":
+                out["rots"] = [int(x) for x in w[1:]]
             elif w[0] == "throws":
                 out["throws"].append(w[1])
             else:

@@ -7,8 +7,8 @@
 * N = 2^12, dense ternary secret, sparse-secret encapsulation (weight-32
   s'): bit-exact against the oracle and > 20 bits.
 * N = 2^16 (30 limbs, h = 192, the headline primes): two bootstraps served
-  together decrypt to their messages, without and with sparse-secret
-  encapsulation; the precision is printed.
+  together decrypt to their messages: >= 19 bits with sparse-secret
+  encapsulation, >= 17 without.
 """
 import time
 
@@ -134,4 +134,5 @@ def test_bootstrap_cfg3_precision(encaps):
         dec = C.decode(keys.decrypt_coef(r.result, P.output.level), P.output.scale, N)
         bits = C.precision_bits(dec, z)
         say("N=2^16 bootstrap from level %d: %.1f bits of precision" % (P.inputs[0].level, bits))
-        assert bits > 15, bits
+        # measured 19.4-19.7 bits with encapsulation, 17.6-17.8 without
+        assert bits >= (19.0 if encaps else 17.0), bits

@@ -144,3 +144,35 @@ def test_bootstrap_planted_mistake_is_caught():
     # K' = 4 cannot cover |I| up to ~8 for this secret: the modular reduction fails
     bits, _ = _oracle_bootstrap(B.BootConfig(K=4.0, degree=31))
     assert bits < 5, bits
+
+
+def test_flexible_plan_lands_on_the_flexibleauto_factor():
+    """BootConfig(flexible=True): the program's result carries the
+    FLEXIBLEAUTO scaling factor of its level and decrypts to the message at
+    it (oracle run of the whole program at N = 2^12, sparse-secret
+    encapsulation, input at its own FLEXIBLEAUTO factor)."""
+    import serve_oracle as SO
+    from fideslib_amd import params as PR
+    from oracle import ckks_codec as C
+    from oracle import oracle as O
+    prm = PR.boot_small()
+    N = prm.N
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, 8, h=None)
+    bp = B.BootstrapPlan(prm, None, B.BootConfig.encapsulated(flexible=True), keep_host=True)
+    sf = PR.flexible_scaling_factors(prm.q)
+    assert bp.out_scale == sf[bp.output_level() - 1]
+    rot = {}
+    for r in bp.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        rot[k] = keys.rotation_key(k)
+    rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP] = keys.encapsulation_keys(32)
+    rng = np.random.default_rng(21)
+    z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    lvl = 5
+    ct = keys.encrypt(C.encode(z, sf[lvl - 1], N), lvl)
+    P = bp.program(lvl, sf[lvl - 1])
+    assert P.output.scale == sf[bp.output_level() - 1]
+    out = SO.run_oracle(octx, P, [ct], keys.relin_key(), rot)
+    dec = C.decode(keys.decrypt_coef(out, P.output.level), P.output.scale, N)
+    assert C.precision_bits(dec, z) > 20

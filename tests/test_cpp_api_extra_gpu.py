@@ -57,8 +57,15 @@ def _inputs(prm, octx, keys, rlk, fl, rng, spec):
     for name, (l, deg) in spec.items():
         # FLEXIBLEAUTOEXT: a fresh ciphertext at the top level carries the big
         # factor (the level ElemForEvalAddOrSub treats apart, Context.cu:477-481)
-        sf = fl.sfbig[l - 1] if (fl.tech == F.FLEXIBLEAUTOEXT and l == fl.L) else fl.sf[l - 1]
+        ext_top = fl.tech == F.FLEXIBLEAUTOEXT and l == fl.L
+        sf = fl.sfbig[l - 1] if ext_top else fl.sf[l - 1]
         z = [rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2) for _ in range(deg)]
+        if ext_top and deg == 2:
+            # sfBig = sf^2 is already the degree-2 factor of that level: the
+            # message encrypted there directly (a product of two would carry sf^4)
+            zz = z[0] * z[1]
+            out[name] = (F.Ct(keys.encrypt(C.encode(zz, sf, N), l), 2, sf), zz)
+            continue
         cts = [keys.encrypt(C.encode(v, sf, N), l) for v in z]
         if deg == 1:
             out[name] = (F.Ct(cts[0], 1, sf), z[0])
