@@ -394,6 +394,53 @@ def cpu_baseline(prm, budget_s=12.0):
     }
 
 
+def bootstrap_precision_check(prm, ctx, w, boot, levels=(3, 17), h=192, seed=SEED + 11):
+    """Part of the CPU baseline leg, with the oracle as checker only: it
+    generates a real secret (ternary, Hamming weight h), its relinearization,
+    rotation and encapsulation keys, encrypts a random full-slot message at
+    each level's FLEXIBLEAUTO factor, bootstraps it on the GPU with the same
+    plan as the timed leg (one request each through the request loop), and
+    decrypts.  Reports bits of precision: -log2 max |decoded - message|."""
+    from fideslib_amd import serve as S
+    from oracle import ckks_codec as C
+    from oracle import oracle as O
+    N = prm.N
+    t0 = time.time()
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    keys = C.Keys(octx, seed, h=h)
+    host = {}
+    for r in boot.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        host[k] = keys.rotation_key(k)
+    host[boot.ENCAPS_DOWN], host[boot.ENCAPS_UP] = keys.encapsulation_keys(boot.cfg.encaps_h)
+    dev = {k: ctx.upload_key(v) for k, v in host.items()}
+    rlk = ctx.upload_key(keys.relin_key())
+    del host
+    t_keys = time.time() - t0
+    be = S.GpuBackend(ctx, rlk, dev, 8 << 30)
+    rng = np.random.default_rng(seed)
+    bits = {}
+    for lvl in levels:
+        z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+        sc = boot_input_scale(prm, boot, lvl)
+        P = boot.program(lvl, sc)
+        ct = keys.encrypt(C.encode(z, sc if sc is not None else 2.0 ** prm.scale_bits, N), lvl)
+        req = S.Request(0, P, 0.0, inputs=[ct], keep_result=True)
+        S.Server(be, N).run([req], realtime=False)
+        if req.status != "done":
+            raise RuntimeError("bootstrap precision check: request %s" % req.status)
+        dec = C.decode(keys.decrypt_coef(req.result, P.output.level), P.output.scale, N)
+        bits[lvl] = round(float(C.precision_bits(dec, z)), 2)
+    del be
+    for k in list(dev):
+        dev.pop(k).close()
+    rlk.close()
+    return {"min_bits": min(bits.values()), "bits_by_input_limbs": bits,
+            "secret": "ternary, Hamming weight %d; encapsulation h'=%d" % (h, boot.cfg.encaps_h),
+            "message": "uniform real and imaginary parts in [-1, 1), 2^15 slots",
+            "keys_s": round(t_keys, 1), "total_s": round(time.time() - t0, 1)}
+
+
 def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
     """Summarise a rocprofv3 run of this script: per-kernel time of the
     batched step, the roofline launch's kernels, (with the two --pmc runs) the
@@ -597,18 +644,37 @@ def bench_keys(prm, w, boot):
     return {k: w.rot_keys[i] for i, k in enumerate(idx)}
 
 
+def boot_plan(prm, ctx):
+    """The cfg4 bootstrap: FLEXIBLEAUTO, sparse-secret encapsulation (h'=32)."""
+    from fideslib_amd import bootstrap as BT
+    boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated(flexible=True))
+    boot.build_plaintexts()
+    return boot
+
+
+def boot_input_scale(prm, boot, level):
+    """A fresh ciphertext's scale at `level` limbs (its FLEXIBLEAUTO factor)."""
+    if not boot.cfg.flexible:
+        return None
+    return params.flexible_scaling_factors(prm.q)[level - 1]
+
+
 def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
-    """n_boot full-slot bootstraps (N=2^16, 30 limbs -> 14 limbs) served
-    together by the request loop; synthetic keys and ciphertexts (precision
-    is measured by tests/test_bootstrap_gpu.py with real keys)."""
+    """n_boot full-slot bootstraps (N=2^16, 30 limbs -> 14 limbs, FLEXIBLEAUTO
+    factors, sparse-secret encapsulation) served together by the request
+    loop; synthetic keys and ciphertexts (values do not affect timing).
+    Precision is measured with real keys by bootstrap_precision_check."""
     from fideslib_amd import serve as S
     N = prm.N
     rng = np.random.default_rng(SEED + 9)
     be = S.GpuBackend(ctx, w.rlk, keys, 16 << 30)
     src = capi.DeviceBuffer.from_array(np.stack([rand_limbs(rng, prm.q[:3], N) for _ in range(2)]))
 
+    lvl = 3
+    P3 = boot.program(lvl, boot_input_scale(prm, boot, lvl))
+
     def reqs(n):
-        return [S.Request(i, boot.program(3), 0.0, inputs=[("dev", src.ptr)]) for i in range(n)]
+        return [S.Request(i, P3, 0.0, inputs=[("dev", src.ptr)]) for i in range(n)]
 
     S.Server(be, N).run(reqs(1), realtime=False)   # warm-up
     ctx.sync()
@@ -617,7 +683,7 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
     wall = srv.stats["wall_s"]
     # the same batch captured into one HIP graph, replayed (timed with events on its stream)
     stream = capi.Stream()
-    foot = S.Server(be, N).footprint_bytes(boot.program(3))
+    foot = S.Server(be, N).footprint_bytes(P3)
     gbe = S.GpuBackend(ctx, w.rlk, keys, n_boot * foot + (256 << 20), stream=stream)
     gsrv = S.Server(gbe, N)
     cap = gsrv.capture(reqs(n_boot), scratch_bytes=40 << 30, task_bytes=1 << 30)
@@ -636,17 +702,17 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
     del gsrv, gbe
     stream.close()
     return {"workload": "cfg4: %d full-slot bootstraps (2^15 slots) captured in one HIP graph, N=2^16, 30 limbs "
-                        "(60+29x59 bit), dnum=3, level budget {3,3}, sparse-secret encapsulation (h'=%s, "
-                        "K'=%g), EvalMod Chebyshev degree %d (baby-step giant-step) + %d double angles, output %d limbs"
-                        % (n_boot, boot.cfg.encaps_h, boot.cfg.K, boot.cfg.degree, boot.cfg.double_angle,
-                           boot.output_level()),
+                        "(60+29x59 bit), dnum=3, level budget {3,3}, %s scaling, sparse-secret encapsulation "
+                        "(h'=%s, K'=%g), EvalMod Chebyshev degree %d (baby-step giant-step) + %d double angles, "
+                        "input %d limbs, output %d limbs"
+                        % (n_boot, "FLEXIBLEAUTO" if boot.cfg.flexible else "FIXEDMANUAL", boot.cfg.encaps_h,
+                           boot.cfg.K, boot.cfg.degree, boot.cfg.double_angle, lvl, boot.output_level()),
             "bootstraps_per_s": round(n_boot * 1e3 / graph_ms, 3), "ms_per_bootstrap": round(graph_ms / n_boot, 3),
             "graph_ms": round(graph_ms, 3), "graph_launches": launches, "replays": reps,
             "request_loop_bootstraps_per_s": round(n_boot / wall, 3),
             "keyswitch_ops": srv.stats["keyswitch_ops"], "steps": srv.stats["steps"],
             "launches": srv.stats["launches"],
-            "precision_bits": "19.4-19.7 at N=2^16, h=192 with encapsulation (h'=32); 17.6-17.8 without "
-                              "(tests/test_bootstrap_gpu.py, real keys)"}
+            "precision_bits": None}
 
 
 def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=()):
@@ -674,7 +740,8 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=()):
             elif u < 0.9:
                 P = S.rotsum_program(prm, int(rng.integers(1, prm.L + 1)), int(rng.integers(1, SERVE_KMAX + 1)))
             else:
-                P = boot.program(int(rng.integers(2, prm.L + 1)))
+                lb = int(rng.integers(2, prm.L + 1))
+                P = boot.program(lb, boot_input_scale(prm, boot, lb))
             out.append(S.Request(i, P, float(arrivals[i]), inputs=[("dev", pool[P.inputs[0].level].ptr)]))
         return out
 
@@ -780,15 +847,11 @@ def main():
         tuning = ctx.autotune([prm.L], reps=3)
         print("bench: autotune %s (%.1f s)" % (tuning, time.time() - t0), file=sys.stderr, flush=True)
     if args.boot_only:
-        from fideslib_amd import bootstrap as BT
-        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
-        boot.build_plaintexts()
+        boot = boot_plan(prm, ctx)
         print(json.dumps({"bootstrap": bootstrap_leg(prm, ctx, w, boot, bench_keys(prm, w, boot))}))
         return
     if args.serve_only:
-        from fideslib_amd import bootstrap as BT
-        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
-        boot.build_plaintexts()
+        boot = boot_plan(prm, ctx)
         more = [float(x) for x in args.serve_load_points.split(",") if x.strip()]
         print(json.dumps({"serve": serve_leg(prm, ctx, w, boot, bench_keys(prm, w, boot), args.serve_requests,
                                              args.serve_rate, SEED + 5, more)}))
@@ -876,10 +939,8 @@ def main():
     out["config"]["primitives"] = primitives_leg(ev)
     print("bench: primitives %s" % out["config"]["primitives"], file=sys.stderr, flush=True)
     if not (args.no_serve and args.no_boot):
-        from fideslib_amd import bootstrap as BT
         print("bench: bootstrap plaintexts", file=sys.stderr, flush=True)
-        boot = BT.BootstrapPlan(prm, ctx, BT.BootConfig.encapsulated())
-        boot.build_plaintexts()
+        boot = boot_plan(prm, ctx)
         keys = bench_keys(prm, w, boot)
     if not args.no_boot:
         out["bootstrap"] = bootstrap_leg(prm, ctx, w, boot, keys)
@@ -891,6 +952,13 @@ def main():
     if not args.no_cpu:
         print("bench: cpu baseline", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(prm)
+        if not args.no_boot:
+            # the oracle as checker: real keys, one GPU bootstrap per level, decrypted
+            print("bench: bootstrap precision check (real keys)", file=sys.stderr, flush=True)
+            chk = bootstrap_precision_check(prm, ctx, w, boot)
+            out["bootstrap"]["precision_bits"] = chk["min_bits"]
+            out["bootstrap"]["precision_check"] = chk
+            print("bench: bootstrap precision %s" % chk, file=sys.stderr, flush=True)
     print(json.dumps(out))
 
 
