@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "libckks_oracle.so")
+# FIDES_ORACLE_SANITIZE=1 selects the AddressSanitizer/UBSan build (make sanitize)
+_SANITIZE = os.environ.get("FIDES_ORACLE_SANITIZE") == "1"
+_LIB_PATH = os.path.join(_HERE, "build", "libckks_oracle_san.so" if _SANITIZE else "libckks_oracle.so")
 
 u64p = ctypes.POINTER(ctypes.c_uint64)
 u32p = ctypes.POINTER(ctypes.c_uint32)
@@ -23,7 +25,8 @@ u32p = ctypes.POINTER(ctypes.c_uint32)
 def build(force: bool = False) -> str:
     """Compile the oracle with its Makefile (gcc, OpenMP)."""
     if force or not os.path.exists(_LIB_PATH):
-        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []) + ["sanitize" if _SANITIZE else "all"],
+                       check=True)
     return _LIB_PATH
 
 
