@@ -134,7 +134,7 @@ class Workload:
 ROOF_LIMBS = 960  # 32 limbs for each of the 30 Q primes: the size of the batched step's NTT launches
 ROOF_REPS = 10
 ROOF_LAUNCHES = 2 + ROOF_REPS  # the roofline's warm-up and timed launches, the first of this size in a run
-ROOF_KERNELS = ("void k_ntt_fwd_col<16, 0>", "void k_ntt_fwd_row<16, 0>")
+ROOF_KERNELS = ("void k_ntt_fwd_col<16, 0, 1>", "void k_ntt_fwd_row<16, 0, 1>")  # primes below 2^60
 VALU_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
@@ -193,6 +193,9 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
                     "min_launch_us": round(t_min * 1e6, 2), "frac": round(t_min / (ms * 1e-3), 4),
                     "profiled_busy": {k: v[k]["valu_busy"] for k in ROOF_KERNELS},
                     "profiled_clock_ghz": {k: v[k]["clock_ghz"] for k in ROOF_KERNELS}}
+            w = weighted_valu_bound(prm, ms, {k: v[k]["clock_ghz"] for k in ROOF_KERNELS})
+            if w:
+                valu["weighted"] = w
     return {
         "kernel": "ntt_fwd = k_ntt_fwd_col + k_ntt_fwd_row, %d limbs x 2^16 (one fides_ntt call)" % ROOF_LIMBS,
         # 64-bit modular butterflies on 32-bit multipliers: VALU issue binds,
@@ -208,6 +211,43 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
         "alg_bytes_per_launch": alg_bytes,
         "valu": valu,
     }
+
+
+def weighted_valu_bound(prm, ms, clocks):
+    """VALU issue bound of the roofline launch with every opcode weighted by
+    its measured issue cost: the passes' instruction mix
+    (profiles/ntt_fwd_valu_mix.json, counted in this source's gfx950
+    assembly) times cycles per wave64 instruction
+    (profiles/valu_rates_gfx950.json, the valu_probe microbenchmark; opcodes
+    it does not list take the median of those it does).  Per pass:
+    waves x sum(count x cycles) / (1024 SIMDs x clock), at the clock the
+    profiled run measured for that pass and at the 2.4 GHz peak; frac is the
+    bound over the event-timed launch."""
+    fr = os.path.join(ROOT, "profiles", "valu_rates_gfx950.json")
+    fm = os.path.join(ROOT, "profiles", "ntt_fwd_valu_mix.json")
+    if not (os.path.exists(fr) and os.path.exists(fm)):
+        return None
+    rates = json.load(open(fr))["ops"]
+    mix = json.load(open(fm))["kernels"]
+    if not all(k in mix for k in ROOF_KERNELS):
+        return None
+    default = float(np.median(list(rates.values())))
+    butterflies = ROOF_LIMBS * (prm.N // 2) * (prm.logN // 2)  # per pass: N/2 butterflies x 8 stages
+    cyc, t_prof, t_peak = {}, 0.0, 0.0
+    for k in ROOF_KERNELS:
+        m = mix[k]
+        per_thread = sum(n * rates.get(op, default) for op, n in m["ops"].items())
+        cyc[k] = round(per_thread / m["butterflies_per_thread"], 2)
+        waves = butterflies / m["butterflies_per_thread"] / 64
+        t_prof += waves * per_thread / (1024 * clocks[k] * 1e9)
+        t_peak += waves * per_thread / (1024 * VALU_CLOCK_GHZ * 1e9)
+    return {"cycles_per_butterfly": cyc, "default_cycles": round(default, 3),
+            "min_launch_us_at_profiled_clock": round(t_prof * 1e6, 2),
+            "frac_at_profiled_clock": round(t_prof / (ms * 1e-3), 4),
+            "min_launch_us_at_peak_clock": round(t_peak * 1e6, 2),
+            "frac_at_peak_clock": round(t_peak / (ms * 1e-3), 4),
+            "sources": ["profiles/ntt_fwd_valu_mix.json", "profiles/valu_rates_gfx950.json",
+                        "profiles/ntt_fwd_960x65536.json (clock per pass)"]}
 
 
 def ntt_roofline_cfg2(ev, reps=50):

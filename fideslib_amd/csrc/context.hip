@@ -465,6 +465,9 @@ extern "C" int fides_context_create(const fides_params* P, fides_context** out) 
   c->T.ipsi = c->tw + (size_t)2 * NP * N;
   c->T.ipsi_sh = c->tw + (size_t)3 * NP * N;
   c->T.prm = c->prm;
+  c->T.narrow = 1;
+  for (u32 i = 0; i < NP; ++i)
+    if (c->hprm[i].q >> 60) c->T.narrow = 0;
   // ---- ModDown and rescale scalars
   c->pinv.resize(L);
   c->pinv_sh.resize(L);

@@ -75,6 +75,9 @@ struct NttTables {
   const u64* ipsi;     // [P][N] psi^-brev(j)
   const u64* ipsi_sh;
   const PrimeC* prm;   // [P]
+  // every prime of the context below 2^60: the forward butterflies may let
+  // values grow to 16q between reductions (host-side launch selection)
+  u32 narrow;
 };
 
 // Fused prologue of the forward column pass.

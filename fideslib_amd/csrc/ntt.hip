@@ -31,15 +31,100 @@ struct NttCfg {
 };
 
 // ---------------------------------------------------------------- butterflies
-// Forward Cooley-Tukey, lazy: X, Y in [0, 4q) -> [0, 4q).  WS: twiddles are
-// wave-uniform.
-template <bool WS = false>
-__device__ __forceinline__ void ct_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
-  const u64 x = csub_bf(X, q2);
+// Forward Cooley-Tukey, lazy: X' = x + t, Y' = x + 2q - t with t = Y w mod q
+// in [0, 2q) (any Y < 2^64) and x = X, or with RED x = X - m if X >= m.  So
+// the outputs exceed x's bound by 2q.  WS: twiddles are wave-uniform.
+template <bool WS = false, bool RED = true>
+__device__ __forceinline__ void ct_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2, u64 m) {
+  const u64 x = RED ? csub_bf(X, m) : X;
   const u64 t = shoup_lazy_bf<WS>(Y, w, wp, (u64)0 - q);
   X = x + t;
   Y = x + q2 - t;
 }
+
+// Reduction schedule of the forward transform, a function of the global
+// stage only (so the column pass, the row pass and the fused row pass of the
+// key switch agree).  Values entering global stage s lie below bnd(s) q; the
+// input of stage 0 below 4q.  A stage reduces its X operands only when its
+// outputs could otherwise reach MAXB q, the largest multiple of q that stays
+// below 2^64, by one conditional subtraction of SUB q (which maps
+// [0, 2 SUB q) onto [0, SUB q)):
+//   primes up to 2^62: MAXB = 4, SUB = 2 (Harvey's butterfly, every stage);
+//   primes below 2^60 (NARROW): MAXB = 16, SUB = 8, every fourth stage.
+// The last stage brings X below 2q (subtractions of 8q, 4q, 2q as the bound
+// needs), so the transform ends below 4q and its final reduction to [0, q)
+// takes two subtractions per element instead of four.
+enum : int { RED_NONE = 0, RED_SUB = 1, RED_FULL = 2 };
+
+template <int NARROW, int LOGN>
+struct FwdLazy {
+  static constexpr int MAXB = NARROW ? 16 : 4;
+  static constexpr int SUB = NARROW ? 8 : 2;
+  static constexpr int kind_for(int s, int b) {
+    return (s == LOGN - 1 && b > 2) ? RED_FULL : (b + 2 > MAXB ? RED_SUB : RED_NONE);
+  }
+  static constexpr int after(int s, int b) {
+    const int k = kind_for(s, b);
+    return (k == RED_FULL ? 2 : k == RED_SUB ? (b - SUB > SUB ? b - SUB : SUB) : b) + 2;
+  }
+  static constexpr int bnd(int s) {
+    int b = 4;
+    for (int i = 0; i < s; ++i) b = after(i, b);
+    return b;
+  }
+  // every stage's input fits (b <= MAXB), a single subtraction of SUB q
+  // suffices where one is made (b <= 2 SUB), the outputs fit, and the
+  // transform ends below 4q
+  static constexpr bool valid() {
+    for (int s = 0; s < LOGN; ++s) {
+      const int b = bnd(s), k = kind_for(s, b);
+      if (b > MAXB || (k == RED_SUB && b > 2 * SUB)) return false;
+      const int bx = k == RED_FULL ? 2 : k == RED_SUB ? (b - SUB > SUB ? b - SUB : SUB) : b;
+      if (bx + 2 > MAXB) return false;
+    }
+    return bnd(LOGN) <= 4;
+  }
+};
+static_assert(FwdLazy<0, 16>::valid() && FwdLazy<1, 16>::valid() && FwdLazy<1, 10>::valid() &&
+                  FwdLazy<1, 17>::valid() && FwdLazy<0, 10>::valid(),
+              "reduction schedule out of range");
+
+// One forward butterfly of global stage s under the schedule.  Callers pass
+// the stage from fully unrolled loops, so s is a constant and the branches
+// fold; the schedule enters as a table of booleans (the lookups of an int
+// table of kinds were left to run time).
+template <int NARROW, int LOGN>
+struct FwdSubTable {
+  struct T {
+    bool v[18];
+  };
+  static constexpr T make() {
+    using Lz = FwdLazy<NARROW, LOGN>;
+    T t{};
+    for (int s = 0; s < 18; ++s) t.v[s] = s < LOGN && Lz::kind_for(s, Lz::bnd(s)) == RED_SUB;
+    return t;
+  }
+  static constexpr T tab = make();
+};
+
+template <bool WS, int NARROW, int LOGN>
+__device__ __forceinline__ void fwd_bfly(int s, u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2, u64 msub) {
+  constexpr int bl = FwdLazy<NARROW, LOGN>::bnd(LOGN - 1);  // the last stage reduces fully
+  if (s == LOGN - 1) {
+    if (bl > 8) X = csub_bf(X, q << 3);
+    if (bl > 4) X = csub_bf(X, q << 2);
+    X = csub_bf(X, q2);
+    ct_bfly<WS, false>(X, Y, w, wp, q, q2, msub);
+  } else if (FwdSubTable<NARROW, LOGN>::tab.v[s]) {
+    ct_bfly<WS, true>(X, Y, w, wp, q, q2, msub);
+  } else {
+    ct_bfly<WS, false>(X, Y, w, wp, q, q2, msub);
+  }
+}
+
+// The transform's output (below 4q) -> [0, q).
+__device__ __forceinline__ u64 fwd_final(u64 x, u64 q, u64 q2) { return csub_bf(csub_bf(x, q2), q); }
+
 // Inverse Gentleman-Sande, lazy: X, Y in [0, 2q) -> [0, 2q).
 template <bool WS = false>
 __device__ __forceinline__ void gs_bfly(u64& X, u64& Y, u64 w, u64 wp, u64 q, u64 q2) {
@@ -111,15 +196,16 @@ __device__ __forceinline__ u64 epilogue_apply(u64 v, u64 a, u64 b, const NttTask
 }
 
 // ------------------------------------------------------------------ forward
-// Column pass: global stages [0, L1).  Output kept lazy in [0, 4q).
-template <int LOGN, int PRO>
+// Column pass: global stages [0, L1).  Output kept lazy (below bnd(L1) q of
+// the reduction schedule).
+template <int LOGN, int PRO, int NARROW>
 __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T, NttFuse F) {
   using C = NttCfg<LOGN, 0>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
   const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
   u64* data = tk.data;
   const u32 pid = tk.pid;
-  const u64 q = T.prm[pid].q, q2 = 2 * q;
+  const u64 q = T.prm[pid].q, q2 = 2 * q, msub = (u64)FwdLazy<NARROW, LOGN>::SUB * q;
   const u64* tw = T.psi + ((u64)pid << LOGN);
   const u64* tws = T.psi_sh + ((u64)pid << LOGN);
   const u32 tid = threadIdx.x;
@@ -158,7 +244,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << u) + (k >> (C::A - u));
-      ct_bfly<true>(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+      fwd_bfly<true, NARROW, LOGN>(u, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
     }
   }
   u64* L = lds + ll * C::LDS_LINE;
@@ -179,7 +265,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << u) + (r1 << v) + (k >> (C::B - v));
-        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+        fwd_bfly<false, NARROW, LOGN>(u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2, msub);
       }
     }
 #pragma unroll
@@ -187,15 +273,24 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
   }
 }
 
-// Row pass: global stages [L1, LOGN).  Input in [0, 4q); output in [0, q).
+// Row pass: global stages [L1, LOGN).  Input lazy as the column pass left
+// it; output in [0, q).
+// Occupancy target of the row pass at N = 2^16 per epilogue (waves per
+// SIMD), set so the register allocation neither spills nor takes more than
+// the kernel needs (left free, it grew to 4 waves where 7 fit).  Other ring
+// dimensions keep the compiler's choice (1 = no constraint).
 template <int LOGN, int EPI>
-__global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
+struct RowWaves {
+  static constexpr int value = LOGN != 16 ? 1 : EPI == EPI_MODDOWN_ADD ? 5 : EPI == EPI_MDRS ? 6 : 7;
+};
+template <int LOGN, int EPI, int NARROW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LOGN, EPI>::value))) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
   using C = NttCfg<LOGN, 1>;
   __shared__ u32 lds32[C::LPW * C::LDS_LINE];
   const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
   u64* data = tk.data;
   const u32 pid = tk.pid;
-  const u64 q = T.prm[pid].q, q2 = 2 * q;
+  const u64 q = T.prm[pid].q, q2 = 2 * q, msub = (u64)FwdLazy<NARROW, LOGN>::SUB * q;
   const u64* tw = T.psi + ((u64)pid << LOGN);
   const u64* tws = T.psi_sh + ((u64)pid << LOGN);
   const u32 tid = threadIdx.x;
@@ -212,7 +307,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
-      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+      fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
     }
   }
   u32* L = lds32 + ll * C::LDS_LINE;
@@ -247,7 +342,8 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
-        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+        fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2,
+                                      msub);
       }
     }
   }
@@ -287,7 +383,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
       for (int k = 0; k < EPC; ++k) {
         const u32 c = r0 + C::TPL * (k0 + k);
         const u32 j = (row << C::L2) | c;
-        u64 v = csub(csub(x[k0 + k], q2), q);
+        u64 v = fwd_final(x[k0 + k], q, q2);
         v = epilogue_apply<EPI>(v, ea[k], eb[k], tk, q);
         const u32 jr = __brev(j) >> (32 - LOGN);
         const u32 s = (((2u * jr + 1u) * tk.perm) & mask) >> 1;
@@ -305,7 +401,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_row(LimbSpan span, NttTables T,
 #pragma unroll
     for (int k = 0; k < EPC; ++k) {
       const u32 c = r0 + C::TPL * (k0 + k);
-      u64 v = csub(csub(x[k0 + k], q2), q);
+      u64 v = fwd_final(x[k0 + k], q, q2);
       v = epilogue_apply<EPI>(v, ea[k], eb[k], tk, q);
       stg(dst, c, v);
     }
@@ -367,11 +463,12 @@ __device__ __forceinline__ void intt_row_stages(u32 row, u32 r0, u64* L, const u
 // pass of their INTT.
 // One digit of k_ntt_row_dot: the row stages of digit j's rows (or the input
 // limb itself for the digit owning t), fully reduced, into `out`.
-template <int LOGN>
+template <int LOGN, int NARROW>
 __device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, u64 rbase, u32 row, u32 r0,
                                               u64* L, const u64* tw, const u64* tws, u64 q, u64 q2,
                                               u64 (&out)[NttCfg<LOGN, 1>::E]) {
   using C = NttCfg<LOGN, 1>;
+  const u64 msub = (u64)FwdLazy<NARROW, LOGN>::SUB * q;
   if (j == tk.jown) {  // uniform per workgroup
 #pragma unroll
     for (int k = 0; k < C::E; ++k) out[k] = ldg(tk.d, rbase + r0 + C::TPL * k);
@@ -388,7 +485,7 @@ __device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, 
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
-      ct_bfly(x[k], x[k + half], tw[idx], tws[idx], q, q2);
+      fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
     }
   }
   __syncthreads();  // earlier readers of this workgroup's LDS are done
@@ -409,7 +506,8 @@ __device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, 
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
-        ct_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+        fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2,
+                                      msub);
       }
     }
 #pragma unroll
@@ -417,10 +515,10 @@ __device__ __forceinline__ void row_dot_digit(const DotTask& tk, u32 j, u64 es, 
   }
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < C::E; ++k) out[k] = csub(csub(L[C::lx(r0 + C::TPL * k)], q2), q);
+  for (int k = 0; k < C::E; ++k) out[k] = fwd_final(L[C::lx(r0 + C::TPL * k)], q, q2);
 }
 
-template <int LOGN, int BETA>
+template <int LOGN, int BETA, int NARROW>
 __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ tasks, NttTables T, u32 kstride) {
   using C = NttCfg<LOGN, 1>;
   constexpr int RPW = 64 / C::TPL;  // rows per workgroup
@@ -438,10 +536,10 @@ __global__ __launch_bounds__(64) void k_ntt_row_dot(const DotTask* __restrict__ 
   const u64 rbase = (u64)row << C::L2;
   u64* L = lds + ll * C::LDS_LINE;
   u64 e0[C::E], e1[C::E], e2[C::E], e3[C::E];
-  row_dot_digit<LOGN>(tk, 0, es, rbase, row, r0, L, tw, tws, q, q2, e0);
-  if constexpr (BETA > 1) row_dot_digit<LOGN>(tk, 1, es, rbase, row, r0, L, tw, tws, q, q2, e1);
-  if constexpr (BETA > 2) row_dot_digit<LOGN>(tk, 2, es, rbase, row, r0, L, tw, tws, q, q2, e2);
-  if constexpr (BETA > 3) row_dot_digit<LOGN>(tk, 3, es, rbase, row, r0, L, tw, tws, q, q2, e3);
+  row_dot_digit<LOGN, NARROW>(tk, 0, es, rbase, row, r0, L, tw, tws, q, q2, e0);
+  if constexpr (BETA > 1) row_dot_digit<LOGN, NARROW>(tk, 1, es, rbase, row, r0, L, tw, tws, q, q2, e1);
+  if constexpr (BETA > 2) row_dot_digit<LOGN, NARROW>(tk, 2, es, rbase, row, r0, L, tw, tws, q, q2, e2);
+  if constexpr (BETA > 3) row_dot_digit<LOGN, NARROW>(tk, 3, es, rbase, row, r0, L, tw, tws, q, q2, e3);
 #pragma unroll
   for (int k = 0; k < C::E; ++k) {
     const u64 i = rbase + r0 + C::TPL * k;
@@ -627,24 +725,50 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
 }
 
 // ------------------------------------------------------------------ launchers
+template <int LOGN, int NARROW>
+static void launch_col_n(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
+  using C0 = NttCfg<LOGN, 0>;
+  dim3 g0(C0::NLINES / C0::LPW, s.total);
+  if (pro == PRO_SWITCH_MODULUS)
+    k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS, NARROW><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else if (pro == PRO_MDRS)
+    k_ntt_fwd_col<LOGN, PRO_MDRS, NARROW><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  else
+    k_ntt_fwd_col<LOGN, PRO_NONE, NARROW><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+}
+
+template <int LOGN, int NARROW>
+static void launch_row_n(const LimbSpan& s, const NttTables& T, int epi, const NttFuse& F, hipStream_t st) {
+  using C1 = NttCfg<LOGN, 1>;
+  dim3 g1(C1::NLINES / C1::LPW, s.total);
+  switch (epi) {
+    case EPI_MODDOWN: k_ntt_fwd_row<LOGN, EPI_MODDOWN, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    case EPI_MODDOWN_ADD:
+      k_ntt_fwd_row<LOGN, EPI_MODDOWN_ADD, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F);
+      break;
+    case EPI_RESCALE: k_ntt_fwd_row<LOGN, EPI_RESCALE, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    case EPI_MDRS: k_ntt_fwd_row<LOGN, EPI_MDRS, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+    default: k_ntt_fwd_row<LOGN, EPI_NONE, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
+  }
+}
+
+// The column pass leaves its output lazy (below bnd(L1) q of the schedule
+// it ran); the row pass that follows is launched with the same schedule.
+template <int LOGN>
+static void launch_col(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
+  if (T.narrow)
+    launch_col_n<LOGN, 1>(s, T, pro, F, st);
+  else
+    launch_col_n<LOGN, 0>(s, T, pro, F, st);
+}
+
 template <int LOGN>
 static void launch_fwd(const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F, hipStream_t st) {
-  using C0 = NttCfg<LOGN, 0>;
-  using C1 = NttCfg<LOGN, 1>;
-  dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
-  if (pro == PRO_SWITCH_MODULUS)
-    k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
-  else if (pro == PRO_MDRS)
-    k_ntt_fwd_col<LOGN, PRO_MDRS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
+  launch_col<LOGN>(s, T, pro, F, st);
+  if (T.narrow)
+    launch_row_n<LOGN, 1>(s, T, epi, F, st);
   else
-    k_ntt_fwd_col<LOGN, PRO_NONE><<<g0, C0::THREADS, 0, st>>>(s, T, F);
-  switch (epi) {
-    case EPI_MODDOWN: k_ntt_fwd_row<LOGN, EPI_MODDOWN><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
-    case EPI_MODDOWN_ADD: k_ntt_fwd_row<LOGN, EPI_MODDOWN_ADD><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
-    case EPI_RESCALE: k_ntt_fwd_row<LOGN, EPI_RESCALE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
-    case EPI_MDRS: k_ntt_fwd_row<LOGN, EPI_MDRS><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
-    default: k_ntt_fwd_row<LOGN, EPI_NONE><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
-  }
+    launch_row_n<LOGN, 0>(s, T, epi, F, st);
 }
 
 template <int LOGN>
@@ -656,28 +780,23 @@ static void launch_inv(const LimbSpan& s, const NttTables& T, hipStream_t st) {
   k_intt_col<LOGN><<<g0, C0::THREADS, 0, st>>>(s, T);
 }
 
-template <int LOGN>
-static void launch_col(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
-  using C0 = NttCfg<LOGN, 0>;
-  dim3 g0(C0::NLINES / C0::LPW, s.total);
-  if (pro == PRO_SWITCH_MODULUS)
-    k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
-  else if (pro == PRO_MDRS)
-    k_ntt_fwd_col<LOGN, PRO_MDRS><<<g0, C0::THREADS, 0, st>>>(s, T, F);
-  else
-    k_ntt_fwd_col<LOGN, PRO_NONE><<<g0, C0::THREADS, 0, st>>>(s, T, F);
-}
 
 template <int LOGN>
 static void launch_row_dot(const DotTask* tasks, u32 n, int beta, const NttTables& T, u32 kstride, hipStream_t st) {
   using C1 = NttCfg<LOGN, 1>;
   dim3 g(C1::NLINES / (64 / C1::TPL), n);
+#define ROWDOT(B)                                                 \
+  if (T.narrow)                                                   \
+    k_ntt_row_dot<LOGN, B, 1><<<g, 64, 0, st>>>(tasks, T, kstride); \
+  else                                                            \
+    k_ntt_row_dot<LOGN, B, 0><<<g, 64, 0, st>>>(tasks, T, kstride);
   switch (beta) {
-    case 1: k_ntt_row_dot<LOGN, 1><<<g, 64, 0, st>>>(tasks, T, kstride); break;
-    case 2: k_ntt_row_dot<LOGN, 2><<<g, 64, 0, st>>>(tasks, T, kstride); break;
-    case 3: k_ntt_row_dot<LOGN, 3><<<g, 64, 0, st>>>(tasks, T, kstride); break;
-    default: k_ntt_row_dot<LOGN, 4><<<g, 64, 0, st>>>(tasks, T, kstride); break;
+    case 1: ROWDOT(1) break;
+    case 2: ROWDOT(2) break;
+    case 3: ROWDOT(3) break;
+    default: ROWDOT(4) break;
   }
+#undef ROWDOT
 }
 
 // Host entry points (called by the context code).  Return 0 or -1 for an
