@@ -4,8 +4,10 @@
   running the same program op by op (bit-exact), equals the same request
   served alone, and decrypts to the FP64 evaluation of its program
   (Chebyshev series / rotate-and-sum) to a stated precision.
-* N = 2^16, 30 limbs, synthetic keys: results identical alone and in a mix
-  (exact arithmetic: any difference is a race or a storage-reuse bug).
+* N = 2^16, 30 limbs, synthetic keys: Chebyshev, rotate-and-sum and
+  bootstrap requests (FLEXIBLEAUTO, sparse-secret encapsulation) give
+  identical results alone and in a mix (exact arithmetic: any difference is
+  a race or a storage-reuse bug).
 """
 import numpy as np
 import pytest
@@ -108,7 +110,9 @@ def test_serve_small_memory_pressure_and_rejection(small):
 
 
 def test_serve_cfg3_alone_equals_mix():
-    """N=2^16, 30 limbs, synthetic keys and ciphertexts."""
+    """N=2^16, 30 limbs, synthetic keys and ciphertexts; two of the ten
+    requests are bootstraps."""
+    from fideslib_amd import bootstrap as B
     prm = params.config3()
     say("serve cfg3 determinism")
     N = prm.N
@@ -123,7 +127,17 @@ def test_serve_cfg3_alone_equals_mix():
     kmax = 3
     rlk = gctx.upload_key(rkey())
     keys = {pow(5, 1 << i, 2 * N): gctx.upload_key(rkey()) for i in range(kmax)}
-    be = S.GpuBackend(gctx, rlk, keys, 8 << 30)
+    bp = B.BootstrapPlan(prm, gctx, B.BootConfig.encapsulated(flexible=True))
+    bp.build_plaintexts()
+    shared = rkey()  # one synthetic key, a distinct device copy per bootstrap key
+    for r in bp.rotations() + [None]:
+        k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
+        if k not in keys:
+            keys[k] = gctx.upload_key(shared)
+    for k in bp.switch_keys():
+        keys[k] = gctx.upload_key(shared)
+    sf = params.flexible_scaling_factors(prm.q)
+    be = S.GpuBackend(gctx, rlk, keys, 24 << 30)
     coef = S.chebyshev_coefficients(SIGMOID, 15, -8, 8)
     work = []
     for i in range(8):
@@ -134,6 +148,10 @@ def test_serve_cfg3_alone_equals_mix():
         l = P.inputs[0].level
         ct = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in prm.q[:l]]) for _ in range(2)])
         work.append((P, ct))
+    for lb in (3, 17):
+        P = bp.program(lb, sf[lb - 1])
+        ct = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in prm.q[:lb]]) for _ in range(2)])
+        work.insert(3 if lb == 3 else 7, (P, ct))
     reqs = [S.Request(i, P, 0.001 * i, inputs=[ct], keep_result=True) for i, (P, ct) in enumerate(work)]
     srv = S.Server(be, N)
     srv.run(reqs, realtime=True)
