@@ -28,6 +28,14 @@ mathematics and computed on the host):
   7. SlotToCoeff  the decoding map, scaled back by q_0 / (2 pi s)
   8. x 2^c      exact integer multiply
 
+Sparse packing (cfg.slots = n_s < N/2, the reference's sparse path,
+src/CKKS/Bootstrap.cu:100,116-143): the message lives in the subring of
+X^(N / 2 n_s).  After ModRaise the partial trace (x += Rot(x, j n_s),
+j = 1, 2, 4, ...) keeps that subring's part, times N / (2 n_s) (folded into
+CoeffToSlot); the transforms are those of n_s slots on 2 n_s-periodic slot
+vectors, CoeffToSlot leaves [2 Re w | 2 Im w] in one ciphertext so that one
+EvalMod serves both halves, and SlotToCoeff recombines them (sparse_fold).
+
 Decoding map: z_j = sum_k w_k zeta^(5^j k) for the complex-packed
 coefficients w.  It factors as U = B_logn ... B_1 BR (BR the bit reversal,
 B_s the butterfly stage of length 2^s with twiddles exp(2 pi i (5^j mod 2^(s+2))
@@ -195,18 +203,50 @@ class BootConfig:
     encaps_h: int | None = None  # sparse-secret encapsulation: Hamming weight of s' (None: off)
     flexible: bool = False      # FLEXIBLEAUTO: the result carries its level's FLEXIBLEAUTO
                                 # scaling factor (folded into the first SlotToCoeff group)
+    slots: int | None = None    # sparse packing: n_s slots (a power of two < N/2); None: N/2
 
     @staticmethod
-    def encapsulated(h: int = 32, flexible: bool = False) -> "BootConfig":
+    def encapsulated(h: int = 32, flexible: bool = False, slots: int | None = None) -> "BootConfig":
         """Sparse-secret encapsulation with a weight-h secret: |I| stays
         below ~8 with overwhelming probability for h = 32 (I is a sum of h + 1
         terms uniform in [-1/2, 1/2]), so K' = 16."""
-        return BootConfig(K=16.0, encaps_h=h, flexible=flexible)
+        return BootConfig(K=16.0, encaps_h=h, flexible=flexible, slots=slots)
+
+
+def tile_diags(D: dict, n2: int) -> dict:
+    """Diagonals of a matrix on n-periodic vectors, as a matrix on
+    n2-periodic ones (n | n2): the same offsets, each diagonal tiled."""
+    return {d: np.tile(v, n2 // len(v)) for d, v in D.items()}
+
+
+def sparse_fold(cts: list, stc: list, ns: int):
+    """The sparse-packing transforms on 2 ns-periodic slot vectors.
+
+    A message in ns slots (replicated over N/2) is a polynomial in
+    X^(N / 2 ns): its 2 ns coefficients t_k pack into ns complex slots
+    w_k = t_k + i t_(k+ns), and the decoding map of ns slots (the group
+    matrices of ns slots) takes w to the message.  On 2 ns-periodic vectors:
+    CoeffToSlot ends with the mask (1 on [0, ns), -i on [ns, 2 ns)), so that
+    ct + conj(ct) holds [2 Re w | 2 Im w] and one EvalMod treats both halves;
+    SlotToCoeff starts with the recombination w_p = y_p + i y_(p+ns) (p < ns,
+    and periodically), diagonals {0: a, ns: b}.  The mask is folded into the
+    last CoeffToSlot group and the recombination into the first SlotToCoeff
+    group (no extra level)."""
+    n2 = 2 * ns
+    cts = [tile_diags(D, n2) for D in cts]
+    stc = [tile_diags(D, n2) for D in stc]
+    low = np.arange(n2) < ns
+    mask = np.where(low, 1.0 + 0j, -1j)
+    cts[-1] = {d: mask * v for d, v in cts[-1].items()}
+    comb = {0: np.where(low, 1.0 + 0j, 1j), ns: np.where(low, 1j, 1.0 + 0j)}
+    stc[0] = diag_mul(stc[0], comb, n2)
+    return cts, stc
 
 
 class BootstrapPlan:
-    """Tables and device plaintexts for bootstrapping full-slot ciphertexts
-    of one parameter set; `program()` builds the request program.
+    """Tables and device plaintexts for bootstrapping ciphertexts of one
+    parameter set (full slots, or cfg.slots < N/2 with sparse packing);
+    `program()` builds the request program.
 
     ctx: a capi.Context (plaintexts are uploaded and prepared on it), or None
     to plan without a device (keep_host=True then keeps the plaintext
@@ -222,12 +262,24 @@ class BootstrapPlan:
         self.delta = float(2 ** prm.scale_bits)
         self.ctx = ctx
         self.keep_host = keep_host
-        n, N = self.n, self.N
-        cts = transform_groups(n, N, cfg.budget[0], inverse=True)
-        stc = transform_groups(n, N, cfg.budget[1], inverse=False)
+        N = self.N
+        self.slots = cfg.slots or N // 2
+        ns = self.slots
+        if ns < 2 or ns > N // 2 or ns & (ns - 1):
+            raise ValueError("slots must be a power of two in [2, N/2]")
+        self.sparse = ns < N // 2
+        cts = transform_groups(ns, N, cfg.budget[0], inverse=True)
+        stc = transform_groups(ns, N, cfg.budget[1], inverse=False)
+        if self.sparse:
+            cts, stc = sparse_fold(cts, stc, ns)
+            self.n = 2 * ns            # slot period the group plans work on
+        n = self.n
         q0 = float(self.q[0])
-        # CoeffToSlot output y = t / (q_0 K'), with the 1/2 of the Re/Im split folded in
+        # CoeffToSlot output y = t / (q_0 K'), with the 1/2 of the Re/Im split
+        # folded in (and 2 ns / N, the factor of the partial trace, when sparse)
         self.f_cts = self.delta / (2.0 * q0 * cfg.K)
+        if self.sparse:
+            self.f_cts *= 2.0 * ns / N
         # SlotToCoeff input sin(2 pi t/q_0) ~ 2 pi m/q_0  ->  m / s1 with s1 ~ delta
         self.f_stc = q0 / (2.0 * math.pi * self.delta)
         self.cts_plans = [plan_group(D, n, self.f_cts if i == 0 else 1.0) for i, D in enumerate(cts)]
@@ -237,7 +289,8 @@ class BootstrapPlan:
         # levels
         self.l_raise = self.L
         self.evalmod_depth = chebyshev_eval_depth(self.cheb) + cfg.double_angle
-        self.depth = cfg.budget[0] + cfg.budget[1] + self.evalmod_depth
+        # (fewer groups than the budget when the slots have fewer stages)
+        self.depth = len(self.cts_plans) + len(self.stc_plans) + self.evalmod_depth
         if self.l_raise - self.depth < 1:
             raise ValueError("not enough levels for bootstrapping: need %d, have %d" % (self.depth + 1, self.L))
         self._pt_cache = {}
@@ -252,9 +305,19 @@ class BootstrapPlan:
             self.f_stc *= self.out_scale / s_nom
             self.stc_plans = [plan_group(D, n, self.f_stc if i == 0 else 1.0) for i, D in enumerate(stc)]
 
-    # -- rotation indices every program uses (slot rotations, mod n); conjugation separately
+    # -- rotation indices every program uses (slot rotations, mod N/2); conjugation separately
+    def trace_rotations(self) -> list:
+        """Sparse packing: the partial trace after ModRaise, x += Rot(x, j ns)
+        for j = 1, 2, 4, ... below N / (2 ns) (the rotations that fix the
+        subring of X^(N / 2 ns))."""
+        out, j = [], 1
+        while self.sparse and j * self.slots < self.N // 2:
+            out.append(j * self.slots)
+            j <<= 1
+        return out
+
     def rotations(self) -> list:
-        r = set()
+        r = set(self.trace_rotations())
         for P in self.cts_plans + self.stc_plans:
             r |= P.rotations()
         r.discard(0)
@@ -309,9 +372,11 @@ class BootstrapPlan:
             l -= 1
         l_split = l
         mono = (lambda p, lv: Plaintext(0, lv, 1.0)) if scale_only else self._monomial
-        self._minus_i = mono(3 * self.N // 2, l_split)
+        if not self.sparse:  # the Re/Im split and recombination of full slots
+            self._minus_i = mono(3 * self.N // 2, l_split)
         l -= self.evalmod_depth
-        self._plus_i = mono(self.N // 2, l)
+        if not self.sparse:
+            self._plus_i = mono(self.N // 2, l)
         for P in self.stc_plans:
             pts["stc"].append(self._group_pts(P, l, scale_only))
             l -= 1
@@ -352,11 +417,13 @@ class BootstrapPlan:
 
     # -- program pieces
     def _linear(self, Pg: Program, x: Reg, plan: GroupPlan, pts: dict) -> Reg:
-        babies = {b: (x if b == 0 else Pg.rotate(x, b * plan.stride)) for b in plan.babies}
+        # rotation amounts modulo the plan's slot period (the data repeats with
+        # it), the same indices rotations() lists
+        babies = {b: (x if b == 0 else Pg.rotate(x, (b * plan.stride) % plan.n)) for b in plan.babies}
         parts = []
         for k in sorted(pts):
             inner = Pg.ptdot([(babies[b], pt) for b, pt in pts[k]])
-            parts.append(inner if k == 0 else Pg.rotate(inner, k * plan.g * plan.stride))
+            parts.append(inner if k == 0 else Pg.rotate(inner, (k * plan.g * plan.stride) % plan.n))
         # one fused sum (a linear combination with unit constants) instead of
         # a tree of pairwise additions: one launch and one step
         if len(parts) == 1:
@@ -371,7 +438,7 @@ class BootstrapPlan:
         return c
 
     def program(self, level_in: int, scale_in: float | None = None) -> Program:
-        """Bootstrap of one full-slot ciphertext at `level_in` limbs (>= 2)."""
+        """Bootstrap of one ciphertext (self.slots slots) at `level_in` limbs (>= 2)."""
         self.build_plaintexts()
         cfg = self.cfg
         Pg = Program(self.q, self.prm.logN, self.prm.scale_bits, "bootstrap")
@@ -390,14 +457,20 @@ class BootstrapPlan:
         r = Pg.mod_raise(x, self.l_raise, self.delta)
         if cfg.encaps_h:
             r = Pg.switch_key(r, self.ENCAPS_UP)
+        for amount in self.trace_rotations():
+            r = Pg.add(r, Pg.rotate(r, amount))
         for plan, pts in zip(self.cts_plans, self._pts["cts"]):
             r = self._linear(Pg, r, plan, pts)
         cj = Pg.conjugate(r)
-        re = Pg.add(r, cj)
-        im = Pg.ptdot([(Pg.sub(r, cj), self._minus_i)])
-        re = self._evalmod(Pg, re)
-        im = self._evalmod(Pg, im)
-        w = Pg.add(re, Pg.ptdot([(im, self._plus_i)]))
+        if self.sparse:
+            # [2 Re w | 2 Im w] in one ciphertext: one EvalMod
+            w = self._evalmod(Pg, Pg.add(r, cj))
+        else:
+            re = Pg.add(r, cj)
+            im = Pg.ptdot([(Pg.sub(r, cj), self._minus_i)])
+            re = self._evalmod(Pg, re)
+            im = self._evalmod(Pg, im)
+            w = Pg.add(re, Pg.ptdot([(im, self._plus_i)]))
         for plan, pts in zip(self.stc_plans, self._pts["stc"]):
             w = self._linear(Pg, w, plan, pts)
         # value = the input message: the nominal scale of the result is the

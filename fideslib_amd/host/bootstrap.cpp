@@ -140,6 +140,38 @@ std::vector<Diag> transform_groups(uint32_t n, uint32_t N, uint32_t budget, bool
   return mats;
 }
 
+// Sparse packing (ns < N/2 slots): the ns-slot transforms on 2 ns-periodic
+// slot vectors, CoeffToSlot ending with the mask (1 on [0, ns), -i on
+// [ns, 2 ns)) so that ct + conj(ct) holds [2 Re w | 2 Im w], SlotToCoeff
+// starting with the recombination w_p = y_p + i y_(p+ns) (diagonals
+// {0: a, ns: b}); the mask folds into the last CoeffToSlot group, the
+// recombination into the first SlotToCoeff group (the Python plan's
+// sparse_fold).
+void sparse_fold(std::vector<Diag>& cts, std::vector<Diag>& stc, uint32_t ns) {
+  const uint32_t n2 = 2 * ns;
+  auto tile = [&](std::vector<Diag>& groups) {
+    for (Diag& D : groups)
+      for (auto& kv : D) {
+        Vec v(n2);
+        for (uint32_t p = 0; p < n2; ++p) v[p] = kv.second[p % kv.second.size()];
+        kv.second = std::move(v);
+      }
+  };
+  tile(cts);
+  tile(stc);
+  for (auto& kv : cts.back())
+    for (uint32_t p = ns; p < n2; ++p) kv.second[p] *= cd(0.0, -1.0);
+  Vec a(n2), b(n2);
+  for (uint32_t p = 0; p < n2; ++p) {
+    a[p] = p < ns ? cd(1.0, 0.0) : cd(0.0, 1.0);
+    b[p] = p < ns ? cd(0.0, 1.0) : cd(1.0, 0.0);
+  }
+  Diag comb;
+  comb[0] = a;
+  comb[ns] = b;
+  stc.front() = diag_mul(stc.front(), comb, n2);
+}
+
 struct GroupPlan {
   uint32_t stride = 0, g = 1, n = 0;
   std::vector<int64_t> babies;
@@ -225,6 +257,9 @@ struct BootPlan {
   double f_cts = 0, f_stc = 0;
   double out_scale = 0;  // nominal scale of the result (0: the tracked one)
   bool flexible = false; // FLEXIBLEAUTO: the prescale also brings the input's scale to delta
+  uint32_t half = 0;     // N/2, the ring's slots
+  uint32_t slots = 0;    // message slots (N/2: full packing)
+  bool sparse = false;   // slots < N/2: partial trace, 2 slots-periodic transforms, one EvalMod
   std::vector<GroupPlan> cts, stc;
   std::vector<double> cheb;
   // device plaintexts: per group, per giant k, (baby b, prepared plaintext)
@@ -239,8 +274,29 @@ namespace {
 
 using GroupPts = std::map<int64_t, std::vector<std::pair<int64_t, BootPlan::Pt>>>;
 
+// Sparse packing: the partial trace after ModRaise, x += Rot(x, j slots) for
+// j = 1, 2, 4, ... below N / (2 slots) (the rotations that fix the subring of
+// X^(N / 2 slots); reference src/CKKS/Bootstrap.cu:116-143).
+std::vector<int64_t> trace_rotations(const BootPlan& P) {
+  std::vector<int64_t> out;
+  if (!P.sparse) return out;
+  for (int64_t j = 1; j * (int64_t)P.slots < (int64_t)P.half; j <<= 1) out.push_back(j * (int64_t)P.slots);
+  return out;
+}
+
+// Message slots of the plan (0: N/2) and the slot period its groups work on.
+void set_slots(BootPlan& P, uint32_t slots, uint32_t N) {
+  const uint32_t ns = slots ? slots : N / 2;
+  if (ns < 2 || ns > N / 2 || (ns & (ns - 1)) != 0)
+    throw std::runtime_error("EvalBootstrapSetup: slots must be a power of two in [2, N/2]");
+  P.half = N / 2;
+  P.slots = ns;
+  P.sparse = ns < N / 2;
+  P.n = P.sparse ? 2 * ns : N / 2;
+}
+
 // Rotation indices (slot rotations mod n, without 0) of the groups' baby
-// and giant steps.
+// and giant steps, and of the partial trace.
 std::vector<int32_t> plan_rotations(const BootPlan& P) {
   const int64_t n = P.n;
   std::vector<int32_t> rots;
@@ -252,6 +308,7 @@ std::vector<int32_t> plan_rotations(const BootPlan& P) {
   };
   for (size_t i = 0; i < P.cts.size(); ++i) add(P.cts[i], P.cts_pts[i]);
   for (size_t i = 0; i < P.stc.size(); ++i) add(P.stc[i], P.stc_pts[i]);
+  for (int64_t r : trace_rotations(P)) rots.push_back((int32_t)r);
   std::sort(rots.begin(), rots.end());
   rots.erase(std::unique(rots.begin(), rots.end()), rots.end());
   rots.erase(std::remove(rots.begin(), rots.end(), 0), rots.end());
@@ -270,26 +327,31 @@ void placeholder_pts(BootPlan& P, const std::vector<uint64_t>& q) {
   P.stc_pts.clear();
   uint32_t l = P.l_raise;
   for (const GroupPlan& G : P.cts) P.cts_pts.push_back(group(G, l--));
-  P.minus_i = BootPlan::Pt{nullptr, l, 1.0};
+  if (!P.sparse) P.minus_i = BootPlan::Pt{nullptr, l, 1.0};
   l -= P.evalmod_depth;
-  P.plus_i = BootPlan::Pt{nullptr, l, 1.0};
+  if (!P.sparse) P.plus_i = BootPlan::Pt{nullptr, l, 1.0};
   for (const GroupPlan& G : P.stc) P.stc_pts.push_back(group(G, l--));
 }
 
 // The bootstrap program (fideslib_amd/bootstrap.py BootstrapPlan.program):
 // prescale by 2^-c, one limb, [switch to the sparse secret], ModRaise,
-// [switch back], CoeffToSlot, Re/Im split, EvalMod, recombine, SlotToCoeff,
-// x 2^c.
+// [switch back], [partial trace (sparse packing)], CoeffToSlot, Re/Im split
+// and two EvalMods (full packing) or one EvalMod on [Re | Im] (sparse),
+// recombine, SlotToCoeff, x 2^c.
 Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fides_key* up) {
   auto linear = [&](Reg v, const GroupPlan& G, const GroupPts& pts) {
+    // rotation amounts modulo the plan's slot period (the data repeats with
+    // it), the indices plan_rotations lists
+    const int64_t n = (int64_t)G.n;
+    auto amount = [n](int64_t a) { return ((a % n) + n) % n; };
     std::map<int64_t, Reg> babies;
-    for (int64_t b : G.babies) babies[b] = b == 0 ? v : X.rotate(v, b * (int64_t)G.stride);
+    for (int64_t b : G.babies) babies[b] = b == 0 ? v : X.rotate(v, amount(b * (int64_t)G.stride));
     std::vector<Reg> parts;
     for (const auto& kv : pts) {
       std::vector<std::pair<Reg, const BootPlan::Pt*>> terms;
       for (const auto& bp : kv.second) terms.emplace_back(babies.at(bp.first), &bp.second);
       Reg inner = X.ptdot(terms);
-      parts.push_back(kv.first == 0 ? inner : X.rotate(inner, kv.first * (int64_t)G.g * (int64_t)G.stride));
+      parts.push_back(kv.first == 0 ? inner : X.rotate(inner, amount(kv.first * (int64_t)G.g * (int64_t)G.stride)));
     }
     if (parts.size() == 1) return X.rescale(parts[0]);
     std::vector<std::pair<Reg, i128>> t;
@@ -315,14 +377,20 @@ Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fid
   if (P.cfg.encapsulation) x = X.switch_key(x, down);
   Reg r = X.mod_raise(x, P.l_raise, X.delta);
   if (P.cfg.encapsulation) r = X.switch_key(r, up);
+  for (int64_t a : trace_rotations(P)) r = X.add(r, X.rotate(r, a));
   for (size_t i = 0; i < P.cts.size(); ++i) r = linear(r, P.cts[i], P.cts_pts[i]);
   Reg cj = X.conjugate(r);
-  Reg re = X.add(r, cj);
-  Reg d = X.sub(r, cj);
-  Reg im = X.ptdot({{d, &P.minus_i}});
-  re = evalmod(re);
-  im = evalmod(im);
-  Reg w = X.add(re, X.ptdot({{im, &P.plus_i}}));
+  Reg w;
+  if (P.sparse) {  // [2 Re w | 2 Im w] in one ciphertext: one EvalMod
+    w = evalmod(X.add(r, cj));
+  } else {
+    Reg re = X.add(r, cj);
+    Reg d = X.sub(r, cj);
+    Reg im = X.ptdot({{d, &P.minus_i}});
+    re = evalmod(re);
+    im = evalmod(im);
+    w = X.add(re, X.ptdot({{im, &P.plus_i}}));
+  }
   for (size_t i = 0; i < P.stc.size(); ++i) w = linear(w, P.stc[i], P.stc_pts[i]);
   const i128 corr = (i128)1 << P.cfg.correction;
   return X.lin(w, &corr, nullptr, false, nullptr, P.out_scale > 0 ? P.out_scale : w.scale);
@@ -355,15 +423,18 @@ static BootPlan::Pt monomial_pt(fides_context* ctx, uint32_t N, const std::vecto
 std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapConfig& cfg) {
   auto P = std::make_shared<BootPlan>();
   P->cfg = cfg;
-  P->n = N / 2;
+  set_slots(*P, cfg.slots, N);
   const uint32_t n = P->n;
   const double q0 = (double)q[0];
-  // CoeffToSlot output y = t / (q_0 K'), the 1/2 of the Re/Im split folded in;
+  // CoeffToSlot output y = t / (q_0 K'), the 1/2 of the Re/Im split folded in
+  // (and 2 slots / N, the factor of the partial trace, when sparse);
   // SlotToCoeff input sin(2 pi t / q_0) ~ 2 pi m / q_0 -> m / delta
   P->f_cts = boot_delta() / (2.0 * q0 * cfg.K);
+  if (P->sparse) P->f_cts *= 2.0 * (double)P->slots / (double)N;
   P->f_stc = q0 / (2.0 * M_PI * boot_delta());
-  const auto cts = transform_groups(n, N, cfg.budgetCtS, true);
-  const auto stc = transform_groups(n, N, cfg.budgetStC, false);
+  auto cts = transform_groups(P->slots, N, cfg.budgetCtS, true);
+  auto stc = transform_groups(P->slots, N, cfg.budgetStC, false);
+  if (P->sparse) sparse_fold(cts, stc, P->slots);
   for (size_t i = 0; i < cts.size(); ++i) P->cts.push_back(plan_group(cts[i], n, i == 0 ? P->f_cts : 1.0));
   for (size_t i = 0; i < stc.size(); ++i) P->stc.push_back(plan_group(stc[i], n, i == 0 ? P->f_stc : 1.0));
   const double fa[2] = {cfg.K, std::ldexp(1.0, (int)cfg.doubleAngle)};
@@ -389,20 +460,23 @@ std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapConfig
     for (size_t i = 0; i < stc.size(); ++i) P->stc.push_back(plan_group(stc[i], n, i == 0 ? P->f_stc : 1.0));
   }
   // plaintexts: encode v at `level` limbs with scale `scale`
+  const uint32_t half = N / 2;
   const std::vector<uint64_t> slot_t = [&] {
-    std::vector<uint64_t> t(n);
+    std::vector<uint64_t> t(half);
     uint64_t g = 1;
-    for (uint32_t j = 0; j < n; ++j) {
+    for (uint32_t j = 0; j < half; ++j) {
       t[j] = (g - 1) / 2;
       g = g * 5 % (2ull * N);
     }
     return t;
   }();
+  // z: a diagonal of period n (n | N/2), replicated over the N/2 slots
   auto encode = [&](const Vec& z, uint32_t level, double scale) {
     std::vector<cd> v(N);
-    for (uint32_t j = 0; j < n; ++j) {
-      v[slot_t[j]] = z[j];
-      v[N - 1 - slot_t[j]] = std::conj(z[j]);
+    for (uint32_t j = 0; j < half; ++j) {
+      const cd zj = z[j % z.size()];
+      v[slot_t[j]] = zj;
+      v[N - 1 - slot_t[j]] = std::conj(zj);
     }
     fft(v);
     std::vector<uint64_t> host((size_t)level * N);
@@ -429,9 +503,9 @@ std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapConfig
   P->stc_pts.clear();
   uint32_t l = P->l_raise;
   for (const GroupPlan& G : P->cts) P->cts_pts.push_back(group_pts(G, l--));
-  P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l);
+  if (!P->sparse) P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l);
   l -= P->evalmod_depth;
-  P->plus_i = monomial_pt(ctx, N, q, N / 2, l);
+  if (!P->sparse) P->plus_i = monomial_pt(ctx, N, q, N / 2, l);
   for (const GroupPlan& G : P->stc) P->stc_pts.push_back(group_pts(G, l--));
   P->rotations = plan_rotations(*P);
   boot = P;
@@ -443,7 +517,7 @@ std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapPrecom
   P->cfg = pre.cfg;
   P->cfg.budgetCtS = (uint32_t)pre.coeffToSlot.size();
   P->cfg.budgetStC = (uint32_t)pre.slotToCoeff.size();
-  P->n = N / 2;
+  set_slots(*P, pre.cfg.slots, N);
   P->cheb = pre.chebyshev;
   if (P->cheb.size() < 2) throw std::runtime_error("EvalBootstrapSetup: the EvalMod series needs degree >= 1");
   P->l_raise = L;
@@ -477,9 +551,9 @@ std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapPrecom
   };
   load(pre.coeffToSlot, P->l_raise, P->cts, P->cts_pts);
   const uint32_t l_split = P->l_raise - P->cfg.budgetCtS;
-  P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l_split);
+  if (!P->sparse) P->minus_i = monomial_pt(ctx, N, q, 3ull * N / 2, l_split);
   const uint32_t l_stc = l_split - P->evalmod_depth;
-  P->plus_i = monomial_pt(ctx, N, q, N / 2, l_stc);
+  if (!P->sparse) P->plus_i = monomial_pt(ctx, N, q, N / 2, l_stc);
   load(pre.slotToCoeff, l_stc, P->stc, P->stc_pts);
   P->rotations = plan_rotations(*P);
   boot = P;

@@ -103,6 +103,11 @@ struct BootstrapConfig {
   // and back right after it, with the keys of AddEncapsulationKeys; K can
   // then be 16 for a weight-32 sparse secret.
   bool encapsulation = false;
+  // Message slots (a power of two up to N/2; 0 = N/2).  Fewer slots (sparse
+  // packing, api/CryptoContext.hpp:80 `slots`, reference Bootstrap.cu:100,
+  // 116-143): a partial trace after ModRaise, transforms of that many slots,
+  // one EvalMod; the input holds its slots replicated over N/2.
+  uint32_t slots = 0;
 };
 
 // Bootstrapping tables computed elsewhere (the reference takes them from

@@ -224,12 +224,13 @@ struct State {
       c.K = D(5);
       c.correction = (uint32_t)I(6);
       c.encapsulation = I(7) != 0;
+      if (w.size() > 8) c.slots = (uint32_t)I(8);
       const auto rots = cc->EvalBootstrapSetup(c);
       meta << "rots";
       for (int32_t r : rots) meta << " " << r;
       meta << "\n";
     } else if (op == "bootsetup_pre") {
-      // [ncts nstc ncheb correction doubleAngle encapsulation outScale] cheb[ncheb]
+      // [ncts nstc ncheb correction doubleAngle encapsulation slots outScale] cheb[ncheb]
       // then per group: stride g nbabies babies[] nterms, per term: giant baby level scale coef[level N]
       const auto b = read_u64(w.at(1));
       size_t o = 0;
@@ -245,6 +246,7 @@ struct State {
       pre.cfg.correction = (uint32_t)u();
       pre.cfg.doubleAngle = (uint32_t)u();
       pre.cfg.encapsulation = u() != 0;
+      pre.cfg.slots = (uint32_t)u();
       pre.outScale = d();
       for (uint64_t i = 0; i < ncheb; ++i) pre.chebyshev.push_back(d());
       for (uint64_t gi = 0; gi < ncts + nstc; ++gi) {

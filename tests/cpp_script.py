@@ -46,7 +46,7 @@ class Script:
     def bootsetup_pre(self, pre, cfg):
         """Write BootstrapPlan.export_precomputation() `pre` (config `cfg`)."""
         words = [len(pre["cts"]), len(pre["stc"]), len(pre["cheb"]), cfg.correction, cfg.double_angle,
-                 1 if cfg.encaps_h else 0]
+                 1 if cfg.encaps_h else 0, cfg.slots or 0]
         f64 = lambda x: int(np.array([x], np.float64).view(np.uint64)[0])
         parts = [np.asarray(words + [f64(pre["out_scale"])] + [f64(c) for c in pre["cheb"]], np.uint64)]
         for g in pre["cts"] + pre["stc"]:

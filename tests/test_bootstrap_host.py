@@ -8,6 +8,11 @@
 * The whole bootstrap program, run op by op on the oracle with real keys
   (N = 2^12, sparse secret), decrypts to the input message; a plausible
   mistake (an argument bound K' below the actual |I|) is caught.
+* Sparse packing (n_s < N/2 slots): the folded transforms on 2 n_s-periodic
+  vectors equal the n_s-slot maps followed by the [Re | Im] layout (and
+  back); the program decrypts on the oracle for 2, 64, 256 and 1024 slots
+  (every message slot, replicated over N/2); leaving out the partial trace
+  (a plausible mistake) is caught.
 """
 import math
 
@@ -80,7 +85,7 @@ def test_plan_levels_and_rotations():
         B.BootstrapPlan(params.serve_small(), None)   # 12 limbs: too shallow
 
 
-def _oracle_bootstrap(cfg, seed=3, h=64):
+def _oracle_bootstrap(cfg, seed=3, h=64, patch=None):
     from oracle import ckks_codec as C
     from oracle import oracle as O
     from serve_oracle import run_oracle
@@ -89,19 +94,23 @@ def _oracle_bootstrap(cfg, seed=3, h=64):
     octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
     keys = C.Keys(octx, 5, h=h)
     bp = B.BootstrapPlan(prm, None, cfg, keep_host=True)
-    P = bp.program(3)
     rot = {}
     for r in bp.rotations() + [None]:
         k = pow(5, r, 2 * N) if r is not None else 2 * N - 1
         rot[k] = keys.rotation_key(k)
+    if patch is not None:
+        patch(bp)
+    P = bp.program(3)
     if cfg.encaps_h:
         rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP] = keys.encapsulation_keys(cfg.encaps_h)
     rng = np.random.default_rng(seed)
-    z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    ns = bp.slots
+    z = rng.uniform(-1, 1, ns) + 1j * rng.uniform(-1, 1, ns)
     ct = keys.encrypt(C.encode(z, 2.0 ** prm.scale_bits, N), 3)
     out = run_oracle(octx, P, [ct], keys.relin_key(), rot)
     dec = C.decode(keys.decrypt_coef(out, P.output.level), P.output.scale, N)
-    return C.precision_bits(dec, z), P
+    # every slot: the message replicated over the N/2 slots
+    return C.precision_bits(dec, np.tile(z, N // 2 // ns)), P
 
 
 def test_bootstrap_on_the_oracle_decrypts():
@@ -176,3 +185,73 @@ def test_flexible_plan_lands_on_the_flexibleauto_factor():
     out = SO.run_oracle(octx, P, [ct], keys.relin_key(), rot)
     dec = C.decode(keys.decrypt_coef(out, P.output.level), P.output.scale, N)
     assert C.precision_bits(dec, z) > 20
+
+
+# ------------------------------------------------------------------ sparse packing
+@pytest.mark.parametrize("ns", [2, 8, 64, 256])
+def test_sparse_fold_matches_slot_maps(ns):
+    """The folded transforms on 2 ns-periodic vectors: CoeffToSlot then
+    x + conj(x) gives [2 Re w | 2 Im w] for w the ns-slot CoeffToSlot of the
+    message; SlotToCoeff of [Re w | Im w] gives the ns-slot SlotToCoeff of w,
+    replicated; every baby-step/giant-step plan evaluates its folded group."""
+    N = 1 << 12
+    rng = np.random.default_rng(ns)
+    cts = B.transform_groups(ns, N, 3, inverse=True)
+    stc = B.transform_groups(ns, N, 3, inverse=False)
+    cts2, stc2 = B.sparse_fold(cts, stc, ns)
+    z = rng.normal(size=ns) + 1j * rng.normal(size=ns)
+    w = z.copy()
+    for D in cts:
+        w = B.diag_apply(D, w)
+    x = np.tile(z, 2)
+    for D in cts2:
+        P = B.plan_group(D, 2 * ns)
+        x = B.bsgs_apply(P, x)
+    assert np.allclose(x + np.conj(x), np.concatenate([2 * w.real, 2 * w.imag]), atol=1e-9)
+    y = np.concatenate([w.real, w.imag])
+    for D in stc2:
+        y = B.bsgs_apply(B.plan_group(D, 2 * ns), y)
+    back = w.copy()
+    for D in stc:
+        back = B.diag_apply(D, back)
+    assert np.allclose(y, np.tile(back, 2), atol=1e-9)
+    assert np.allclose(back, z, atol=1e-9)   # the two maps are inverse
+
+
+def test_sparse_plan_levels_and_rotations():
+    prm = params.boot_small()
+    N = prm.N
+    full = B.BootstrapPlan(prm, None)
+    bp = B.BootstrapPlan(prm, None, B.BootConfig(slots=64))
+    assert bp.sparse and bp.n == 128 and bp.depth == full.depth
+    assert bp.trace_rotations() == [64, 128, 256, 512, 1024]   # j * 64 below N/2 = 2048
+    assert set(bp.trace_rotations()) <= set(bp.rotations())
+    assert all(0 < r < N // 2 for r in bp.rotations())
+    small = B.BootstrapPlan(prm, None, B.BootConfig(slots=4))   # 2 stages: fewer groups than the budget
+    assert len(small.cts_plans) == 2 and small.depth == full.depth - 2
+    for bad in (1, 3, N):
+        with pytest.raises(ValueError):
+            B.BootstrapPlan(prm, None, B.BootConfig(slots=bad))
+
+
+@pytest.mark.parametrize("ns,encaps", [(2, False), (64, False), (256, True), (1024, True)])
+def test_sparse_bootstrap_on_the_oracle_decrypts(ns, encaps):
+    cfg = B.BootConfig.encapsulated(slots=ns) if encaps else B.BootConfig(slots=ns)
+    bits, P = _oracle_bootstrap(cfg, seed=ns, h=None if encaps else 64)
+    # one EvalMod: half the full-slot bootstrap's HMults of the series
+    full_hmul = sum(1 for o in _oracle_program(B.BootConfig()).ops if o.kind == S.HMUL)
+    assert sum(1 for o in P.ops if o.kind == S.HMUL) * 2 == full_hmul
+    assert bits > 20, bits
+
+
+def _oracle_program(cfg):
+    return B.BootstrapPlan(params.boot_small(), None, cfg, keep_host=True).program(3)
+
+
+def test_sparse_bootstrap_without_the_trace_fails():
+    # leaving out the partial trace keeps the non-subring part of q_0 I(X),
+    # which CoeffToSlot of 64 slots folds into the slots: wrong result
+    def no_trace(bp):
+        bp.trace_rotations = lambda: []
+    bits, _ = _oracle_bootstrap(B.BootConfig(slots=64), seed=7, patch=no_trace)
+    assert bits < 5, bits
