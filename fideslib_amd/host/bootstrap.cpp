@@ -560,9 +560,44 @@ std::vector<int32_t> CryptoContextImpl::EvalBootstrapSetup(const BootstrapPrecom
   return P->rotations;
 }
 
-Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t, uint32_t) {
+// numIterations > 1 (OpenFHE's iterated bootstrapping; the reference's GPU
+// path runs one iteration, api/CryptoContext.cpp:1719-1745): with
+// bt = EvalBootstrap(x, numIterations - 1), the error 2^p (x - bt) is formed
+// at the lower of the two levels (FIXEDAUTO/FLEXIBLEAUTO: EvalSub's
+// alignment; FIXEDMANUAL: both cut to the fewer limbs), bootstrapped, and added to
+// 2^p bt; the sum is 2^p x up to the second bootstrap's error, returned with
+// the scaling factor s_out 2^p (exact, no level spent on a division).  p is
+// `precision`, the bits one bootstrap keeps (2^p (x - bt) must stay below 1).
+Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numIterations, uint32_t precision) {
+  if (numIterations <= 1) return BootstrapOnce(ct0);
+  if (precision == 0 || precision > 50)
+    throw std::runtime_error("EvalBootstrap: numIterations > 1 needs the precision of one bootstrap (1..50 bits)");
+  Ciphertext x = Regular(ct0);
+  if (x->NoiseLevel == 2) x = Rescale(x);
+  const Ciphertext bt = EvalBootstrap(x, numIterations - 1, precision);
+  auto pow2 = [&](uint32_t limbs) {
+    std::vector<uint64_t> r(limbs);
+    for (uint32_t i = 0; i < limbs; ++i) r[i] = detail::powmod(2, precision, q[i]);
+    return r;
+  };
+  Ciphertext e;
+  if (technique == FIXEDMANUAL) {
+    const uint32_t l = std::min(x->limbs, bt->limbs);
+    e = EvalSub(DropToLimbs(x, l), DropToLimbs(bt, l));
+  } else {
+    e = EvalSub(x, bt);
+  }
+  e = MulConst(e, pow2(e->limbs));
+  const Ciphertext be = BootstrapOnce(e);
+  Ciphertext r = EvalAdd(MulConst(bt, pow2(bt->limbs)), be);
+  r->NoiseFactor = bt->NoiseFactor * std::ldexp(1.0, (int)precision);
+  return r;
+}
+
+Ciphertext CryptoContextImpl::BootstrapOnce(const Ciphertext& ct0) {
   if (!boot) throw std::runtime_error("EvalBootstrap: call EvalBootstrapSetup first");
-  const Ciphertext ct = Regular(ct0);
+  Ciphertext ct = Regular(ct0);
+  if (ct->NoiseLevel == 2) ct = Rescale(ct);  // as the reference's ModRaise (Bootstrap.cu:423)
   if (!evalMultKey) throw std::runtime_error("EvalBootstrap: relinearization key not loaded");
   const BootPlan& P = *boot;
   if (P.cfg.encapsulation && (!encapDown || !encapUp))

@@ -380,8 +380,13 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   // Switching keys of the sparse-secret encapsulation: s -> s' and s' -> s
   // (GetSecretSwitchingKey, reference Bootstrap.cu:259, 596).
   void AddEncapsulationKeys(const RawKeySwitchKey& toSparse, const RawKeySwitchKey& fromSparse);
-  // numIterations and precision are accepted as the reference's GPU path
-  // accepts them (api/CryptoContext.cpp:1719-1745): one iteration runs.
+  // numIterations > 1: iterated bootstrapping (OpenFHE's meaning of the
+  // arguments; the reference's GPU path runs one iteration,
+  // api/CryptoContext.cpp:1719-1745): each further iteration bootstraps the
+  // error 2^precision (x - result) and adds it back; `precision` is the bits
+  // one bootstrap keeps.  The result's scaling factor is the bootstrap's
+  // output factor times 2^precision per extra iteration (exact; no level is
+  // spent on a division).
   Ciphertext EvalBootstrap(const Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0);
   void EvalBootstrapInPlace(Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0) {
     ct = EvalBootstrap(ct, numIterations, precision);
@@ -421,6 +426,7 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   bool AdjustForAddOrSub(Ciphertext& a, const Ciphertext& b);
   bool AdjustForMult(Ciphertext& a, const Ciphertext& b);
   Ciphertext AddSub(const Ciphertext& a, const Ciphertext& b, bool sub);
+  Ciphertext BootstrapOnce(const Ciphertext& a);
 
   fides_context* ctx = nullptr;
   RawParams raw;

@@ -272,8 +272,8 @@ struct State {
       meta << "rots";
       for (int32_t r : rots) meta << " " << r;
       meta << "\n";
-    } else if (op == "boot") {
-      ct[w.at(1)] = cc->EvalBootstrap(C(w.at(2)));
+    } else if (op == "boot") {  // boot out in [iterations precision]
+      ct[w.at(1)] = cc->EvalBootstrap(C(w.at(2)), w.size() > 3 ? (uint32_t)I(3) : 1, w.size() > 4 ? (uint32_t)I(4) : 0);
     } else if (op == "setlevel") {
       Ciphertext a = C(w.at(2));
       CryptoContextImpl::SetLevel(a, (size_t)I(3));

@@ -12,7 +12,8 @@ And the host C++ API's EvalBootstrap (N = 2^12): driven by the Python plan's
 exported tables it equals the oracle running that plan, bit for bit
 (FIXEDMANUAL; FLEXIBLEAUTO with encapsulation); its own plan with
 encapsulation under FLEXIBLEAUTO lands on the FLEXIBLEAUTO factor and
-decrypts to > 20 bits.
+decrypts to > 20 bits; two iterations (numIterations = 2) gain at least 10
+bits over one.
 """
 import time
 
@@ -199,3 +200,44 @@ def test_cpp_bootstrap_own_plan_flexible_encapsulated(tmp_path):
     bits = C.precision_bits(dec, z)
     say("C++ EvalBootstrap, own plan, FLEXIBLEAUTO + encapsulation: %.1f bits" % bits)
     assert bits > 20
+
+
+@pytest.mark.parametrize("variant", ["fixed", "flexible_encapsulated"])
+def test_cpp_bootstrap_two_iterations(tmp_path, variant):
+    """EvalBootstrap(ct, 2, p): the error 2^p (x - BTS(x)) is bootstrapped
+    and added back; the result carries the output factor times 2^p."""
+    prm = params.boot_small()
+    N = prm.N
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    flex = variant != "fixed"
+    keys = C.Keys(octx, 11, h=None if flex else 64)
+    cfg = B.BootConfig.encapsulated(flexible=True) if flex else B.BootConfig()
+    bp = B.BootstrapPlan(prm, None, cfg)  # the rotation set of the C++ plan
+    rot = host_keys(bp, keys, N)
+    sf = params.flexible_scaling_factors(prm.q) if flex else [2.0 ** prm.scale_bits] * prm.L
+    rng = np.random.default_rng(46)
+    z = rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2)
+    lvl, p = 3, 18
+    ct = keys.encrypt(C.encode(z, sf[lvl - 1], N), lvl)
+    s = Script(tmp_path, prm, F.FLEXIBLEAUTO if flex else F.FIXEDMANUAL)
+    s.rlk(keys.relin_key())
+    for k, v in rot.items():
+        if isinstance(k, int):
+            s.autokey(k, v)
+    if flex:
+        s.encapkeys(rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP])
+    s.op("bootsetup", 3, 3, cfg.degree, cfg.double_angle, cfg.K, cfg.correction, 1 if flex else 0)
+    s.ct("x", ct, 1, sf[lvl - 1], N // 2)
+    s.op("boot", "y1", "x")
+    s.op("boot", "y2", "x", 2, p)
+    s.store("y1", "y2")
+    res = s.run()
+    bits = []
+    for name in ("y1", "y2"):
+        y = res[name]
+        bits.append(C.precision_bits(C.decode(keys.decrypt_coef(y["data"], y["limbs"]), y["sf"], N), z))
+    lo = prm.L - bp.depth
+    assert res["y2"]["limbs"] == lo
+    assert res["y2"]["sf"] == res["y1"]["sf"] * 2.0 ** p
+    say("C++ EvalBootstrap (%s): one iteration %.1f bits, two iterations %.1f bits" % (variant, bits[0], bits[1]))
+    assert bits[1] >= bits[0] + 10, bits
