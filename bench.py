@@ -254,6 +254,45 @@ def weighted_valu_bound(prm, ms, clocks):
                         "profiles/ntt_fwd_960x65536.json (clock per pass)"]}
 
 
+def step_alg_bytes(prm, ctx, levels):
+    """Algorithmic HBM bytes of one batched step (SURVEY's unit costs,
+    one limb = N 8 B): the HMult batch reads 4 l and writes 2 (l - 1) limbs
+    per ciphertext and streams the relinearization key once (2 beta (l + K)
+    limbs at the largest l); every rotation reads and writes 2 (l - 1) limbs
+    and its own key (2 beta (l - 1 + K) limbs)."""
+    K, lmax = prm.K, max(levels)
+    hm = sum(4 * l + 2 * (l - 1) for l in levels) + 2 * ctx.digits(lmax) * (lmax + K)
+    rt = sum(4 * (l - 1) + 2 * ctx.digits(l - 1) * (l - 1 + K) for l in levels)
+    return prm.N * 8 * (hm + rt)
+
+
+def step_roofline(prm, ctx, levels, ms):
+    """The batched step against HBM (algorithmic bytes over the event-timed
+    step) and against VALU issue (the step's VALU wave instructions from the
+    counter run in profiles/keyswitch_step_cfg3.json, at one 4-cycle slot
+    each and at the valu_probe median cost, over 1024 SIMDs at the clock the
+    counter run measured)."""
+    alg = step_alg_bytes(prm, ctx, levels)
+    out = {"alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+           "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    fp = os.path.join(ROOT, "profiles", "keyswitch_step_cfg3.json")
+    fr = os.path.join(ROOT, "profiles", "valu_rates_gfx950.json")
+    if os.path.exists(fp):
+        st = json.load(open(fp))["step"]
+        inst, ghz = st["valu_wave_instructions"], st["clock_ghz"]
+        t4 = inst * 4 / (1024 * ghz * 1e9)
+        out.update({"hbm_bytes_counted": st.get("hbm_bytes"),
+                    "valu_wave_instructions": inst, "profiled_clock_ghz": ghz,
+                    "profiled_valu_busy": st["valu_busy"],
+                    "valu_issue_bound_ms": round(t4 * 1e3, 3), "valu_frac": round(t4 / (ms * 1e-3), 4)})
+        if os.path.exists(fr):
+            c = float(np.median(list(json.load(open(fr))["ops"].values())))
+            tw = inst * c / (1024 * ghz * 1e9)
+            out.update({"valu_issue_bound_ms_weighted": round(tw * 1e3, 3),
+                        "valu_frac_weighted": round(tw / (ms * 1e-3), 4), "cycles_per_instruction": round(c, 3)})
+    return out
+
+
 def ntt_roofline_cfg2(ev, reps=50):
     """The forward NTT at the shape the 70%-of-HBM target is quoted on: one
     fides_ntt call over 32 limbs of N = 2^16 (FIDESlib's 51-bit test primes),
@@ -549,6 +588,22 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
                 "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
             out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in roof.values()))
     res = {}
+
+    def step_dispatches(rws):
+        """Dispatch ids of the last batched step of a counter run: from the
+        HMult batch's tensor product to the first roofline launch, without
+        the calibration copy."""
+        first = {}
+        for r in rws:
+            d = r["Dispatch_Id"]
+            if d not in first:
+                first[d] = (int(r["Start_Timestamp"]), r["Kernel_Name"].split("(")[0], int(r["Grid_Size"]))
+        seq = sorted(first.items(), key=lambda kv: kv[1][0])
+        r0 = next((i for i, (_, (_, k, g)) in enumerate(seq) if g == roof_threads and k in ROOF_KERNELS), None)
+        s0 = max((i for i in range(r0 or 0) if seq[i][1][1] == "k_tensor_batch"), default=None)
+        if r0 is None or s0 is None:
+            return set()
+        return {d for d, (_, k, _) in seq[s0:r0] if "copyBuffer" not in k}
     if fetch_dir and write_dir:
         def counter(d, name):
             vals = {}
@@ -576,6 +631,16 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
         out.append("HBM traffic per roofline launch: %.1f MB (fetch scale %.3f, write scale %.3f "
                    "from a 256 MiB calibration copy); algorithmic %.1f MB" % (
                        tot / 1e6, fs, ws, res["alg_bytes_per_launch"] / 1e6))
+        # the last batched step's counted traffic
+        step_bytes = 0.0
+        for d, name, sc in ((fetch_dir, "FETCH_SIZE", fs), (write_dir, "WRITE_SIZE", ws)):
+            rws = rows(d, "*counter_collection.csv")
+            ids = step_dispatches(rws)
+            step_bytes += sum(float(r["Counter_Value"]) * 1024.0 * sc for r in rws
+                              if r["Dispatch_Id"] in ids and r["Counter_Name"] == name)
+        res.setdefault("step", {})["hbm_bytes"] = int(step_bytes)
+        out.append("HBM traffic of the last batched step (FETCH_SIZE + WRITE_SIZE, scaled): %.1f MB"
+                   % (step_bytes / 1e6))
     if valu_dir:
         # per pass of the roofline launch: VALU instructions per butterfly
         # (wave instructions x 64 lanes / butterflies of the pass) and VALU
@@ -611,6 +676,26 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
                        "valu_busy": round(4 * act / (simds * gui / xcds), 3),
                        "clock_ghz": round(gui / xcds / ns, 3), "launches": len(ds)}
         res["valu"] = vres
+        # the last batched step: VALU wave instructions, busy and clock over its dispatches
+        vrows = rows(valu_dir, "*counter_collection.csv")
+        ids = step_dispatches(vrows)
+        st = {n: 0.0 for n in names}
+        dur = {}
+        for r in vrows:
+            if r["Dispatch_Id"] in ids and r["Counter_Name"] in names:
+                st[r["Counter_Name"]] += float(r["Counter_Value"])
+                dur[r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        if ids and st["GRBM_GUI_ACTIVE"] > 0:
+            kns = sum(dur.values())
+            step = res.setdefault("step", {})
+            step.update({"dispatches": len(ids), "kernel_ns_counter_run": kns,
+                         "valu_wave_instructions": int(st["SQ_INSTS_VALU"]),
+                         "valu_busy": round(4 * st["SQ_ACTIVE_INST_VALU"] / (simds * st["GRBM_GUI_ACTIVE"] / xcds), 4),
+                         "clock_ghz": round(st["GRBM_GUI_ACTIVE"] / xcds / kns, 4)})
+            out.append("")
+            out.append("last batched step in the counter run: %d dispatches, %.3f ms of kernel time, %.3g VALU "
+                       "wave instructions, VALU busy %.1f%% at %.2f GHz" % (
+                           len(ids), kns / 1e6, st["SQ_INSTS_VALU"], 100 * step["valu_busy"], step["clock_ghz"]))
         # every kernel of the counter run: VALU busy over all its dispatches
         out.append("")
         out.append("VALU busy per kernel over the whole counter run (weighted by GRBM_GUI_ACTIVE; short "
@@ -965,6 +1050,7 @@ def main():
     value = ops_per_step / (ms_step * 1e-3)
     roof = ntt_roofline(ctx, prm, ev)
     roof["cfg2_32_limbs"] = ntt_roofline_cfg2(ev)
+    roof["batched_step"] = step_roofline(prm, ctx, w.levels, ms_batch)
     print("bench: roofline %s" % roof, file=sys.stderr, flush=True)
     out = {
         "metric": "KeySwitch ops/s N=2^16 L=29 batched vs plain, bit-exact; serve p50/p99 latency",
