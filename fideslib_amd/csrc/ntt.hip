@@ -1,7 +1,21 @@
 // Two-pass LDS negacyclic NTT / INTT kernels (see ntt.h for the design).
 #include "kernels.h"
 
-template <int LOGN, int PASS>
+// Columns per workgroup of the forward and inverse column passes (build
+// options).  16 is the measured best on MI355X at N = 2^16: 32 and 64 columns
+// give the memory system longer row segments and, at 64, wave-uniform
+// phase-B twiddles, but the passes issue VALU work at the same rate and the
+// smaller launches lose workgroups.
+#ifndef FIDES_FCOL_W
+#define FIDES_FCOL_W 16
+#endif
+#ifndef FIDES_ICOL_W
+#define FIDES_ICOL_W 16
+#endif
+constexpr int kFwdColW = FIDES_FCOL_W;
+constexpr int kInvColW = FIDES_ICOL_W;
+
+template <int LOGN, int PASS, int W = 16>
 struct NttCfg {
   static constexpr int L1 = LOGN / 2;               // log #rows (column length)
   static constexpr int L2 = LOGN - L1;              // log row length
@@ -15,7 +29,7 @@ struct NttCfg {
   static constexpr int NLINES = PASS == 0 ? (1 << L2) : (1 << L1);
   // row pass: one wave per workgroup (the LDS footprint, 8.5 KiB per wave at
   // N = 2^16, then allows 18 waves per CU instead of 16 in 4-wave groups)
-  static constexpr int LPW0 = PASS == 0 ? 16 : 64 / TPL;
+  static constexpr int LPW0 = PASS == 0 ? W : 64 / TPL;
   static constexpr int LPW = LPW0 < NLINES ? LPW0 : NLINES;  // lines per workgroup
   static constexpr int THREADS = LPW * TPL;
   static constexpr int S0 = PASS == 0 ? 0 : L1;     // first global stage
@@ -199,8 +213,8 @@ __device__ __forceinline__ u64 epilogue_apply(u64 v, u64 a, u64 b, const NttTask
 // Column pass: global stages [0, L1).  Output kept lazy (below bnd(L1) q of
 // the reduction schedule).
 template <int LOGN, int PRO, int NARROW>
-__global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T, NttFuse F) {
-  using C = NttCfg<LOGN, 0>;
+__global__ __launch_bounds__((NttCfg<LOGN, 0, kFwdColW>::THREADS)) void k_ntt_fwd_col(LimbSpan span, NttTables T, NttFuse F) {
+  using C = NttCfg<LOGN, 0, kFwdColW>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
   const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
   u64* data = tk.data;
@@ -209,7 +223,10 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
   const u64* tw = T.psi + ((u64)pid << LOGN);
   const u64* tws = T.psi_sh + ((u64)pid << LOGN);
   const u32 tid = threadIdx.x;
-  const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
+  // with 64 or more columns per workgroup a wave shares one r0, so the
+  // phase-B twiddles are wave-uniform (scalar loads)
+  constexpr bool WSB = C::LPW >= 64;
+  const u32 ll = tid % C::LPW, r0 = WSB ? __builtin_amdgcn_readfirstlane(tid / C::LPW) : tid / C::LPW;
   const u32 col = blockIdx.x * C::LPW + ll;
   u64 x[C::E];
   // The prologues load all their operands before the arithmetic, and the
@@ -247,11 +264,11 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
       fwd_bfly<true, NARROW, LOGN>(u, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
     }
   }
+  constexpr int EB = 1 << C::B;
   u64* L = lds + ll * C::LDS_LINE;
 #pragma unroll
   for (int k = 0; k < C::E; ++k) L[r0 + C::TPL * k] = x[k];
   __syncthreads();
-  constexpr int EB = 1 << C::B;
 #pragma unroll
   for (int g = 0; g < C::G; ++g) {
     const u32 r1 = r0 * C::G + g;
@@ -265,7 +282,7 @@ __global__ __launch_bounds__(256) void k_ntt_fwd_col(LimbSpan span, NttTables T,
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << u) + (r1 << v) + (k >> (C::B - v));
-        fwd_bfly<false, NARROW, LOGN>(u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2, msub);
+        fwd_bfly<WSB, NARROW, LOGN>(u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2, msub);
       }
     }
 #pragma unroll
@@ -640,8 +657,8 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
 
 // Column pass: global stages L1-1 .. 0; N^-1 folded into stage 0.  [0, 2q) -> [0, q).
 template <int LOGN>
-__global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
-  using C = NttCfg<LOGN, 0>;
+__global__ __launch_bounds__((NttCfg<LOGN, 0, kInvColW>::THREADS)) void k_intt_col(LimbSpan span, NttTables T) {
+  using C = NttCfg<LOGN, 0, kInvColW>;
   // The exchange moves 32-bit halves (low words, then high words) through a
   // line of M + 4 words per column: half the LDS of a 64-bit line, so LDS no
   // longer caps the pass at 4 waves per SIMD; the stride keeps the strided
@@ -661,7 +678,8 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
   const u64* tw = T.ipsi + ((u64)pid << LOGN);
   const u64* tws = T.ipsi_sh + ((u64)pid << LOGN);
   const u32 tid = threadIdx.x;
-  const u32 ll = tid % C::LPW, r0 = tid / C::LPW;
+  constexpr bool WSB = C::LPW >= 64;  // one r0 per wave: uniform twiddles in the first phase
+  const u32 ll = tid % C::LPW, r0 = WSB ? __builtin_amdgcn_readfirstlane(tid / C::LPW) : tid / C::LPW;
   const u32 col = blockIdx.x * C::LPW + ll;
   u32* L = lds32 + ll * S32;
   u64 x[C::E];
@@ -679,7 +697,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << u) + (r1 << v) + (k >> (C::B - v));
-        gs_bfly(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
+        gs_bfly<WSB>(x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2);
       }
     }
   }
@@ -727,7 +745,7 @@ __global__ __launch_bounds__(256) void k_intt_col(LimbSpan span, NttTables T) {
 // ------------------------------------------------------------------ launchers
 template <int LOGN, int NARROW>
 static void launch_col_n(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
-  using C0 = NttCfg<LOGN, 0>;
+  using C0 = NttCfg<LOGN, 0, kFwdColW>;
   dim3 g0(C0::NLINES / C0::LPW, s.total);
   if (pro == PRO_SWITCH_MODULUS)
     k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS, NARROW><<<g0, C0::THREADS, 0, st>>>(s, T, F);
@@ -773,7 +791,7 @@ static void launch_fwd(const LimbSpan& s, const NttTables& T, int pro, int epi, 
 
 template <int LOGN>
 static void launch_inv(const LimbSpan& s, const NttTables& T, hipStream_t st) {
-  using C0 = NttCfg<LOGN, 0>;
+  using C0 = NttCfg<LOGN, 0, kInvColW>;
   using C1 = NttCfg<LOGN, 1>;
   dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
   k_intt_row<LOGN><<<g1, C1::THREADS, 0, st>>>(s, T);
@@ -822,7 +840,7 @@ int fides_launch_intt_col(int logN, const LimbSpan& s, const NttTables& T, hipSt
   switch (logN) {
 #define CASE(n)                                                                                         \
   case n: {                                                                                             \
-    using C0 = NttCfg<n, 0>;                                                                            \
+    using C0 = NttCfg<n, 0, kInvColW>;                                                                  \
     k_intt_col<n><<<dim3(C0::NLINES / C0::LPW, s.total), C0::THREADS, 0, st>>>(s, T);                  \
     return 0;                                                                                           \
   }
