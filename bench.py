@@ -19,8 +19,9 @@ the same operation.
 Further legs: 8 full-slot bootstraps (N = 2^16, FLEXIBLEAUTO, sparse-secret
 encapsulation) captured in one HIP graph ("bootstrap"); Poisson arrivals of
 Chebyshev-series, rotate-and-sum and bootstrap requests through the
-continuous-batching loop (fideslib_amd/serve.py), reported as p50/p99 latency
-at several rates and the highest rate within the p99 bound ("serve").  With
+continuous-batching loop (fideslib_amd/serve.py) with a warm per-shape graph
+cache, reported as p50/p99 latency at several rates, the highest rate within
+the p99 bound and the same arrivals through the batched steps alone ("serve").  With
 the CPU baseline, the oracle also checks one GPU bootstrap per input level
 with real keys and reports its precision in bits.
 
@@ -844,13 +845,43 @@ def bootstrap_leg(prm, ctx, w, boot, keys, n_boot=8):
             "precision_bits": None}
 
 
+SERVE_GRAPH_SCRATCH_GIB = 2   # key-switch scratch shared by the cached request graphs
+
+
+def serve_graph_cache(prm, ctx, w, boot, keys, coef, dmin):
+    """Warm per-shape graph cache of the serving mix: one captured graph per
+    start level of the Chebyshev program (dmin..L) and of the bootstrap
+    (2..L), resident in their own device memory with one shared scratch."""
+    from fideslib_amd import serve as S
+    progs = [S.chebyshev_program(prm, l, coef, -8.0, 8.0) for l in range(dmin, prm.L + 1)]
+    progs += [boot.program(l, boot_input_scale(prm, boot, l)) for l in range(2, prm.L + 1)]
+    words = S.GraphCache.words_needed(progs)
+    gbe = S.GpuBackend(ctx, w.rlk, keys, words * 8, stream=capi.Stream())
+    scratch = capi.DeviceBuffer((SERVE_GRAPH_SCRATCH_GIB << 30) // 8)
+    cache = S.GraphCache(gbe, prm.N, SERVE_GRAPH_SCRATCH_GIB << 30, scratch_ptr=scratch.ptr)
+    cache._scratch = scratch
+    t0 = time.time()
+    for P in progs:
+        cache.add(P)
+    ctx.sync()
+    info = dict(shapes=cache.stats["shapes"], graphs=cache.stats["graphs"],
+                registers_gib=round(words * 8 / 2 ** 30, 2), scratch_gib=SERVE_GRAPH_SCRATCH_GIB,
+                task_pools_gib=round(cache.stats["task_bytes"] / 2 ** 30, 2), capture_s=round(time.time() - t0, 2))
+    return cache, info
+
+
 def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=True):
     """Poisson(rate) arrivals of the serving mix through the continuous-
-    batching loop; keys resident.  Mix: 60% degree-15 Chebyshev series of
-    the sigmoid on slots in [-8, 8] (start limbs U{7..30}), 30% rotate-and-sum
-    over 2^k slots (k ~ U{1..15}, start limbs U{1..30}), 10% full-slot
-    bootstrap (start limbs U{2..30}).  Inputs are copied from resident
-    ciphertexts at admission."""
+    batching loop with a warm per-shape graph cache (every Chebyshev and
+    bootstrap start level captured once; a request whose shape has a free
+    graph runs as one replay on the cache's stream, concurrent with the
+    batched steps; the others join the steps); keys resident.  Mix: 60%
+    degree-15 Chebyshev series of the sigmoid on slots in [-8, 8] (start
+    limbs U{7..30}), 30% rotate-and-sum over 2^k slots (k ~ U{1..15}, start
+    limbs U{1..30}), 10% full-slot bootstrap (start limbs U{2..30}).  Inputs
+    are copied from resident ciphertexts at admission.  The same arrivals
+    and programs also run through the batched steps alone
+    ("batched_only")."""
     from fideslib_amd import serve as S
     N = prm.N
     rng = np.random.default_rng(seed)
@@ -874,7 +905,15 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
             out.append(S.Request(i, P, float(arrivals[i]), inputs=[("dev", pool[P.inputs[0].level].ptr)]))
         return out
 
-    # warm-up, then capacity: every request present at t = 0
+    def kinds_of(reqs):
+        kinds = {}
+        for name in ("chebyshev15", "rotsum", "bootstrap"):
+            sub = [r for r in reqs if r.program.name.startswith(name)]
+            if sub:
+                kinds[name] = {k: round(v, 3) for k, v in S.latency_summary(sub).items()}
+        return kinds
+
+    # warm-up, then capacity of the batched steps: every request present at t = 0
     S.Server(be, N).run(make(16, np.zeros(16)), realtime=False)
     ctx.sync()
     cap_reqs = make(n_req, np.zeros(n_req))
@@ -885,21 +924,37 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
                steps=srv.stats["steps"], launches=srv.stats["launches"], ops=srv.stats["ops"],
                arena_peak_gib=round(srv.arena.peak * N * 8 / 2 ** 30, 2))
     print("bench: serve capacity %s" % cap, file=sys.stderr, flush=True)
-    # open loop at the stated rate
+    cache, cinfo = serve_graph_cache(prm, ctx, w, boot, keys, coef, dmin)
+    print("bench: serve graph cache %s" % cinfo, file=sys.stderr, flush=True)
+    # open loop at the stated rate, with the graph cache
     arr = np.cumsum(rng.exponential(1.0 / rate, n_req))
+    draw = rng.bit_generator.state
     reqs = make(n_req, arr)
     srv = S.Server(be, N)
-    srv.run(reqs, realtime=True)
+    srv.run(reqs, realtime=True, graphs=cache)
     lat = S.latency_summary(reqs)
-    kinds = {}
-    for name in ("chebyshev15", "rotsum", "bootstrap"):
-        sub = [r for r in reqs if r.program.name.startswith(name)]
-        if sub:
-            kinds[name] = {k: round(v, 3) for k, v in S.latency_summary(sub).items()}
+    # the same arrivals and programs through the batched steps alone
+    after = rng.bit_generator.state
+    rng.bit_generator.state = draw
+    reqs_b = make(n_req, arr)
+    rng.bit_generator.state = after
+    srv_b = S.Server(be, N)
+    srv_b.run(reqs_b, realtime=True)
+    lat_b = S.latency_summary(reqs_b)
+    batched_only = {
+        "note": "the same arrivals and programs without the graph cache: every request joins the batched steps",
+        "p50_ms": round(lat_b["p50_ms"], 3), "p99_ms": round(lat_b["p99_ms"], 3),
+        "mean_ms": round(lat_b["mean_ms"], 3), "p99_within_bound": bool(lat_b["p99_ms"] <= SERVE_P99_BOUND_MS),
+        "by_kind": kinds_of(reqs_b), "achieved_req_per_s": round(n_req / srv_b.stats["wall_s"], 2),
+        "steps": srv_b.stats["steps"], "launches": srv_b.stats["launches"],
+    }
+    print("bench: serve batched only %s" % {k: v for k, v in batched_only.items() if k != "note"},
+          file=sys.stderr, flush=True)
     res = {
         "workload": "cfg5: Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 60%% degree-15 Chebyshev (sigmoid "
                     "on [-8,8], start limbs U{%d..30}), 30%% rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
-                    "U{1..30}), 10%% bootstrap (start limbs U{2..30}); keys resident; admission cap %d GiB"
+                    "U{1..30}), 10%% bootstrap (start limbs U{2..30}); keys resident; admission cap %d GiB; "
+                    "warm graph cache of every Chebyshev and bootstrap start level"
                     % (dmin, SERVE_ARENA_GIB),
         "rate_req_per_s": rate,
         "requests": n_req,
@@ -908,24 +963,29 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
         "p99_bound_ms": SERVE_P99_BOUND_MS,
         "p99_within_bound": bool(lat["p99_ms"] <= SERVE_P99_BOUND_MS),
         "mean_ms": round(lat["mean_ms"], 3),
-        "by_kind": kinds,
+        "by_kind": kinds_of(reqs),
+        "graph_replays": sum(r.graph for r in reqs),
         "achieved_req_per_s": round(n_req / srv.stats["wall_s"], 2),
-        "keyswitch_ops_per_s": round(srv.stats["keyswitch_ops"] / srv.stats["wall_s"], 2),
-        "ops_per_s": round(srv.stats["ops"] / srv.stats["wall_s"], 2),
+        "keyswitch_ops_per_s_batched_steps": round(srv.stats["keyswitch_ops"] / srv.stats["wall_s"], 2),
         "steps": srv.stats["steps"],
         "launches": srv.stats["launches"],
         "arena_peak_gib": round(srv.arena.peak * N * 8 / 2 ** 30, 2),
+        "graph_cache": cinfo,
+        "batched_only": batched_only,
         "capacity_closed_batch": cap,
     }
-    # the same mix at higher arrival rates (fresh draws)
+    print("bench: serve %s" % {k: res[k] for k in ("p50_ms", "p99_ms", "graph_replays")}, file=sys.stderr,
+          flush=True)
+
+    # the same mix at higher arrival rates (fresh draws), with the graph cache
     def at_rate(r2):
         arr = np.cumsum(rng.exponential(1.0 / r2, n_req))
         rq = make(n_req, arr)
         s2 = S.Server(be, N)
-        s2.run(rq, realtime=True)
+        s2.run(rq, realtime=True, graphs=cache)
         l2 = S.latency_summary(rq)
         pt = {"rate_req_per_s": r2, "p50_ms": round(l2["p50_ms"], 3), "p99_ms": round(l2["p99_ms"], 3),
-              "achieved_req_per_s": round(n_req / s2.stats["wall_s"], 2)}
+              "achieved_req_per_s": round(n_req / s2.stats["wall_s"], 2), "graph_replays": sum(r.graph for r in rq)}
         print("bench: serve at %g req/s %s" % (r2, pt), file=sys.stderr, flush=True)
         return pt
 
@@ -953,9 +1013,10 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
         res["max_rate_within_bound"] = {
             "rate_req_per_s": lo, "p99_bound_ms": SERVE_P99_BOUND_MS, "next_failing_rate": hi,
             "probes": probes,
-            "note": "highest measured Poisson rate (fresh arrivals per probe) with p99 <= the bound; bisection "
-                    "between the passing and failing rates above to 10 req/s; p99 of 400 requests varies by "
-                    "several ms from run to run"}
+            "note": "highest measured Poisson rate (fresh arrivals per probe, graph cache on) with p99 <= the "
+                    "bound; bisection between the passing and failing rates above to 10 req/s; p99 of 400 "
+                    "requests varies by several ms from run to run"}
+    cache.close()
     return res
 
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "fides_stream_create": (_int, [ctypes.POINTER(_vp)]),
     "fides_stream_destroy": (_int, [_vp]),
     "fides_graph_begin": (_int, [_vp, _vp, ctypes.c_size_t, ctypes.c_size_t]),
+    "fides_graph_begin_shared": (_int, [_vp, _vp, _vp, ctypes.c_size_t, ctypes.c_size_t]),
     "fides_graph_end": (_int, [_vp, ctypes.POINTER(_vp)]),
     "fides_graph_launch": (_int, [_vp, _vp]),
     "fides_graph_destroy": (_int, [_vp]),
@@ -461,8 +462,16 @@ class Context:
 
     # -- graph capture: operations issued on `stream` until graph_end are
     #    recorded, not run
-    def graph_begin(self, stream: "Stream", scratch_bytes: int, task_bytes: int) -> None:
-        check(load().fides_graph_begin(self.h, stream.h, int(scratch_bytes), int(task_bytes)), "fides_graph_begin")
+    def graph_begin(self, stream: "Stream", scratch_bytes: int, task_bytes: int, scratch: int | None = None) -> None:
+        """Start capturing operations issued on `stream`; `scratch` (a device
+        pointer) makes the graph use the caller's key-switch scratch of
+        scratch_bytes instead of allocating its own."""
+        if scratch is None:
+            check(load().fides_graph_begin(self.h, stream.h, int(scratch_bytes), int(task_bytes)),
+                  "fides_graph_begin")
+        else:
+            check(load().fides_graph_begin_shared(self.h, stream.h, scratch, int(scratch_bytes), int(task_bytes)),
+                  "fides_graph_begin_shared")
 
     def graph_end(self) -> "Graph":
         h = _vp()

@@ -191,6 +191,7 @@ struct Capture {
   size_t pool_cap = 0, pool_used = 0;
   u64* arena = nullptr;
   size_t arena_words = 0;
+  bool own_arena = true;  // false: the caller's scratch (fides_graph_begin_shared)
   u64* eager_arena = nullptr;  // the context's own arena, restored at the end
   size_t eager_words = 0;
 };
@@ -626,18 +627,23 @@ struct fides_graph {
 
 static void capture_free(Capture* g) {
   if (g->pool) (void)hipFree(g->pool);
-  if (g->arena) (void)hipFree(g->arena);
+  if (g->arena && g->own_arena) (void)hipFree(g->arena);
   delete g;
 }
 
-extern "C" int fides_graph_begin(fides_context* c, void* stream, size_t scratch_bytes, size_t task_bytes) {
+// scratch: the caller's key-switch scratch (not owned), or null for one the
+// graph allocates and owns.
+static int graph_begin(fides_context* c, void* stream, void* scratch, size_t scratch_bytes, size_t task_bytes) {
   if (!c || !stream) return fail(FIDES_ERR_ARG, "graph capture needs a context and a (non-default) stream");
   if (c->cap) return fail(FIDES_ERR_ARG, "a graph capture is already in progress on this context");
   if (task_bytes == 0 || scratch_bytes == 0) return fail(FIDES_ERR_ARG, "empty graph pools");
+  if (scratch && ((uintptr_t)scratch & 255)) return fail(FIDES_ERR_ARG, "shared scratch must be 256-byte aligned");
   Capture* g = new Capture();
   g->st = (hipStream_t)stream;
+  g->own_arena = scratch == nullptr;
+  if (scratch) g->arena = (u64*)scratch;
   if (hipMalloc((void**)&g->pool, task_bytes) != hipSuccess ||
-      hipMalloc((void**)&g->arena, scratch_bytes) != hipSuccess) {
+      (!scratch && hipMalloc((void**)&g->arena, scratch_bytes) != hipSuccess)) {
     capture_free(g);
     return fail(FIDES_ERR_NOMEM, "graph pool allocation");
   }
@@ -654,6 +660,16 @@ extern "C" int fides_graph_begin(fides_context* c, void* stream, size_t scratch_
   c->arena_words = g->arena_words;
   c->cap = g;
   return FIDES_OK;
+}
+
+extern "C" int fides_graph_begin(fides_context* c, void* stream, size_t scratch_bytes, size_t task_bytes) {
+  return graph_begin(c, stream, nullptr, scratch_bytes, task_bytes);
+}
+
+extern "C" int fides_graph_begin_shared(fides_context* c, void* stream, void* scratch, size_t scratch_bytes,
+                                        size_t task_bytes) {
+  if (!scratch) return fail(FIDES_ERR_ARG, "null shared scratch");
+  return graph_begin(c, stream, scratch, scratch_bytes, task_bytes);
 }
 
 extern "C" int fides_graph_end(fides_context* c, fides_graph** out) {
@@ -680,7 +696,8 @@ extern "C" int fides_graph_end(fides_context* c, fides_graph** out) {
   fides_graph* G = new fides_graph();
   G->graph = graph;
   G->exec = exec;
-  G->mem = {g->pool, g->arena};
+  G->mem = {g->pool};
+  if (g->own_arena) G->mem.push_back(g->arena);
   delete g;
   *out = G;
   return FIDES_OK;

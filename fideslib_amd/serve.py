@@ -621,10 +621,10 @@ class GpuBackend:
         self.capi.check(self.capi.load().fides_stream_sync(self.st), "sync")
 
     # -- graph capture of the issued operations (capi.Context.graph_begin)
-    def begin_capture(self, scratch_bytes: int, task_bytes: int) -> None:
+    def begin_capture(self, scratch_bytes: int, task_bytes: int, scratch_ptr: int | None = None) -> None:
         if self.stream is None:
             raise ServeError("graph capture needs a backend created with a stream")
-        self.ctx.graph_begin(self.stream, scratch_bytes, task_bytes)
+        self.ctx.graph_begin(self.stream, scratch_bytes, task_bytes, scratch_ptr)
 
     def end_capture(self):
         return self.ctx.graph_end()
@@ -634,6 +634,12 @@ class GpuBackend:
 
     def wait(self, ev) -> None:
         self.hip.wait(ev)
+
+    def query(self, ev) -> bool:
+        return self.hip.query(ev)
+
+    def release(self, ev) -> None:
+        self.hip.release(ev)
 
 
 class _Hip:
@@ -645,6 +651,7 @@ class _Hip:
         self.lib.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.lib.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.lib.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        self.lib.hipEventQuery.argtypes = [ctypes.c_void_p]
         self.pool = []
 
     def record(self, stream=None):
@@ -663,6 +670,18 @@ class _Hip:
             raise ServeError("hipEventSynchronize failed")
         self.pool.append(e)
 
+    def query(self, e) -> bool:
+        """True once the work recorded before `e` has finished."""
+        rc = self.lib.hipEventQuery(e)
+        if rc == 0:
+            return True
+        if rc == 600:  # hipErrorNotReady
+            return False
+        raise ServeError("hipEventQuery failed (%d)" % rc)
+
+    def release(self, e) -> None:
+        self.pool.append(e)
+
 
 # ----------------------------------------------------------------- requests and the loop
 @dataclass
@@ -678,6 +697,8 @@ class Request:
     result: np.ndarray | None = None
     status: str = "pending"
     compiled: Compiled | None = None
+    graph: bool = False                 # served by a GraphCache replay
+    signature: tuple | None = None      # program_signature, computed once when a GraphCache looks it up
 
     @property
     def latency(self) -> float:
@@ -791,7 +812,8 @@ class Server:
         req.done_at = now
         req.status = "done"
 
-    def capture(self, requests, scratch_bytes: int = 4 << 30, task_bytes: int = 512 << 20) -> "CapturedBatch":
+    def capture(self, requests, scratch_bytes: int = 4 << 30, task_bytes: int = 512 << 20,
+                scratch_ptr: int | None = None) -> "CapturedBatch":
         """A closed batch recorded into one HIP graph: every request gets its
         arena storage now, then the copy of the inputs into it (one copy
         kernel) and every step of the same schedule `run` would issue are
@@ -799,7 +821,9 @@ class Server:
         every replay from where they are; host inputs are staged in device
         buffers owned by the batch.  `CapturedBatch.replay()` executes it;
         results are read after a replay.  All requests must fit the arena
-        at once."""
+        at once.  Requests with no inputs listed read their input registers
+        in place (the caller writes them before each replay).  scratch_ptr:
+        key-switch scratch shared with other graphs (fides_graph_begin_shared)."""
         active, staged, src, dst, lv = [], [], [], [], []
         try:
             for r in requests:
@@ -820,7 +844,10 @@ class Server:
                     lv.append(reg.level)
                 r.status = "running"
             self.be.sync()
-            self.be.begin_capture(scratch_bytes, task_bytes)
+            if scratch_ptr is None:
+                self.be.begin_capture(scratch_bytes, task_bytes)
+            else:
+                self.be.begin_capture(scratch_bytes, task_bytes, scratch_ptr)
             try:
                 if src:
                     self.be.drop(src, lv, lv, dst)    # the input copy, part of every replay
@@ -842,9 +869,12 @@ class Server:
             raise
         return CapturedBatch(self, graph, active, staged)
 
-    def run(self, requests, realtime: bool = True, clock=time.perf_counter):
+    def run(self, requests, realtime: bool = True, clock=time.perf_counter, graphs: "GraphCache | None" = None):
         """Serve `requests` (arrival times relative to the start).  With
-        realtime=False every request is available at once (closed batch)."""
+        realtime=False every request is available at once (closed batch).
+        graphs: a GraphCache; a request whose program has a free captured
+        graph runs as one replay of it (on the cache's stream, concurrent
+        with this loop's steps) instead of joining the batched steps."""
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
             r.compiled = _compile(r.program)
@@ -867,11 +897,18 @@ class Server:
                 self._retire(a, t)
                 active.remove(a)
 
-        while pending or queue or active:
+        while pending or queue or active or (graphs is not None and graphs.inflight):
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
                 queue.append(pending.popleft())
+            if graphs is not None and graphs.inflight:
+                graphs.poll(now)
             while queue:
+                if graphs is not None:
+                    e = graphs.free_entry(queue[0])
+                    if e is not None:
+                        graphs.start(e, queue.popleft(), t)
+                        continue
                 a = self._admit(queue[0], t)
                 if a is None:
                     break
@@ -886,6 +923,11 @@ class Server:
                     drain_one()
             elif inflight:
                 drain_one()
+            elif graphs is not None and graphs.inflight:
+                if pending or queue:
+                    time.sleep(GRAPH_POLL_S)   # an arrival or a free slot may come first
+                else:
+                    graphs.wait_one(now)
             elif pending and not queue:
                 wait = pending[0].arrival - now()
                 if wait > 0:
@@ -896,6 +938,159 @@ class Server:
             drain_one()
         self.stats["wall_s"] = now()
         return reqs
+
+
+GRAPH_POLL_S = 2e-4   # host poll interval while only captured graphs run
+
+
+def program_signature(P: Program) -> tuple:
+    """Structural identity of a request program: programs with equal
+    signatures issue the same device work on the same register layout (the
+    tracked scales are nominal and never reach the device; the integer
+    constants they produce are part of the op arguments)."""
+    def arg(o):
+        return tuple(p.ptr for p in o.arg[0]) if o.kind == PTDOT else o.arg
+    return (P.logN, tuple((r.id, r.level) for r in P.inputs), P.output.id if P.output else None,
+            tuple((o.kind, o.dst.id, o.dst.level, tuple(x.id for x in o.srcs), o.level, arg(o)) for o in P.ops))
+
+
+class _GraphEntry:
+    __slots__ = ("batch", "active", "busy", "req", "event")
+
+    def __init__(self, batch):
+        self.batch = batch
+        self.active = batch.active[0]
+        self.busy = False
+        self.req = None
+        self.event = None
+
+
+class GraphCache:
+    """Request programs captured once as HIP graphs and replayed with new
+    inputs (the reference caches CUDA graphs of its key-switch sequence,
+    src/CKKS/LimbPartitionMGPU.cu:601-609; here a whole request program is
+    one graph).
+
+    The cache is warm: the caller captures the program shapes it expects
+    (say every start level of its Chebyshev and bootstrap programs) before
+    serving; a shape is identified by program_signature.  Each captured
+    program owns a register block in the cache's own device memory (the
+    backend given here: its own stream and arena, outside the serving
+    loop's admission cap; resident for the life of the cache) and a task
+    pool.  All graphs share one key-switch scratch region (scratch_ptr), so
+    they run one at a time on the cache's stream, while the batched loop's
+    steps run concurrently on their own stream.  A replay first writes the
+    request's inputs into the captured input registers, then launches the
+    graph; an entry stays busy until its request is retired (the result
+    read back), so `copies` entries of a shape serve that many requests at
+    once."""
+
+    def __init__(self, backend, N: int, scratch_bytes: int, scratch_ptr: int | None = None,
+                 task_bytes: int = 16 << 20):
+        self.be = backend
+        self.N = N
+        self.srv = Server(backend, N)
+        self.scratch_bytes = scratch_bytes
+        self.scratch_ptr = scratch_ptr
+        self.task_bytes = task_bytes
+        self.entries = {}        # signature -> [_GraphEntry]
+        self.inflight = []       # entries running a request, in launch order
+        self.stats = dict(shapes=0, graphs=0, replays=0, capture_s=0.0, task_bytes=0)
+
+    @staticmethod
+    def words_needed(programs, copies: int = 1) -> int:
+        """Device words the register blocks of `programs` take (distinct
+        signatures, `copies` each): the size of the cache backend's arena."""
+        seen, total = set(), 0
+        for P in programs:
+            sig = program_signature(P)
+            if sig not in seen:
+                seen.add(sig)
+                total += copies * _compile(P).block * P.N
+        return total
+
+    def add(self, program: Program, copies: int = 1) -> None:
+        """Capture `program`'s shape until it has `copies` entries."""
+        sig = program_signature(program)
+        lst = self.entries.setdefault(sig, [])
+        if not lst:
+            self.stats["shapes"] += 1
+        t0 = time.perf_counter()
+        for _ in range(copies - len(lst)):
+            req = Request(-1, program, 0.0, inputs=[])   # inputs are written in place before each replay
+            tb = self.task_bytes
+            while True:
+                try:
+                    batch = self.srv.capture([req], self.scratch_bytes, tb, self.scratch_ptr)
+                    break
+                except Exception as e:  # a task pool too small for the program: capture again, larger
+                    if "task pool exhausted" not in str(e) or tb >= (1 << 31):
+                        raise
+                    tb *= 2
+            self.stats["task_bytes"] += tb
+            lst.append(_GraphEntry(batch))
+            self.stats["graphs"] += 1
+        self.stats["capture_s"] += time.perf_counter() - t0
+
+    def free_entry(self, req: Request):
+        if req.signature is None:
+            req.signature = program_signature(req.program)
+        sig = req.signature
+        for e in self.entries.get(sig, ()):
+            if not e.busy:
+                return e
+        return None
+
+    def start(self, e: _GraphEntry, req: Request, now: float) -> None:
+        a = e.active
+        for r, src in zip(a.req.program.inputs, req.inputs):
+            p = self.be.ptr(a.reg_ptr_words(r))
+            if isinstance(src, tuple) and src[0] == "dev":
+                self.be.copy(p, src[1], r.words * self.N)
+            else:
+                self.be.upload(p, src)
+        self.be.launch(e.batch.graph)
+        e.event = self.be.record()
+        e.busy, e.req = True, req
+        req.admitted_at = now
+        req.status = "running"
+        req.graph = True
+        self.inflight.append(e)
+        self.stats["replays"] += 1
+
+    def _retire(self, e: _GraphEntry, t: float) -> None:
+        req = e.req
+        if req.keep_result:
+            out = e.active.req.program.output
+            req.result = self._read_output(e.active, out)
+        req.done_at = t
+        req.status = "done"
+        e.busy, e.req, e.event = False, None, None
+        self.inflight.remove(e)
+
+    def _read_output(self, a, out):
+        return self.be.download(self.be.ptr(a.reg_ptr_words(out)), out.words * self.N).reshape(
+            2, out.level, self.N)
+
+    def poll(self, now) -> None:
+        """Retire every request whose graph has finished (non-blocking)."""
+        for e in list(self.inflight):
+            if self.be.query(e.event):
+                self.be.release(e.event)
+                self._retire(e, now())
+
+    def wait_one(self, now) -> None:
+        """Block until the oldest running graph finishes, then retire it."""
+        e = self.inflight[0]
+        self.be.wait(e.event)
+        self._retire(e, now())
+
+    def close(self) -> None:
+        for lst in self.entries.values():
+            for e in lst:
+                e.batch.close()
+        self.entries = {}
+        self.inflight = []
 
 
 class CapturedBatch:

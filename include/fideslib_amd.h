@@ -337,6 +337,16 @@ int fides_automorph(fides_context* ctx, const uint64_t* d_in, uint32_t nlimbs, u
  * kernels on the buffers named at capture time (update inputs in place to
  * run on new data); launches of one graph must not overlap each other. */
 int fides_graph_begin(fides_context* ctx, void* stream, size_t scratch_bytes, size_t task_bytes);
+/* fides_graph_begin with the key-switch scratch supplied by the caller:
+ * `scratch` (device memory of scratch_bytes, 256-byte aligned) stays the
+ * caller's and is not freed with the graph.  Graphs that share one scratch
+ * region must never run at the same time (launch them all on one stream);
+ * eager operations of the context use their own scratch and may overlap
+ * them.  No counterpart in the reference: it serves many resident graphs,
+ * one per request-program shape of a serving loop, without one scratch
+ * arena each. */
+int fides_graph_begin_shared(fides_context* ctx, void* stream, void* scratch, size_t scratch_bytes,
+                             size_t task_bytes);
 int fides_graph_end(fides_context* ctx, fides_graph** graph);
 int fides_graph_launch(fides_graph* graph, void* stream);
 /* Waits for the device, then frees the graph and its pools. */
