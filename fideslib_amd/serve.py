@@ -878,7 +878,9 @@ class Server:
         reqs = sorted(requests, key=lambda r: r.arrival)
         for r in reqs:
             r.compiled = _compile(r.program)
-            if r.compiled.block > self.arena.words:
+            # too large for the arena: rejected, unless a cached graph (with
+            # storage of its own) can serve it
+            if r.compiled.block > self.arena.words and (graphs is None or not graphs.has_shape(r)):
                 r.status = "rejected"
                 self.stats["rejected"] += 1
         pending = deque(r for r in reqs if r.status != "rejected")
@@ -1031,6 +1033,11 @@ class GraphCache:
             lst.append(_GraphEntry(batch))
             self.stats["graphs"] += 1
         self.stats["capture_s"] += time.perf_counter() - t0
+
+    def has_shape(self, req: Request) -> bool:
+        if req.signature is None:
+            req.signature = program_signature(req.program)
+        return req.signature in self.entries
 
     def free_entry(self, req: Request):
         if req.signature is None:

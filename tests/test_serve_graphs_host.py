@@ -162,3 +162,21 @@ def test_add_tops_up_to_copies():
     P = S.rotsum_program(PRM, 6, 2)
     cache = cache_for([P, S.rotsum_program(PRM, 6, 2)], copies=2)
     assert cache.stats["graphs"] == 2 and cache.stats["shapes"] == 1
+
+
+def test_cached_shape_larger_than_the_arena_is_served_by_its_graph():
+    # the loop's arena holds none of these programs; the cached shape still
+    # runs (its graph has storage of its own), the uncached one is rejected
+    P = S.chebyshev_program(PRM, 20, COEF, -8, 8)
+    cache = cache_for([P])
+    small = MockBackend(2 * 20)
+    reqs = [S.Request(0, S.chebyshev_program(PRM, 20, COEF, -8, 8), 0.0, inputs=[(hash("a"), 20)], keep_result=True),
+            S.Request(1, S.rotsum_program(PRM, 20, 3), 0.0, inputs=[(hash("b"), 20)], keep_result=True),
+            S.Request(2, S.chebyshev_program(PRM, 20, COEF, -8, 8), 0.0, inputs=[(hash("c"), 20)], keep_result=True)]
+    srv = _MockServer(small, 1)
+    srv.run(reqs, realtime=False, graphs=cache)
+    assert reqs[0].status == "done" and reqs[0].graph
+    assert reqs[0].result == sequential(reqs[0].program, [hash("a")])
+    assert reqs[1].status == "rejected"
+    assert reqs[2].status == "done" and reqs[2].graph   # waited for the entry, then replayed
+    assert reqs[2].result == sequential(reqs[2].program, [hash("c")])
