@@ -180,6 +180,24 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
     ms = ev.elapsed_ms(e0, e1) / reps
     alg_bytes = 2 * prm.N * 8 * ROOF_LIMBS
     achieved = alg_bytes / (ms * 1e-3) / 1e9
+    # the same bytes moved by a plain device-to-device copy (read + write of
+    # every limb once): what the memory system delivers to a streaming kernel.
+    # Two halves of 240 MiB, so the 256 MiB calibration copy above stays the
+    # largest copy of the run (the counter summary calibrates on it).
+    dst = capi.DeviceBuffer(ROOF_LIMBS * prm.N)
+    half = ROOF_LIMBS // 2 * prm.N
+
+    def cp():
+        for o in (0, half):
+            capi.check(capi.load().fides_copy_device(dst.ptr + 8 * o, buf.ptr + 8 * o, half * 8, None), "copy")
+    cp()
+    ctx.sync()
+    ev.record(e0)
+    for _ in range(reps):
+        cp()
+    ev.record(e1)
+    copy_ms = ev.elapsed_ms(e0, e1) / reps
+    del dst
     traffic, valu = None, None
     tf = os.path.join(ROOT, "profiles", "ntt_fwd_960x65536.json")
     if os.path.exists(tf):
@@ -214,6 +232,11 @@ def ntt_roofline(ctx, prm, ev, reps=ROOF_REPS):
         "launch_us": round(ms * 1e3, 2),
         "us_per_limb": round(ms * 1e3 / ROOF_LIMBS, 4),
         "alg_bytes_per_launch": alg_bytes,
+        "copy_same_bytes": {"us": round(copy_ms * 1e3, 2),
+                            "gbs": round(alg_bytes / (copy_ms * 1e-3) / 1e9, 1),
+                            "note": "hipMemcpyAsync device-to-device of the 960 limbs in two halves (one read "
+                                    "and one write of each): the streaming rate the memory system gives; the two "
+                                    "NTT passes move every limb twice"},
         "valu": valu,
     }
 
