@@ -873,22 +873,31 @@ SERVE_GRAPH_SCRATCH_GIB = 2   # key-switch scratch shared by the cached request 
 
 def serve_graph_cache(prm, ctx, w, boot, keys, coef, dmin):
     """Warm per-shape graph cache of the serving mix: one captured graph per
-    start level of the Chebyshev program (dmin..L) and of the bootstrap
-    (2..L), resident in their own device memory with one shared scratch."""
+    start level of the Chebyshev program (dmin..L) on one lane and of the
+    bootstrap (2..L) on a second, so short requests never wait behind a
+    bootstrap replay; each lane has its own stream, register memory and
+    key-switch scratch."""
     from fideslib_amd import serve as S
-    progs = [S.chebyshev_program(prm, l, coef, -8.0, 8.0) for l in range(dmin, prm.L + 1)]
-    progs += [boot.program(l, boot_input_scale(prm, boot, l)) for l in range(2, prm.L + 1)]
-    words = S.GraphCache.words_needed(progs)
-    gbe = S.GpuBackend(ctx, w.rlk, keys, words * 8, stream=capi.Stream())
-    scratch = capi.DeviceBuffer((SERVE_GRAPH_SCRATCH_GIB << 30) // 8)
-    cache = S.GraphCache(gbe, prm.N, SERVE_GRAPH_SCRATCH_GIB << 30, scratch_ptr=scratch.ptr)
-    cache._scratch = scratch
+    cheb = [S.chebyshev_program(prm, l, coef, -8.0, 8.0) for l in range(dmin, prm.L + 1)]
+    boots = [boot.program(l, boot_input_scale(prm, boot, l)) for l in range(2, prm.L + 1)]
+    sb = SERVE_GRAPH_SCRATCH_GIB << 30
+    words = []
+    lanes = []
+    for progs in (cheb, boots):
+        nw = S.GraphCache.words_needed(progs)
+        words.append(nw)
+        lanes.append((S.GpuBackend(ctx, w.rlk, keys, nw * 8, stream=capi.Stream()), capi.DeviceBuffer(sb // 8)))
+    cache = S.GraphCache(lanes[0][0], prm.N, sb, scratch_ptr=lanes[0][1].ptr)
+    boot_lane = cache.add_lane(lanes[1][0], scratch_ptr=lanes[1][1].ptr)
+    cache._scratch = [x for _, x in lanes]
     t0 = time.time()
-    for P in progs:
+    for P in cheb:
         cache.add(P)
+    for P in boots:
+        cache.add(P, lane=boot_lane)
     ctx.sync()
-    info = dict(shapes=cache.stats["shapes"], graphs=cache.stats["graphs"],
-                registers_gib=round(words * 8 / 2 ** 30, 2), scratch_gib=SERVE_GRAPH_SCRATCH_GIB,
+    info = dict(shapes=cache.stats["shapes"], graphs=cache.stats["graphs"], lanes=len(cache.lanes),
+                registers_gib=round(sum(words) * 8 / 2 ** 30, 2), scratch_gib=len(lanes) * SERVE_GRAPH_SCRATCH_GIB,
                 task_pools_gib=round(cache.stats["task_bytes"] / 2 ** 30, 2), capture_s=round(time.time() - t0, 2))
     return cache, info
 
@@ -977,7 +986,7 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
         "workload": "cfg5: Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 60%% degree-15 Chebyshev (sigmoid "
                     "on [-8,8], start limbs U{%d..30}), 30%% rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
                     "U{1..30}), 10%% bootstrap (start limbs U{2..30}); keys resident; admission cap %d GiB; "
-                    "warm graph cache of every Chebyshev and bootstrap start level"
+                    "warm graph cache of every Chebyshev and bootstrap start level (two lanes)"
                     % (dmin, SERVE_ARENA_GIB),
         "rate_req_per_s": rate,
         "requests": n_req,
@@ -1016,29 +1025,32 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
     if pts:
         res["load_points"] = pts
     if search:
-        # highest arrival rate whose p99 met the bound: bisection between the
-        # best passing and the lowest failing rate measured, to 10 req/s
+        # highest arrival rate whose p99 met the bound, read conservatively:
+        # only passing rates below the lowest failing one count (p99 of 400
+        # requests is noisy near the bound); bisection to 10 req/s between
+        # them, or upward probes (x1.5) while everything passes
         ok = [rate] if res["p99_within_bound"] else []
         bad = [] if res["p99_within_bound"] else [rate]
         for p in pts:
             (ok if p["p99_ms"] <= SERVE_P99_BOUND_MS else bad).append(p["rate_req_per_s"])
-        lo = max(ok) if ok else None
-        hi = min([b for b in bad if lo is None or b > lo], default=None)
         probes = []
-        while lo is not None and hi is not None and hi - lo > 10.0:
-            mid = round((lo + hi) / 2.0)
-            p = at_rate(float(mid))
+        while len(probes) < 6:
+            hi = min(bad) if bad else None
+            lo = max([r for r in ok if hi is None or r < hi], default=None)
+            if lo is None or (hi is not None and hi - lo <= 10.0):
+                break
+            nxt = float(round(lo * 1.5)) if hi is None else float(round((lo + hi) / 2.0))
+            p = at_rate(nxt)
             probes.append(p)
-            if p["p99_ms"] <= SERVE_P99_BOUND_MS:
-                lo = float(mid)
-            else:
-                hi = float(mid)
+            (ok if p["p99_ms"] <= SERVE_P99_BOUND_MS else bad).append(nxt)
+        hi = min(bad) if bad else None
+        lo = max([r for r in ok if hi is None or r < hi], default=None)
         res["max_rate_within_bound"] = {
             "rate_req_per_s": lo, "p99_bound_ms": SERVE_P99_BOUND_MS, "next_failing_rate": hi,
             "probes": probes,
             "note": "highest measured Poisson rate (fresh arrivals per probe, graph cache on) with p99 <= the "
-                    "bound; bisection between the passing and failing rates above to 10 req/s; p99 of 400 "
-                    "requests varies by several ms from run to run"}
+                    "bound and below every rate that failed it; bisection to 10 req/s, or upward probes while "
+                    "all rates pass (at most 6 probes); p99 of 400 requests varies by several ms from run to run"}
     cache.close()
     return res
 

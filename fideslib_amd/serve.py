@@ -957,14 +957,26 @@ def program_signature(P: Program) -> tuple:
 
 
 class _GraphEntry:
-    __slots__ = ("batch", "active", "busy", "req", "event")
+    __slots__ = ("batch", "active", "lane", "busy", "req", "event")
 
-    def __init__(self, batch):
+    def __init__(self, batch, lane):
         self.batch = batch
         self.active = batch.active[0]
+        self.lane = lane
         self.busy = False
         self.req = None
         self.event = None
+
+
+class _Lane:
+    """One stream of the cache: a backend (its stream and register memory), a
+    Server over that memory for capturing, and the key-switch scratch every
+    graph of the lane shares."""
+
+    def __init__(self, backend, N, scratch_ptr):
+        self.be = backend
+        self.srv = Server(backend, N)
+        self.scratch_ptr = scratch_ptr
 
 
 class GraphCache:
@@ -976,33 +988,42 @@ class GraphCache:
     The cache is warm: the caller captures the program shapes it expects
     (say every start level of its Chebyshev and bootstrap programs) before
     serving; a shape is identified by program_signature.  Each captured
-    program owns a register block in the cache's own device memory (the
-    backend given here: its own stream and arena, outside the serving
-    loop's admission cap; resident for the life of the cache) and a task
-    pool.  All graphs share one key-switch scratch region (scratch_ptr), so
-    they run one at a time on the cache's stream, while the batched loop's
-    steps run concurrently on their own stream.  A replay first writes the
-    request's inputs into the captured input registers, then launches the
-    graph; an entry stays busy until its request is retired (the result
-    read back), so `copies` entries of a shape serve that many requests at
-    once."""
+    program owns a register block in the device memory of its lane and a
+    task pool; both stay resident for the life of the cache, outside the
+    serving loop's admission cap.  A lane is one stream with its own backend
+    (register memory) and one key-switch scratch region that all of its
+    graphs share, so a lane runs its graphs one at a time; different lanes,
+    and the batched loop's steps, run concurrently.  Putting long programs
+    (bootstraps) on a lane of their own keeps short ones from waiting behind
+    them.  A replay first writes the request's inputs into the captured input
+    registers, then launches the graph; an entry stays busy until its request
+    is retired (the result read back), so `copies` entries of a shape serve
+    that many requests at once."""
 
     def __init__(self, backend, N: int, scratch_bytes: int, scratch_ptr: int | None = None,
                  task_bytes: int = 16 << 20):
-        self.be = backend
         self.N = N
-        self.srv = Server(backend, N)
+        self.lanes = [_Lane(backend, N, scratch_ptr)]
         self.scratch_bytes = scratch_bytes
-        self.scratch_ptr = scratch_ptr
         self.task_bytes = task_bytes
         self.entries = {}        # signature -> [_GraphEntry]
         self.inflight = []       # entries running a request, in launch order
         self.stats = dict(shapes=0, graphs=0, replays=0, capture_s=0.0, task_bytes=0)
 
+    @property
+    def be(self):
+        return self.lanes[0].be
+
+    def add_lane(self, backend, scratch_ptr: int | None = None) -> int:
+        """Another stream (backend with its own stream and register memory,
+        and its own scratch); returns its index for add()."""
+        self.lanes.append(_Lane(backend, self.N, scratch_ptr))
+        return len(self.lanes) - 1
+
     @staticmethod
     def words_needed(programs, copies: int = 1) -> int:
         """Device words the register blocks of `programs` take (distinct
-        signatures, `copies` each): the size of the cache backend's arena."""
+        signatures, `copies` each): the size of a lane backend's arena."""
         seen, total = set(), 0
         for P in programs:
             sig = program_signature(P)
@@ -1011,26 +1032,27 @@ class GraphCache:
                 total += copies * _compile(P).block * P.N
         return total
 
-    def add(self, program: Program, copies: int = 1) -> None:
-        """Capture `program`'s shape until it has `copies` entries."""
+    def add(self, program: Program, copies: int = 1, lane: int = 0) -> None:
+        """Capture `program`'s shape on `lane` until it has `copies` entries."""
         sig = program_signature(program)
         lst = self.entries.setdefault(sig, [])
         if not lst:
             self.stats["shapes"] += 1
+        L = self.lanes[lane]
         t0 = time.perf_counter()
         for _ in range(copies - len(lst)):
             req = Request(-1, program, 0.0, inputs=[])   # inputs are written in place before each replay
             tb = self.task_bytes
             while True:
                 try:
-                    batch = self.srv.capture([req], self.scratch_bytes, tb, self.scratch_ptr)
+                    batch = L.srv.capture([req], self.scratch_bytes, tb, L.scratch_ptr)
                     break
                 except Exception as e:  # a task pool too small for the program: capture again, larger
                     if "task pool exhausted" not in str(e) or tb >= (1 << 31):
                         raise
                     tb *= 2
             self.stats["task_bytes"] += tb
-            lst.append(_GraphEntry(batch))
+            lst.append(_GraphEntry(batch, lane))
             self.stats["graphs"] += 1
         self.stats["capture_s"] += time.perf_counter() - t0
 
@@ -1042,22 +1064,22 @@ class GraphCache:
     def free_entry(self, req: Request):
         if req.signature is None:
             req.signature = program_signature(req.program)
-        sig = req.signature
-        for e in self.entries.get(sig, ()):
+        for e in self.entries.get(req.signature, ()):
             if not e.busy:
                 return e
         return None
 
     def start(self, e: _GraphEntry, req: Request, now: float) -> None:
+        be = self.lanes[e.lane].be
         a = e.active
         for r, src in zip(a.req.program.inputs, req.inputs):
-            p = self.be.ptr(a.reg_ptr_words(r))
+            p = be.ptr(a.reg_ptr_words(r))
             if isinstance(src, tuple) and src[0] == "dev":
-                self.be.copy(p, src[1], r.words * self.N)
+                be.copy(p, src[1], r.words * self.N)
             else:
-                self.be.upload(p, src)
-        self.be.launch(e.batch.graph)
-        e.event = self.be.record()
+                be.upload(p, src)
+        be.launch(e.batch.graph)
+        e.event = be.record()
         e.busy, e.req = True, req
         req.admitted_at = now
         req.status = "running"
@@ -1069,27 +1091,27 @@ class GraphCache:
         req = e.req
         if req.keep_result:
             out = e.active.req.program.output
-            req.result = self._read_output(e.active, out)
+            req.result = self._read_output(self.lanes[e.lane].be, e.active, out)
         req.done_at = t
         req.status = "done"
         e.busy, e.req, e.event = False, None, None
         self.inflight.remove(e)
 
-    def _read_output(self, a, out):
-        return self.be.download(self.be.ptr(a.reg_ptr_words(out)), out.words * self.N).reshape(
-            2, out.level, self.N)
+    def _read_output(self, be, a, out):
+        return be.download(be.ptr(a.reg_ptr_words(out)), out.words * self.N).reshape(2, out.level, self.N)
 
     def poll(self, now) -> None:
         """Retire every request whose graph has finished (non-blocking)."""
         for e in list(self.inflight):
-            if self.be.query(e.event):
-                self.be.release(e.event)
+            be = self.lanes[e.lane].be
+            if be.query(e.event):
+                be.release(e.event)
                 self._retire(e, now())
 
     def wait_one(self, now) -> None:
         """Block until the oldest running graph finishes, then retire it."""
         e = self.inflight[0]
-        self.be.wait(e.event)
+        self.lanes[e.lane].be.wait(e.event)
         self._retire(e, now())
 
     def close(self) -> None:

@@ -6,8 +6,9 @@
   oracle running its program op by op, bit for bit.  Two requests of one
   captured shape in a row check that a replay reads its new inputs.
 * N = 2^16, 30 limbs, synthetic keys: captured Chebyshev and bootstrap
-  (FLEXIBLEAUTO, sparse-secret encapsulation) shapes give the same bits as
-  the batched loop without graphs.
+  (FLEXIBLEAUTO, sparse-secret encapsulation) shapes, on two lanes (streams
+  with their own scratch), give the same bits as the batched loop without
+  graphs.
 """
 import numpy as np
 import pytest
@@ -27,15 +28,24 @@ def say(msg):
     print(msg, flush=True)
 
 
-def make_cache(gctx, rlk, keys, programs, scratch_bytes, copies=1):
+def make_cache(gctx, rlk, keys, programs, scratch_bytes, copies=1, lanes=1):
+    """Programs spread round-robin over `lanes` lanes (stream, register
+    memory and scratch each)."""
     N = programs[0].N
-    words = S.GraphCache.words_needed(programs, copies)
-    be = S.GpuBackend(gctx, rlk, keys, words * 8, stream=capi.Stream())
-    scratch = capi.DeviceBuffer(scratch_bytes // 8)
-    cache = S.GraphCache(be, N, scratch_bytes, scratch_ptr=scratch.ptr)
-    cache._scratch = scratch   # keep alive with the cache
-    for P in programs:
-        cache.add(P, copies)
+    split = [programs[i::lanes] for i in range(lanes)]
+    cache, keep = None, []
+    for i, progs in enumerate(split):
+        words = S.GraphCache.words_needed(progs, copies)
+        be = S.GpuBackend(gctx, rlk, keys, words * 8, stream=capi.Stream())
+        scratch = capi.DeviceBuffer(scratch_bytes // 8)
+        keep.append(scratch)
+        if cache is None:
+            cache = S.GraphCache(be, N, scratch_bytes, scratch_ptr=scratch.ptr)
+        else:
+            assert cache.add_lane(be, scratch_ptr=scratch.ptr) == i
+        for P in progs:
+            cache.add(P, copies, lane=i)
+    cache._scratch = keep   # keep alive with the cache
     return cache
 
 
@@ -114,7 +124,7 @@ def test_graph_replays_equal_batched_cfg3():
     coef = S.chebyshev_coefficients(SIGMOID, 15, -8, 8)
     cheb = S.chebyshev_program(prm, 24, coef, -8, 8)
     boot = bp.program(9, sf[8])
-    cache = make_cache(gctx, rlk, keys, [cheb, boot], 2 << 30)
+    cache = make_cache(gctx, rlk, keys, [cheb, boot], 2 << 30, lanes=2)   # the bootstrap on a lane of its own
     say("captured %s" % cache.stats)
     be = S.GpuBackend(gctx, rlk, keys, 8 << 30)
     progs = [cheb, boot, S.chebyshev_program(prm, 24, coef, -8, 8), S.chebyshev_program(prm, 20, coef, -8, 8),

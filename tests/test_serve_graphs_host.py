@@ -71,8 +71,8 @@ class CaptureMock(MockBackend):
 
 
 class TokenCache(S.GraphCache):
-    def _read_output(self, a, out):
-        return self.be._read(self.be.ptr(a.reg_ptr_words(out)), out.level)
+    def _read_output(self, be, a, out):
+        return be._read(be.ptr(a.reg_ptr_words(out)), out.level)
 
 
 def cache_for(programs, copies=1):
@@ -180,3 +180,29 @@ def test_cached_shape_larger_than_the_arena_is_served_by_its_graph():
     assert reqs[1].status == "rejected"
     assert reqs[2].status == "done" and reqs[2].graph   # waited for the entry, then replayed
     assert reqs[2].result == sequential(reqs[2].program, [hash("c")])
+
+
+def test_lanes_run_their_graphs_independently():
+    # a long graph that never finishes on lane 1 does not hold up lane 0:
+    # requests of lane 0's shape keep replaying, those of lane 1's shape
+    # after the first join the batched steps
+    long_p = S.chebyshev_program(PRM, 25, COEF, -8, 8)
+    short_p = S.rotsum_program(PRM, 9, 2)
+    cache = cache_for([short_p])
+    be1 = CaptureMock(S.GraphCache.words_needed([long_p]))
+    lane = cache.add_lane(be1, scratch_ptr=777)
+    cache.add(long_p, lane=lane)
+    be1.query = lambda ev: False
+    be1.wait = lambda ev: None
+    reqs = [S.Request(0, S.chebyshev_program(PRM, 25, COEF, -8, 8), 0.0, inputs=[(hash("L0"), 25)], keep_result=True)]
+    reqs += [S.Request(1 + i, S.rotsum_program(PRM, 9, 2), 0.005 * (i + 1), inputs=[(hash(("s", i)), 9)],
+                       keep_result=True) for i in range(4)]
+    reqs.append(S.Request(9, S.chebyshev_program(PRM, 25, COEF, -8, 8), 0.001, inputs=[(hash("L1"), 25)],
+                          keep_result=True))
+    srv = _MockServer(MockBackend(300000), 1)
+    srv.run(reqs, realtime=True, graphs=cache)
+    assert reqs[0].graph and not reqs[-1].graph
+    assert all(r.graph for r in reqs[1:5])
+    assert cache.be.launches == 4 and be1.launches == 1 and be1.scratch == [777]
+    for r in reqs:
+        assert r.result == sequential(r.program, [t for t, _ in r.inputs]), r.rid
