@@ -874,23 +874,25 @@ SERVE_BOOT_LANES = 1          # lanes (streams) holding a copy of every bootstra
 
 def serve_graph_cache(prm, ctx, w, boot, keys, coef, dmin):
     """Warm per-shape graph cache of the serving mix: one captured graph per
-    start level of the Chebyshev program (dmin..L) on one lane, and one per
-    start level of the bootstrap (2..L) on each of SERVE_BOOT_LANES further
-    lanes, so short requests never wait behind a bootstrap replay and
-    bootstraps that arrive together run side by side; each lane has its own
-    stream, register memory and key-switch scratch."""
+    start level of the Chebyshev program (dmin..L) on one lane, one per start
+    level of the bootstrap (2..L) on each of SERVE_BOOT_LANES further lanes
+    (so short requests never wait behind a bootstrap replay), and one per
+    (start level, k) of the rotate-and-sum program on a last lane; each lane
+    has its own stream, register memory and key-switch scratch."""
     from fideslib_amd import serve as S
     cheb = [S.chebyshev_program(prm, l, coef, -8.0, 8.0) for l in range(dmin, prm.L + 1)]
     boots = [boot.program(l, boot_input_scale(prm, boot, l)) for l in range(2, prm.L + 1)]
+    rotsums = [S.rotsum_program(prm, l, k) for l in range(1, prm.L + 1) for k in range(1, SERVE_KMAX + 1)]
     sb = SERVE_GRAPH_SCRATCH_GIB << 30
     words = []
     lanes = []
-    for progs in [cheb] + [boots] * SERVE_BOOT_LANES:
+    for progs in [cheb] + [boots] * SERVE_BOOT_LANES + [rotsums]:
         nw = S.GraphCache.words_needed(progs)
         words.append(nw)
         lanes.append((S.GpuBackend(ctx, w.rlk, keys, nw * 8, stream=capi.Stream()), capi.DeviceBuffer(sb // 8)))
     cache = S.GraphCache(lanes[0][0], prm.N, sb, scratch_ptr=lanes[0][1].ptr)
-    boot_lanes = [cache.add_lane(be, scratch_ptr=x.ptr) for be, x in lanes[1:]]
+    boot_lanes = [cache.add_lane(be, scratch_ptr=x.ptr) for be, x in lanes[1:-1]]
+    rot_lane = cache.add_lane(lanes[-1][0], scratch_ptr=lanes[-1][1].ptr)
     cache._scratch = [x for _, x in lanes]
     t0 = time.time()
     for P in cheb:
@@ -898,6 +900,10 @@ def serve_graph_cache(prm, ctx, w, boot, keys, coef, dmin):
     for i, lane in enumerate(boot_lanes):   # one more copy of every bootstrap shape per lane
         for P in boots:
             cache.add(P, copies=i + 1, lane=lane)
+    task_bytes, cache.task_bytes = cache.task_bytes, 1 << 20   # small programs: small task pools (doubled on demand)
+    for P in rotsums:
+        cache.add(P, lane=rot_lane)
+    cache.task_bytes = task_bytes
     ctx.sync()
     info = dict(shapes=cache.stats["shapes"], graphs=cache.stats["graphs"], lanes=len(cache.lanes),
                 registers_gib=round(sum(words) * 8 / 2 ** 30, 2), scratch_gib=len(lanes) * SERVE_GRAPH_SCRATCH_GIB,
@@ -989,7 +995,7 @@ def serve_leg(prm, ctx, w, boot, keys, n_req, rate, seed, more_rates=(), search=
         "workload": "cfg5: Poisson arrivals, N=2^16, 30 Q limbs, dnum=3; mix 60%% degree-15 Chebyshev (sigmoid "
                     "on [-8,8], start limbs U{%d..30}), 30%% rotate-and-sum 2^k slots (k~U{1..15}, start limbs "
                     "U{1..30}), 10%% bootstrap (start limbs U{2..30}); keys resident; admission cap %d GiB; "
-                    "warm graph cache of every Chebyshev and bootstrap start level (Chebyshev lane + %d bootstrap lanes)"
+                    "warm graph cache of every Chebyshev, bootstrap and rotate-and-sum shape (lanes: Chebyshev, %d bootstrap, rotate-and-sum)"
                     % (dmin, SERVE_ARENA_GIB, SERVE_BOOT_LANES),
         "rate_req_per_s": rate,
         "requests": n_req,
