@@ -140,6 +140,8 @@ ROOF_LIMBS = 960  # 32 limbs for each of the 30 Q primes: the size of the batche
 ROOF_REPS = 10
 ROOF_LAUNCHES = 2 + ROOF_REPS  # the roofline's warm-up and timed launches, the first of this size in a run
 ROOF_KERNELS = ("void k_ntt_fwd_col<16, 0, 1>", "void k_ntt_fwd_row<16, 0, 1>")  # primes below 2^60
+# the two passes of a plain 32-limb transform at N = 2^16 (cfg2's 51-bit primes)
+CFG2_KERNELS = ("void k_ntt_fwd_col<16, 0, 1>", "void k_ntt_fwd_row_small<16, 1>")
 VALU_CLOCK_GHZ = 2.4  # MI355X peak engine clock
 
 
@@ -337,9 +339,15 @@ def ntt_roofline_cfg2(ev, reps=50):
     ms = ev.elapsed_ms(e0, e1) / reps
     alg = 2 * p2.N * 8 * p2.L
     gbs = alg / (ms * 1e-3) / 1e9
-    return {"kernel": "ntt_fwd, 32 limbs x 2^16 (cfg2, one fides_ntt call)", "launch_us": round(ms * 1e3, 2),
-            "us_per_limb": round(ms * 1e3 / p2.L, 4), "alg_bytes_per_launch": alg, "achieved": round(gbs, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    out = {"kernel": "ntt_fwd = k_ntt_fwd_col + k_ntt_fwd_row_small, 32 limbs x 2^16 (cfg2, one fides_ntt "
+                     "call)", "launch_us": round(ms * 1e3, 2),
+           "us_per_limb": round(ms * 1e3 / p2.L, 4), "alg_bytes_per_launch": alg, "achieved": round(gbs, 1),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    pf = os.path.join(ROOT, "profiles", "ntt_fwd_32x65536.json")
+    if os.path.exists(pf):
+        with open(pf) as f:
+            out["profiled_pass_us"] = json.load(f).get("pass_durations_us")
+    return out
 
 
 # ----------------------------------------------------------------- batch size
@@ -564,6 +572,7 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
     out = []
     tr = rows(trace_dir, "*kernel_trace.csv")
     roof_threads = 4096 * ROOF_LIMBS
+    small = {}
     if tr:
         per = {}
         for r in tr:
@@ -611,7 +620,19 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
             out.append("roofline launch (%d limbs): " % ROOF_LIMBS + ", ".join(
                 "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in roof.items()))
             out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in roof.values()))
+        for r in seq:
+            k = name(r)
+            if k in CFG2_KERNELS and int(r["Grid_Size_Y"]) == 32:
+                small.setdefault(k, []).append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        if small:
+            out.append("")
+            out.append("32-limb transforms (cfg2): " + ", ".join(
+                "%s avg %.2f us over %d" % (k, sum(v) / len(v) / 1e3, len(v)) for k, v in small.items()))
+            out.append("sum of pass averages: %.2f us" % sum(sum(v) / len(v) / 1e3 for v in small.values()))
     res = {}
+    if small:
+        res["cfg2_32_limbs"] = {"pass_durations_us": {k: round(sum(v) / len(v) / 1e3, 2) for k, v in small.items()},
+                                "launches": {k: len(v) for k, v in small.items()}}
 
     def step_dispatches(rws):
         """Dispatch ids of the last batched step of a counter run: from the

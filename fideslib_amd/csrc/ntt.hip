@@ -300,8 +300,8 @@ template <int LOGN, int EPI>
 struct RowWaves {
   static constexpr int value = LOGN != 16 ? 1 : EPI == EPI_MODDOWN_ADD ? 5 : EPI == EPI_MDRS ? 6 : 7;
 };
-template <int LOGN, int EPI, int NARROW>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LOGN, EPI>::value))) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
+template <int LOGN, int EPI, int NARROW, int PF>
+__device__ __forceinline__ void fwd_row_body(const LimbSpan& span, const NttTables& T, const NttFuse& F) {
   using C = NttCfg<LOGN, 1>;
   __shared__ u32 lds32[C::LPW * C::LDS_LINE];
   const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
@@ -317,6 +317,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LO
   u64 x[C::E];
 #pragma unroll
   for (int k = 0; k < C::E; ++k) x[k] = ldg(rowp, r0 + C::TPL * k);
+  constexpr int EB = 1 << C::B;
+  // PF: every twiddle of both phases (2^u per stage u) is loaded with the
+  // row, so no stage waits for a twiddle load
+  constexpr int NPA = PF ? C::E - 1 : 1, NPB = PF ? C::G * (EB - 1) : 1;
+  [[maybe_unused]] u64 pa[NPA], pas[NPA], pb[NPB], pbs[NPB];
+  if constexpr (PF) {
+#pragma unroll
+    for (int u = 0; u < C::A; ++u)
+#pragma unroll
+      for (int j = 0; j < (1 << u); ++j) {
+        const u32 idx = (1u << (C::S0 + u)) + (row << u) + j;
+        pa[(1 << u) - 1 + j] = tw[idx];
+        pas[(1 << u) - 1 + j] = tws[idx];
+      }
+#pragma unroll
+    for (int g = 0; g < C::G; ++g)
+#pragma unroll
+      for (int v = 0; v < C::B; ++v)
+#pragma unroll
+        for (int j = 0; j < (1 << v); ++j) {
+          const int u = C::A + v;
+          const u32 idx = (1u << (C::S0 + u)) + (row << u) + ((r0 * C::G + g) << v) + j;
+          pb[g * (EB - 1) + (1 << v) - 1 + j] = tw[idx];
+          pbs[g * (EB - 1) + (1 << v) - 1 + j] = tws[idx];
+        }
+  }
 #pragma unroll
   for (int u = 0; u < C::A; ++u) {
     const int half = C::E >> (u + 1);
@@ -324,11 +350,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LO
     for (int k = 0; k < C::E; ++k) {
       if (k & half) continue;
       const u32 idx = (1u << (C::S0 + u)) + (row << u) + (k >> (C::A - u));
-      fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
+      [[maybe_unused]] const int pi = (1 << u) - 1 + (k >> (C::A - u));
+      fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[k], x[k + half], PF ? pa[PF ? pi : 0] : tw[idx],
+                                    PF ? pas[PF ? pi : 0] : tws[idx], q, q2, msub);
     }
   }
   u32* L = lds32 + ll * C::LDS_LINE;
-  constexpr int EB = 1 << C::B;
   {  // strided -> groups of 2^B consecutive, low words then high words
     u32 lo[C::E];
 #pragma unroll
@@ -359,8 +386,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LO
       for (int k = 0; k < EB; ++k) {
         if (k & half) continue;
         const u32 idx = (1u << (C::S0 + u)) + (row << u) + (r1 << v) + (k >> (C::B - v));
-        fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[g * EB + k], x[g * EB + k + half], tw[idx], tws[idx], q, q2,
-                                      msub);
+        [[maybe_unused]] const int pi = g * (EB - 1) + (1 << v) - 1 + (k >> (C::B - v));
+        fwd_bfly<false, NARROW, LOGN>(C::S0 + u, x[g * EB + k], x[g * EB + k + half], PF ? pb[PF ? pi : 0] : tw[idx],
+                                      PF ? pbs[PF ? pi : 0] : tws[idx], q, q2, msub);
       }
     }
   }
@@ -423,6 +451,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LO
       stg(dst, c, v);
     }
   }
+}
+
+template <int LOGN, int EPI, int NARROW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(RowWaves<LOGN, EPI>::value))) void k_ntt_fwd_row(LimbSpan span, NttTables T, NttFuse F) {
+  fwd_row_body<LOGN, EPI, NARROW, 0>(span, T, F);
+}
+// 2 waves per SIMD: room for all 30 twiddle pairs of a thread
+template <int LOGN, int NARROW>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_ntt_fwd_row_small(LimbSpan span, NttTables T, NttFuse F) {
+  fwd_row_body<LOGN, EPI_NONE, NARROW, 1>(span, T, F);
 }
 
 // Inverse row stages (global stages LOGN-1 .. L1) of one row held by its TPL
@@ -752,6 +790,15 @@ __global__ __launch_bounds__((NttCfg<LOGN, 0, kInvColW>::THREADS)) void k_intt_c
 }
 
 // ------------------------------------------------------------------ launchers
+// Plain forward row passes of at most this many limbs at N = 2^16 (at most
+// two waves per SIMD, all resident at once, nothing to hide a twiddle load
+// behind) run the variant that loads all its twiddles with the row: 14.4
+// instead of 15.0 us at 32 limbs, bit-identical.  (The same for the column
+// pass's phase-B twiddles measured no gain.)
+#ifndef FIDES_SMALL_NTT
+#define FIDES_SMALL_NTT 32
+#endif
+constexpr int kSmallNttLimbs = FIDES_SMALL_NTT;
 template <int LOGN, int NARROW>
 static void launch_col_n(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
   using C0 = NttCfg<LOGN, 0, kFwdColW>;
@@ -768,6 +815,12 @@ template <int LOGN, int NARROW>
 static void launch_row_n(const LimbSpan& s, const NttTables& T, int epi, const NttFuse& F, hipStream_t st) {
   using C1 = NttCfg<LOGN, 1>;
   dim3 g1(C1::NLINES / C1::LPW, s.total);
+  if constexpr (LOGN == 16 && kSmallNttLimbs > 0) {
+    if (epi == EPI_NONE && s.total <= kSmallNttLimbs) {
+      k_ntt_fwd_row_small<LOGN, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F);
+      return;
+    }
+  }
   switch (epi) {
     case EPI_MODDOWN: k_ntt_fwd_row<LOGN, EPI_MODDOWN, NARROW><<<g1, C1::THREADS, 0, st>>>(s, T, F); break;
     case EPI_MODDOWN_ADD:
