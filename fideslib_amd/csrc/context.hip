@@ -14,6 +14,8 @@
 #include <thread>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "../../include/fideslib_amd.h"
 #include "kernels.h"
 
@@ -35,6 +37,21 @@ static int fail(int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                          \
       return fail(FIDES_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));  \
   } while (0)
+
+// Tracing: every compute entry point is one ROCTx range named after it, so
+// `rocprofv3 --marker-trace` shows each call beside the kernels it launched
+// (the reference wraps its API calls in NVTX ranges, CudaNvtxRange in
+// src/CudaUtils.cuh:26-45, e.g. api/CryptoContext.cpp:127).  Without a
+// profiler attached a range is two calls into the ROCTx library.
+namespace {
+struct TraceRange {
+  explicit TraceRange(const char* name) { roctxRangePushA(name); }
+  ~TraceRange() { roctxRangePop(); }
+  TraceRange(const TraceRange&) = delete;
+  TraceRange& operator=(const TraceRange&) = delete;
+};
+}  // namespace
+#define FIDES_TRACE() TraceRange fides_trace_range_(__func__)
 
 #define RCHK(x)               \
   do {                        \
@@ -341,6 +358,7 @@ extern "C" void fides_context_destroy(fides_context* c) {
 }
 
 extern "C" int fides_context_create(const fides_params* P, fides_context** out) {
+  FIDES_TRACE();
   if (!P || !out) return fail(FIDES_ERR_ARG, "null argument");
   *out = nullptr;
   if (P->logN < 10 || P->logN > 17) return fail(FIDES_ERR_UNSUPPORTED, "logN must be in [10, 17]");
@@ -704,6 +722,7 @@ extern "C" int fides_graph_end(fides_context* c, fides_graph** out) {
 }
 
 extern "C" int fides_graph_launch(fides_graph* g, void* stream) {
+  FIDES_TRACE();
   if (!g) return fail(FIDES_ERR_ARG, "null graph");
   HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
   return FIDES_OK;
@@ -1246,12 +1265,14 @@ static int raw_transform(fides_context* c, u64* data, const uint32_t* pids, uint
 }
 
 extern "C" int fides_ntt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return raw_transform(c, d, pids, n, false, (hipStream_t)stream);
 }
 
 extern "C" int fides_intt(fides_context* c, uint64_t* d, const uint32_t* pids, uint32_t n, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!d || !pids))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return raw_transform(c, d, pids, n, true, (hipStream_t)stream);
@@ -1259,6 +1280,7 @@ extern "C" int fides_intt(fides_context* c, uint64_t* d, const uint32_t* pids, u
 
 extern "C" int fides_modup(fides_context* c, const uint64_t* d, uint32_t l, uint32_t j, uint64_t* out,
                            void* stream) {
+  FIDES_TRACE();
   if (!c || !d || !out) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L || j >= num_digits(c, l)) return fail(FIDES_ERR_ARG, "level / digit out of range");
   hipStream_t st = (hipStream_t)stream;
@@ -1293,6 +1315,7 @@ extern "C" int fides_modup(fides_context* c, const uint64_t* d, uint32_t l, uint
 }
 
 extern "C" int fides_moddown(fides_context* c, const uint64_t* d, uint32_t l, uint64_t* out, void* stream) {
+  FIDES_TRACE();
   if (!c || !d || !out) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   hipStream_t st = (hipStream_t)stream;
@@ -1336,6 +1359,7 @@ extern "C" int fides_moddown(fides_context* c, const uint64_t* d, uint32_t l, ui
 // keys
 // ======================================================================
 extern "C" int fides_key_upload(fides_context* c, const uint64_t* h, fides_key** out) {
+  FIDES_TRACE();
   if (!c || !h || !out) return fail(FIDES_ERR_ARG, "null argument");
   const size_t LK = c->L + c->K, nl = 2 * (size_t)c->dnum * LK;
   const size_t bytes = nl * c->N * 8;
@@ -1413,6 +1437,7 @@ static int run_keyswitch(fides_context* c, const std::vector<KsJob>& jobs, hipSt
 
 extern "C" int fides_keyswitch(fides_context* c, const uint64_t* d, uint32_t l, const fides_key* key, uint64_t* o0,
                                uint64_t* o1, void* stream) {
+  FIDES_TRACE();
   if (!c) return fail(FIDES_ERR_ARG, "null context");
   RCHK(check_key(c, key, l));
   if (!d || !o0 || !o1) return fail(FIDES_ERR_ARG, "null buffer");
@@ -1531,12 +1556,14 @@ static int hmult_batch(fides_context* c, u32 n, const u64* const* a, const u64* 
 
 extern "C" int fides_hmult(fides_context* c, const uint64_t* a, const uint64_t* b, uint32_t l, const fides_key* rlk,
                            uint64_t* out, void* stream) {
+  FIDES_TRACE();
   if (!c) return fail(FIDES_ERR_ARG, "null context");
   return hmult_batch(c, 1, &a, &b, &l, rlk, false, false, &out, (hipStream_t)stream);
 }
 
 extern "C" int fides_square(fides_context* c, const uint64_t* a, uint32_t l, const fides_key* rlk, uint64_t* out,
                             void* stream) {
+  FIDES_TRACE();
   if (!c) return fail(FIDES_ERR_ARG, "null context");
   const uint64_t* b = nullptr;
   return hmult_batch(c, 1, &a, &b, &l, rlk, true, false, &out, (hipStream_t)stream);
@@ -1545,12 +1572,14 @@ extern "C" int fides_square(fides_context* c, const uint64_t* a, uint32_t l, con
 extern "C" int fides_hmult_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
                                  const uint32_t* l, const fides_key* rlk, int rescale, uint64_t* const* out,
                                  void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return hmult_batch(c, n, a, b, l, rlk, false, rescale != 0, out, (hipStream_t)stream);
 }
 
 extern "C" int fides_rescale(fides_context* c, const uint64_t* in, uint32_t l, uint64_t* out, void* stream) {
+  FIDES_TRACE();
   if (!c || !in || !out) return fail(FIDES_ERR_ARG, "null argument");
   if (l < 2 || l > c->L) return fail(FIDES_ERR_ARG, "rescale needs 2 <= l <= L");
   if (in == out) return fail(FIDES_ERR_ARG, "rescale output must not alias its input");
@@ -1570,6 +1599,7 @@ extern "C" int fides_rescale(fides_context* c, const uint64_t* in, uint32_t l, u
 
 extern "C" int fides_automorph(fides_context* c, const uint64_t* in, uint32_t n, uint32_t k, uint64_t* out,
                                void* stream) {
+  FIDES_TRACE();
   if (!c || !in || !out) return fail(FIDES_ERR_ARG, "null argument");
   if ((k & 1) == 0 || k >= 2 * c->N) return fail(FIDES_ERR_ARG, "automorphism index must be odd and < 2N");
   if (in == out) return fail(FIDES_ERR_ARG, "automorphism output must not alias its input");
@@ -1648,6 +1678,7 @@ static int rotate_batch(fides_context* c, u32 n, const u64* const* in, const u32
 extern "C" int fides_rotate_hoisted(fides_context* c, const uint64_t* in, uint32_t l, uint32_t n,
                                     const fides_key* const* keys, const uint32_t* kidx, uint64_t* const* out,
                                     void* stream) {
+  FIDES_TRACE();
   if (!c || !in || (n && (!keys || !kidx || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   hipStream_t st = (hipStream_t)stream;
@@ -1690,6 +1721,7 @@ extern "C" int fides_rotate_hoisted(fides_context* c, const uint64_t* in, uint32
 
 extern "C" int fides_rotate(fides_context* c, const uint64_t* in, uint32_t l, const fides_key* key, uint32_t k,
                             uint64_t* out, void* stream) {
+  FIDES_TRACE();
   if (!c) return fail(FIDES_ERR_ARG, "null context");
   return rotate_batch(c, 1, &in, &l, &key, &k, &out, (hipStream_t)stream);
 }
@@ -1697,6 +1729,7 @@ extern "C" int fides_rotate(fides_context* c, const uint64_t* in, uint32_t l, co
 extern "C" int fides_rotate_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
                                   const fides_key* const* keys, const uint32_t* k, uint64_t* const* out,
                                   void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l || !keys || !k || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   return rotate_batch(c, n, in, l, keys, k, out, (hipStream_t)stream);
@@ -1716,6 +1749,7 @@ static int run_lin(fides_context* c, const std::vector<LinTask>& tasks, hipStrea
 extern "C" int fides_rotate_hoisted_ext(fides_context* c, const uint64_t* in, uint32_t l, uint32_t n,
                                         const fides_key* const* keys, const uint32_t* kidx, int add_first,
                                         uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || !in || (n && (!keys || !kidx || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   hipStream_t st = (hipStream_t)stream;
@@ -1772,6 +1806,7 @@ extern "C" int fides_rotate_hoisted_ext(fides_context* c, const uint64_t* in, ui
 
 extern "C" int fides_ext_raise_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
                                      uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   const u32 K = c->K;
   std::vector<LinTask> tasks;
@@ -1798,6 +1833,7 @@ extern "C" int fides_ext_raise_batch(fides_context* c, uint32_t n, const uint64_
 
 extern "C" int fides_ext_add_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
                                    const uint32_t* l, int subtract, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   const u32 K = c->K;
   std::vector<LinTask> tasks;
@@ -1824,6 +1860,7 @@ extern "C" int fides_ext_add_batch(fides_context* c, uint32_t n, const uint64_t*
 // NTT_MODDOWN epilogue, as in fides_moddown) for n ext ciphertexts.
 extern "C" int fides_ext_moddown_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
                                        uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -1959,6 +1996,7 @@ static void fill_residues(std::vector<u64>& h, const fides_context* c, u32 nlimb
 }
 
 extern "C" int fides_autotune(fides_context* c, uint32_t n, const uint32_t* l, uint32_t reps, fides_tuning* out) {
+  FIDES_TRACE();
   if (!c || !l || n == 0) return fail(FIDES_ERR_ARG, "null argument or empty batch");
   if (c->cap) return fail(FIDES_ERR_ARG, "autotune during graph capture");
   for (u32 i = 0; i < n; ++i)
@@ -2060,6 +2098,7 @@ static LinTask lin0(const u64* a, u64* out, u32 pid) {
 
 extern "C" int fides_add_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
                                const uint32_t* l, int subtract, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !b || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LinTask> tasks;
   for (u32 i = 0; i < n; ++i) {
@@ -2079,6 +2118,7 @@ extern "C" int fides_add_batch(fides_context* c, uint32_t n, const uint64_t* con
 
 extern "C" int fides_mul_const_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint32_t* l,
                                      const uint64_t* const* res, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !l || !res || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LinTask> tasks;
   for (u32 i = 0; i < n; ++i) {
@@ -2101,6 +2141,7 @@ extern "C" int fides_mul_const_batch(fides_context* c, uint32_t n, const uint64_
 
 extern "C" int fides_add_const_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint32_t* l,
                                      const uint64_t* const* res, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !l || !res || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LinTask> tasks;
   for (u32 i = 0; i < n; ++i) {
@@ -2123,6 +2164,7 @@ extern "C" int fides_add_const_batch(fides_context* c, uint32_t n, const uint64_
 extern "C" int fides_lin_batch(fides_context* c, uint32_t n, const uint64_t* const* a, const uint64_t* const* b,
                                const uint32_t* l, const uint64_t* const* mul_res, const uint64_t* const* add_res,
                                const uint32_t* sub, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!a || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LinTask> tasks;
   for (u32 i = 0; i < n; ++i) {
@@ -2156,6 +2198,7 @@ extern "C" int fides_lin_batch(fides_context* c, uint32_t n, const uint64_t* con
 
 extern "C" int fides_drop_limbs_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l_in,
                                       const uint32_t* l_out, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l_in || !l_out || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LinTask> tasks;
   for (u32 i = 0; i < n; ++i) {
@@ -2173,6 +2216,7 @@ extern "C" int fides_drop_limbs_batch(fides_context* c, uint32_t n, const uint64
 
 extern "C" int fides_rescale_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l,
                                    uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -2200,6 +2244,7 @@ extern "C" int fides_rescale_batch(fides_context* c, uint32_t n, const uint64_t*
 // plaintexts and modulus raising (bootstrap building blocks)
 // ======================================================================
 extern "C" int fides_plaintext_prepare(fides_context* c, uint64_t* pt, uint32_t l, void* stream) {
+  FIDES_TRACE();
   if (!c || !pt) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   hipStream_t st = (hipStream_t)stream;
@@ -2212,6 +2257,7 @@ extern "C" int fides_plaintext_prepare(fides_context* c, uint64_t* pt, uint32_t 
 }
 
 extern "C" int fides_plaintext_embed(fides_context* c, const uint64_t* pt, uint32_t l, uint64_t* ct, void* stream) {
+  FIDES_TRACE();
   if (!c || !pt || !ct) return fail(FIDES_ERR_ARG, "null argument");
   if (l == 0 || l > c->L) return fail(FIDES_ERR_ARG, "level out of range");
   if (overlaps(ct, (size_t)2 * l * c->N, pt, (size_t)l * c->N)) return fail(FIDES_ERR_ARG, "output overlaps the plaintext");
@@ -2227,6 +2273,7 @@ extern "C" int fides_plaintext_embed(fides_context* c, const uint64_t* pt, uint3
 
 extern "C" int fides_ptdot_batch(fides_context* c, uint32_t n, const uint32_t* cnt, const uint64_t* const* ct,
                                  const uint64_t* const* pt, const uint32_t* l, uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!cnt || !ct || !pt || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<PtDotTask> tasks;
   std::vector<PtTerm> terms;
@@ -2276,6 +2323,7 @@ extern "C" int fides_ptdot_batch(fides_context* c, uint32_t n, const uint32_t* c
 
 extern "C" int fides_mod_raise_batch(fides_context* c, uint32_t n, const uint64_t* const* in, const uint32_t* l_out,
                                      uint64_t* const* out, void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!in || !l_out || !out))) return fail(FIDES_ERR_ARG, "null argument");
   if (n == 0) return FIDES_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -2331,6 +2379,7 @@ extern "C" int fides_lincomb_batch(fides_context* c, uint32_t n, const uint32_t*
                                    const uint32_t* x_l, const uint64_t* const* m_res,
                                    const uint64_t* const* add_res, const uint32_t* l, uint64_t* const* out,
                                    void* stream) {
+  FIDES_TRACE();
   if (!c || (n && (!cnt || !x || !x_l || !m_res || !l || !out))) return fail(FIDES_ERR_ARG, "null argument");
   std::vector<LcTask> tasks;
   std::vector<LcTerm> terms;

@@ -40,6 +40,20 @@ def test_library_exports_every_declared_symbol(lib):
         assert hasattr(lib, name), name
 
 
+def test_compute_entry_points_open_roctx_ranges(lib):
+    # the library imports the ROCTx range calls, and every compute entry point
+    # of the key-switching path opens one range named after itself
+    import subprocess
+    dyn = subprocess.run(["nm", "-D", capi.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    assert re.search(r"\bU roctxRangePushA\b", dyn) and re.search(r"\bU roctxRangePop\b", dyn)
+    src = open(os.path.join(ROOT, "fideslib_amd", "csrc", "context.hip")).read()
+    for name in ("fides_ntt", "fides_intt", "fides_modup", "fides_moddown", "fides_keyswitch", "fides_hmult",
+                 "fides_hmult_batch", "fides_rescale", "fides_rotate", "fides_rotate_batch", "fides_rotate_hoisted",
+                 "fides_lincomb_batch", "fides_ptdot_batch", "fides_graph_launch"):
+        m = re.search(r'extern "C" int ' + name + r"\([^{]*\{\n  FIDES_TRACE\(\);", src)
+        assert m, name
+
+
 def test_binding_matches_header():
     assert sorted(capi.SIGNATURES) == declared_functions()
 
