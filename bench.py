@@ -593,7 +593,8 @@ def summarize(trace_dir, fetch_dir=None, write_dir=None, valu_dir=None):
         r0 = next((i for i, r in enumerate(seq) if grid(r) == roof_threads and name(r) in ROOF_KERNELS), None)
         s0 = max((i for i in range(r0 or 0) if name(seq[i]) == "k_tensor_batch"), default=None)
         if r0 is not None and s0 is not None:
-            win, agg = seq[s0:r0], {}
+            # without the roofline's calibration copy, which runs just before it
+            win, agg = [r for r in seq[s0:r0] if "copyBuffer" not in name(r)], {}
             for r in win:
                 k = name(r)
                 if k.startswith("void k_conv_batch<") and not k.startswith("void k_conv_batch<10"):
