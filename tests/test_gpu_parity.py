@@ -54,6 +54,31 @@ def test_ntt_intt_bit_exact(logN):
     assert np.array_equal(buf.download(x.shape), x)
 
 
+@pytest.mark.parametrize("wide", [False, True])
+def test_ntt_small_launch_boundary(wide):
+    """N = 2^16: plain forward transforms of at most 32 limbs run the row
+    pass that loads its twiddles up front, larger ones the regular row pass;
+    1, 32 and 33 limbs (primes below 2^60, or with 62-bit primes among them)
+    against the oracle, and back through the INTT."""
+    logN = 16
+    ok = [x for x in PRIMES["p64"] + PRIMES["sp64"] if (x - 1) % (2 << logN) == 0]
+    narrow = [x for x in ok if x < 1 << 60]
+    q = narrow[:3] if not wide else narrow[:2] + [x for x in ok if x >= 1 << 61][:1]
+    p = [x for x in ok if x not in q][:1]
+    octx = O.Context(logN, q, p, 3)
+    gctx = capi.Context(logN, q, p, 3)
+    rng = np.random.default_rng(33 + wide)
+    for n in (1, 32, 33):
+        say("ntt N=2^16 %d limbs wide=%s" % (n, wide))
+        pids = [i % 4 for i in range(n)]
+        x = rand_limbs(rng, [octx.primes[i] for i in pids], 1 << logN)
+        buf = capi.DeviceBuffer.from_array(x)
+        gctx.ntt(buf, pids)
+        assert np.array_equal(buf.download(x.shape), octx.ntt_limbs(x, pids)), n
+        gctx.intt(buf, pids)
+        assert np.array_equal(buf.download(x.shape), x), n
+
+
 def test_ntt_edge_values():
     """All-zero, all-(q-1) and single-impulse limbs."""
     logN = 12
