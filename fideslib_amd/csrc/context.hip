@@ -913,6 +913,7 @@ struct KsLists {
   // dot tasks grouped by digit count for the fused row pass: [start, count) per beta
   u32 dot_start[ROWDOT_MAXB + 1] = {}, dot_count[ROWDOT_MAXB + 1] = {};
   bool fused_dot = false;
+  u32 conv_up_maxns = 0;  // largest source count in the mixed list conv_up[0]
   u32 n_intt_in = 0, n_conv_up[CONV_MAXS + 1] = {}, n_ntt_ext = 0, n_dot = 0, n_intt_sp = 0, n_conv_down = 0,
       n_ntt_md = 0, n_axpy1 = 0, n_intt_top = 0, n_axpy2 = 0, n_ntt_mdrs = 0;
   bool any_acc = false;
@@ -1091,7 +1092,19 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
     }
   K_.intt_in = blob.add(intt_in);
   K_.n_intt_in = (u32)intt_in.size();
-  for (u32 ns = 1; ns <= CONV_MAXS; ++ns) {
+  // Digits of different sizes (the partial last digit of each level) go into
+  // one launch (list 0, largest digits first) instead of one small launch
+  // per size.
+  u32 sizes = 0;
+  for (u32 ns = 1; ns <= CONV_MAXS; ++ns) sizes += !conv_up[ns].empty();
+  if (sizes > 1)
+    for (u32 ns = CONV_MAXS; ns >= 1; --ns) {
+      if (conv_up[ns].empty()) continue;
+      if (K_.conv_up_maxns == 0) K_.conv_up_maxns = ns;
+      conv_up[0].insert(conv_up[0].end(), conv_up[ns].begin(), conv_up[ns].end());
+      conv_up[ns].clear();
+    }
+  for (u32 ns = 0; ns <= CONV_MAXS; ++ns) {
     K_.conv_up[ns] = blob.add(conv_up[ns]);
     K_.n_conv_up[ns] = (u32)conv_up[ns].size();
   }
@@ -1119,6 +1132,10 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
 
 static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t st) {
   RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_in), K_.n_intt_in, st));
+  for (u32 o = 0; o < K_.n_conv_up[0]; o += 65535)
+    if (fides_launch_conv_mixed(at<ConvTask>(dev, K_.conv_up[0]) + o, std::min<u32>(65535, K_.n_conv_up[0] - o),
+                                K_.conv_up_maxns, c->conv_split, c->prm, c->logN, st))
+      return fail(FIDES_ERR_UNSUPPORTED, "conversion size");
   for (u32 ns = 1; ns <= CONV_MAXS; ++ns)
     if (K_.n_conv_up[ns]) RCHK(launch_conv_list(c, at<ConvTask>(dev, K_.conv_up[ns]), K_.n_conv_up[ns], ns, st));
   if (K_.fused_dot) {

@@ -128,6 +128,9 @@ struct AutoTask {
 
 int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, bool split, const PrimeC* prm, u32 logN,
                             hipStream_t st);
+// Tasks of different source counts (desc->ns, all <= maxns <= CONV_MAXS) in one launch.
+int fides_launch_conv_mixed(const ConvTask* tasks, u32 ntasks, u32 maxns, bool split, const PrimeC* prm, u32 logN,
+                            hipStream_t st);
 void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, u32 kstride,
                             hipStream_t st);
 void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);

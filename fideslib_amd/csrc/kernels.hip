@@ -54,10 +54,8 @@ __global__ __launch_bounds__(256) void k_tensor_batch(const TensorTask* __restri
 // (m mod 2^30) | (m >> 30) << 32.  Otherwise the column accumulator with
 // counted carries handles primes up to 2^62.
 template <int NS, bool SPLIT>
-__global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__ tasks,
-                                                    const PrimeC* __restrict__ prm_, u32 logN) {
-  const CONSTANT_AS ConvTask* tp = (const CONSTANT_AS ConvTask*)(tasks + blockIdx.y);
-  const CONSTANT_AS ConvDesc* d = (const CONSTANT_AS ConvDesc*)tp->desc;
+__device__ __forceinline__ void conv_task(const CONSTANT_AS ConvTask* tp, const CONSTANT_AS ConvDesc* d,
+                                          const PrimeC* __restrict__ prm_, u32 logN) {
   const CONSTANT_AS PrimeC* prm = (const CONSTANT_AS PrimeC*)prm_;
   const u64* __restrict__ in = tp->in;
   u64* __restrict__ out = tp->out;
@@ -112,6 +110,33 @@ __global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__
     acc_value(acc, hi, lo);
     // up to 16 products of values below 2^62: the high word can exceed 4q
     out[((u64)d->dst_off[t] << logN) + k] = mont_redc_any(hi, lo, P);
+  }
+}
+template <int NS, bool SPLIT>
+__global__ __launch_bounds__(256) void k_conv_batch(const ConvTask* __restrict__ tasks,
+                                                    const PrimeC* __restrict__ prm, u32 logN) {
+  const CONSTANT_AS ConvTask* tp = (const CONSTANT_AS ConvTask*)(tasks + blockIdx.y);
+  conv_task<NS, SPLIT>(tp, (const CONSTANT_AS ConvDesc*)tp->desc, prm, logN);
+}
+// Tasks with different source counts (1..MAXNS) in one launch: each
+// workgroup row branches once, uniformly, to the body for its task's count.
+// The ModUp of a batch at mixed levels has a partial last digit of a
+// different size per level; one launch for all of them replaces one launch
+// per size, each too small to fill the GPU.
+template <int MAXNS, bool SPLIT>
+__global__ __launch_bounds__(256) void k_conv_batch_mixed(const ConvTask* __restrict__ tasks,
+                                                          const PrimeC* __restrict__ prm, u32 logN) {
+  const CONSTANT_AS ConvTask* tp = (const CONSTANT_AS ConvTask*)(tasks + blockIdx.y);
+  const CONSTANT_AS ConvDesc* d = (const CONSTANT_AS ConvDesc*)tp->desc;
+  switch (d->ns) {
+#define CASE(n) \
+  case n:       \
+    if constexpr (n <= MAXNS) conv_task<n, SPLIT>(tp, d, prm, logN); \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8)
+    CASE(9) CASE(10) CASE(11) CASE(12) CASE(13) CASE(14) CASE(15) CASE(16)
+#undef CASE
+    default: break;
   }
 }
 
@@ -294,6 +319,26 @@ int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, bool spli
 #undef CASE
     default: return -1;
   }
+}
+
+int fides_launch_conv_mixed(const ConvTask* tasks, u32 ntasks, u32 maxns, bool split, const PrimeC* prm, u32 logN,
+                            hipStream_t st) {
+  if (ntasks == 0) return 0;
+  dim3 g((1u << logN) / 256, ntasks);
+  if (maxns <= 10) {
+    if (split)
+      k_conv_batch_mixed<10, true><<<g, 256, 0, st>>>(tasks, prm, logN);
+    else
+      k_conv_batch_mixed<10, false><<<g, 256, 0, st>>>(tasks, prm, logN);
+  } else if (maxns <= (u32)CONV_MAXS) {
+    if (split)
+      k_conv_batch_mixed<CONV_MAXS, true><<<g, 256, 0, st>>>(tasks, prm, logN);
+    else
+      k_conv_batch_mixed<CONV_MAXS, false><<<g, 256, 0, st>>>(tasks, prm, logN);
+  } else {
+    return -1;
+  }
+  return 0;
 }
 
 void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, u32 kstride,
