@@ -1081,7 +1081,7 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   for (const DotTask& d : dot) maxb = std::max(maxb, d.beta);
   K_.fused_dot = c->tune.fuse_row_dot[cls] && maxb <= (u32)ROWDOT_MAXB;
   std::stable_sort(dot.begin(), dot.end(), [&](const DotTask& a, const DotTask& b) {
-    if (K_.fused_dot && a.beta != b.beta) return a.beta < b.beta;
+    if (K_.fused_dot && a.beta != b.beta) return a.beta > b.beta;
     return a.pid < b.pid;
   });
   if (K_.fused_dot)
@@ -1146,11 +1146,24 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
       sp.total = std::min<u32>(65535, K_.n_ntt_ext - o);
       if (fides_launch_ntt_col(c->logN, sp, c->T, PRO_NONE, fuse0(), st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
     }
+    int counts = 0, maxb = 0;
     for (int b = 1; b <= ROWDOT_MAXB; ++b)
-      for (u32 o = 0; o < K_.dot_count[b]; o += 65535)
-        if (fides_launch_row_dot(c->logN, at<DotTask>(dev, K_.dot) + K_.dot_start[b] + o,
-                                 std::min<u32>(65535, K_.dot_count[b] - o), b, c->T, c->L + c->K, st))
+      if (K_.dot_count[b]) ++counts, maxb = b;
+    // every digit count in one launch (tasks sorted heaviest first) where the
+    // ring has that kernel
+    const bool mixed = counts > 1 && (c->logN == 16 || c->logN == 17);
+    if (mixed) {
+      for (u32 o = 0; o < K_.n_dot; o += 65535)
+        if (fides_launch_row_dot_mixed(c->logN, at<DotTask>(dev, K_.dot) + o, std::min<u32>(65535, K_.n_dot - o),
+                                       maxb, c->T, c->L + c->K, st))
           return fail(FIDES_ERR_UNSUPPORTED, "logN");
+    } else {
+      for (int b = 1; b <= ROWDOT_MAXB; ++b)
+        for (u32 o = 0; o < K_.dot_count[b]; o += 65535)
+          if (fides_launch_row_dot(c->logN, at<DotTask>(dev, K_.dot) + K_.dot_start[b] + o,
+                                   std::min<u32>(65535, K_.dot_count[b] - o), b, c->T, c->L + c->K, st))
+            return fail(FIDES_ERR_UNSUPPORTED, "logN");
+    }
   } else {
     RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_ext), K_.n_ntt_ext, PRO_NONE, EPI_NONE, st));
     for (u32 o = 0; o < K_.n_dot; o += 65535)

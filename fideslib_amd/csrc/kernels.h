@@ -156,3 +156,7 @@ int fides_launch_ntt_col(int logN, const LimbSpan& s, const NttTables& T, int pr
 constexpr int ROWDOT_MAXB = 4;
 int fides_launch_row_dot(int logN, const DotTask* tasks, u32 n, int beta, const NttTables& T, u32 kstride,
                          hipStream_t st);
+// Tasks with 1..maxb digits (2 <= maxb <= ROWDOT_MAXB) in one launch; logN
+// 16 and 17 only (returns 1 for the others).
+int fides_launch_row_dot_mixed(int logN, const DotTask* tasks, u32 n, int maxb, const NttTables& T, u32 kstride,
+                               hipStream_t st);
