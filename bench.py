@@ -297,7 +297,9 @@ def step_roofline(prm, ctx, levels, ms):
     step) and against VALU issue (the step's VALU wave instructions from the
     counter run in profiles/keyswitch_step_cfg3.json, at one 4-cycle slot
     each and at the valu_probe median cost, over 1024 SIMDs at the clock the
-    counter run measured)."""
+    counter run measured).  The *_counter_run fractions divide that bound by
+    the counter run's own kernel time (same run, same clock); the others by
+    the event-timed step of this run, whose clock is not measured."""
     alg = step_alg_bytes(prm, ctx, levels)
     out = {"alg_bytes": alg, "achieved_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
            "hbm_frac": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -307,15 +309,18 @@ def step_roofline(prm, ctx, levels, ms):
         st = json.load(open(fp))["step"]
         inst, ghz = st["valu_wave_instructions"], st["clock_ghz"]
         t4 = inst * 4 / (1024 * ghz * 1e9)
+        kt = st["kernel_ns_counter_run"] * 1e-9
         out.update({"hbm_bytes_counted": st.get("hbm_bytes"),
                     "valu_wave_instructions": inst, "profiled_clock_ghz": ghz,
-                    "profiled_valu_busy": st["valu_busy"],
-                    "valu_issue_bound_ms": round(t4 * 1e3, 3), "valu_frac": round(t4 / (ms * 1e-3), 4)})
+                    "profiled_valu_busy": st["valu_busy"], "counter_run_kernel_ms": round(kt * 1e3, 3),
+                    "valu_issue_bound_ms": round(t4 * 1e3, 3), "valu_frac": round(t4 / (ms * 1e-3), 4),
+                    "valu_frac_counter_run": round(t4 / kt, 4)})
         if os.path.exists(fr):
             c = float(np.median(list(json.load(open(fr))["ops"].values())))
             tw = inst * c / (1024 * ghz * 1e9)
             out.update({"valu_issue_bound_ms_weighted": round(tw * 1e3, 3),
-                        "valu_frac_weighted": round(tw / (ms * 1e-3), 4), "cycles_per_instruction": round(c, 3)})
+                        "valu_frac_weighted": round(tw / (ms * 1e-3), 4),
+                        "valu_frac_weighted_counter_run": round(tw / kt, 4), "cycles_per_instruction": round(c, 3)})
     return out
 
 
