@@ -546,6 +546,49 @@ def test_cfg3_batch_mixed_levels_bit_exact(cfg3):
         assert np.array_equal(r.download((2, l - 1, N)), OB[i].download((2, l - 1, N))), (i, l)
 
 
+@pytest.mark.parametrize("bits,dnum", [(62, 4), (59, 4), (59, 3)])
+def test_mixed_level_batch_every_digit_shape(bits, dnum):
+    """N = 2^16, 8 Q limbs: a batched HMult+Rescale and Rotate at levels
+    whose digit counts (1..4) and partial-digit sizes differ, so the ModUp
+    conversions of every digit size share one launch and the fused inner
+    products of every digit count share one launch; with 62-bit primes the
+    conversion takes the counted-carry path and the NTTs the wide-prime
+    butterflies.  Every ciphertext equals the single-ciphertext calls (one
+    digit shape per launch), three of them the oracle."""
+    prm = params.ckks_params(16, bits, bits, 7, dnum=dnum, special_bits=bits)
+    say("mixed-level batch N=2^16 %d-bit primes dnum=%d K=%d" % (bits, dnum, prm.K))
+    N, L = prm.N, prm.L
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    gctx = capi.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    rng = np.random.default_rng(400 + bits + dnum)
+    key = np.stack([np.stack([rand_limbs(rng, prm.q + prm.p, N) for _ in range(prm.dnum)]) for _ in range(2)])
+    gkey = gctx.upload_key(key)
+    levels = [8, 7, 5, 4, 3, 2]
+    cts = [(np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)]),
+            np.stack([rand_limbs(rng, prm.q[:l], N) for _ in range(2)])) for l in levels]
+    A = [capi.DeviceBuffer.from_array(a) for a, _ in cts]
+    B = [capi.DeviceBuffer.from_array(b) for _, b in cts]
+    OB = [capi.DeviceBuffer(2 * (l - 1) * N) for l in levels]
+    gctx.hmult_batch(A, B, levels, gkey, OB, rescale=True)
+    k = pow(5, 7, 2 * N)
+    OR = [capi.DeviceBuffer(2 * l * N) for l in levels]
+    gctx.rotate_batch(A, levels, [gkey] * len(levels), [k] * len(levels), OR)
+    for i, l in enumerate(levels):
+        m = capi.DeviceBuffer(2 * l * N)
+        r = capi.DeviceBuffer(2 * (l - 1) * N)
+        gctx.hmult(A[i], B[i], l, gkey, m)
+        gctx.rescale(m, l, r)
+        assert np.array_equal(r.download((2, l - 1, N)), OB[i].download((2, l - 1, N))), ("hmult", i, l)
+        o = capi.DeviceBuffer(2 * l * N)
+        gctx.rotate(A[i], l, gkey, k, o)
+        assert np.array_equal(o.download((2, l, N)), OR[i].download((2, l, N))), ("rotate", i, l)
+    for i in (0, 1, 4):
+        l = levels[i]
+        want = octx.rescale_ct(octx.hmult_relin(cts[i][0], cts[i][1], l, key), l)
+        assert np.array_equal(OB[i].download((2, l - 1, N)), want), ("oracle hmult", i, l)
+        assert np.array_equal(OR[i].download((2, l, N)), octx.rotate(cts[i][0], l, key, k)), ("oracle rotate", i, l)
+
+
 def test_cfg3_rotate_batch_real_keys_mixed_levels(cfg3):
     """The headline rotation shape: 16 ciphertexts at limb counts U{6..30}
     (l = 30 and l = 6 included), each with its own real rotation key (index
