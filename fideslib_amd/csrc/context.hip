@@ -898,16 +898,17 @@ static size_t ks_words(const fides_context* c, u32 l, bool shared_modup = false)
 // Appends the task lists of the batched key switch (reference pipeline
 // modup_ksk_moddown_mgpu, src/CKKS/LimbPartitionMGPU.cu:588-1873):
 //   1 INTT of the input (out of place into Dc)
-//   2 ModUp conversion of every digit (grouped by digit size)
+//   2 ModUp conversion of every digit (one launch for all digit sizes)
 //   3 NTT of every conversion output
 //   4 inner product with the key
 //   5 INTT of the special limbs of both products
 //   6 ModDown conversion P -> Q_l of both products
 //   7 NTT fused with (x - conv) * P^-1 [+ acc] into the outputs
-//   (fused Rescale jobs) top limb: acc + T P^-1 -> INTT -> - conv P^-1, then
+//   (fused Rescale jobs) top limb: acc + T P^-1 -> INTT -> - conv P^-1 (one
+//   INTT launch pair with the two element-wise steps fused in), then
 //   one NTT per remaining limb with PRO_MDRS / EPI_MDRS (see ntt.h).
 struct KsLists {
-  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, axpy1, intt_top, axpy2, ntt_mdrs,
+  size_t intt_in, conv_up[CONV_MAXS + 1], ntt_ext, dot, intt_sp, conv_down, ntt_md, intt_top, ntt_mdrs,
       auto_ext;
   u32 n_auto_ext = 0;
   // dot tasks grouped by digit count for the fused row pass: [start, count) per beta
@@ -915,7 +916,7 @@ struct KsLists {
   bool fused_dot = false;
   u32 conv_up_maxns = 0;  // largest source count in the mixed list conv_up[0]
   u32 n_intt_in = 0, n_conv_up[CONV_MAXS + 1] = {}, n_ntt_ext = 0, n_dot = 0, n_intt_sp = 0, n_conv_down = 0,
-      n_ntt_md = 0, n_axpy1 = 0, n_intt_top = 0, n_axpy2 = 0, n_ntt_mdrs = 0;
+      n_ntt_md = 0, n_intt_top = 0, n_ntt_mdrs = 0;
   bool any_acc = false;
 };
 
@@ -932,7 +933,6 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   std::vector<NttTask> intt_in, ntt_ext, intt_sp, ntt_md, intt_top, ntt_mdrs;
   std::vector<ConvTask> conv_up[CONV_MAXS + 1], conv_down;
   std::vector<DotTask> dot;
-  std::vector<AxpyTask> axpy1, axpy2;
   std::vector<AutoTask> auto_ext;
   u64* w = scratch;
   std::vector<u64*> ext_of(jobs.size(), nullptr);
@@ -1037,10 +1037,17 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
       const u32 tp = l - 1;
       const u64 qt = c->primes[tp];
       u64* rs = RS + (size_t)p * N;
-      axpy1.push_back(AxpyTask{c->limb(accp[p], tp), c->limb(Tp[p], tp), rs, c->pinv[tp], c->pinv_sh[tp], tp, 0});
-      intt_top.push_back(ntask(rs, tp));
+      // top = INTT(acc_top + T_top P^-1) - conv_top P^-1 in one fused INTT
+      NttTask top = ntask(rs, tp);
+      top.src = c->limb(accp[p], tp);
+      top.x = c->limb(Tp[p], tp);
+      top.c0 = c->pinv[tp];
+      top.c0sh = c->pinv_sh[tp];
       const u64 npinv = (qt - c->pinv[tp]) % qt;
-      axpy2.push_back(AxpyTask{rs, c->limb(Yp, tp), rs, npinv, shoup_of(npinv, qt), tp, 0});
+      top.src2 = c->limb(Yp, tp);
+      top.c1 = npinv;
+      top.c1sh = shoup_of(npinv, qt);
+      intt_top.push_back(top);
       u64* dst = J.out0 + (size_t)p * (l - 1) * N;
       for (u32 i = 0; i + 1 < l; ++i) {
         const u64 qi = c->primes[i];
@@ -1118,12 +1125,8 @@ static void ks_build(fides_context* c, const std::vector<KsJob>& jobs, u64* scra
   K_.n_conv_down = (u32)conv_down.size();
   K_.ntt_md = blob.add(ntt_md);
   K_.n_ntt_md = (u32)ntt_md.size();
-  K_.axpy1 = blob.add(axpy1);
-  K_.n_axpy1 = (u32)axpy1.size();
   K_.intt_top = blob.add(intt_top);
   K_.n_intt_top = (u32)intt_top.size();
-  K_.axpy2 = blob.add(axpy2);
-  K_.n_axpy2 = (u32)axpy2.size();
   K_.ntt_mdrs = blob.add(ntt_mdrs);
   K_.n_ntt_mdrs = (u32)ntt_mdrs.size();
   K_.auto_ext = blob.add(auto_ext);
@@ -1186,9 +1189,12 @@ static int ks_launch(fides_context* c, char* dev, const KsLists& K_, hipStream_t
     RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_md), K_.n_ntt_md, PRO_NONE,
                          K_.any_acc ? EPI_MODDOWN_ADD : EPI_MODDOWN, st));
   if (K_.n_ntt_mdrs) {
-    fides_launch_axpy_batch(at<AxpyTask>(dev, K_.axpy1), K_.n_axpy1, c->prm, c->logN, st);
-    RCHK(launch_intt_list(c, at<NttTask>(dev, K_.intt_top), K_.n_intt_top, st));
-    fides_launch_axpy_batch(at<AxpyTask>(dev, K_.axpy2), K_.n_axpy2, c->prm, c->logN, st);
+    for (u32 o = 0; o < K_.n_intt_top; o += 65535) {
+      LimbSpan sp = span0();
+      sp.list = at<NttTask>(dev, K_.intt_top) + o;
+      sp.total = std::min<u32>(65535, K_.n_intt_top - o);
+      if (fides_launch_intt_axpy(c->logN, sp, c->T, st)) return fail(FIDES_ERR_UNSUPPORTED, "logN");
+    }
     RCHK(launch_ntt_list(c, at<NttTask>(dev, K_.ntt_mdrs), K_.n_ntt_mdrs, PRO_MDRS, EPI_MDRS, st));
   }
   for (u32 o = 0; o < K_.n_auto_ext; o += 65535)

@@ -55,16 +55,6 @@ struct TensorTask {
   u32 square;
 };
 
-// One limb of out = a + b * c (mod q_pid) with a constant c.
-struct AxpyTask {
-  const u64* a;
-  const u64* b;
-  u64* out;
-  u64 c, csh;
-  u32 pid;
-  u32 pad;
-};
-
 // One output limb of sum_j x_j * m_j (mod q_pid), m_j in Montgomery form
 // (plaintext limbs prepared by fides_plaintext_prepare); terms[first ..
 // first + cnt) hold the operand limbs, cnt <= 16.
@@ -134,7 +124,6 @@ int fides_launch_conv_mixed(const ConvTask* tasks, u32 ntasks, u32 maxns, bool s
 void fides_launch_dot_batch(const DotTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, u32 kstride,
                             hipStream_t st);
 void fides_launch_tensor_batch(const TensorTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
-void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st);
 void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
                               hipStream_t st);
 void fides_launch_lincomb_batch(const LcTask* tasks, const LcTerm* terms, u32 ntasks, const PrimeC* prm, u32 logN,
@@ -145,6 +134,10 @@ void fides_launch_to_mont(u64* key, const PrimeC* prm, u32 logN, u32 nlimbs, u32
 int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, int epi, const NttFuse& F,
                      hipStream_t st);
 int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);
+// INTT of task-list limbs with an element-wise prologue and epilogue:
+// data = INTT(src + x c0) + src2 c1 (mod q), the top limb of a rescale fused
+// into ModDown.
+int fides_launch_intt_axpy(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);
 // Column pass of the inverse NTT alone (after k_ntt_row_dot ran the row
 // stages of the special limbs).
 int fides_launch_intt_col(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st);

@@ -196,15 +196,6 @@ __global__ __launch_bounds__(256) void k_automorph_batch(const AutoTask* __restr
   t.out[i] = v;
 }
 
-// ---------------------------------------------------------------- axpy
-__global__ __launch_bounds__(256) void k_axpy_batch(const AxpyTask* __restrict__ tasks,
-                                                    const PrimeC* __restrict__ prm, u32 logN) {
-  const AxpyTask t = tasks[blockIdx.y];
-  const u32 k = blockIdx.x * 256u + threadIdx.x;
-  const u64 q = prm[t.pid].q;
-  t.out[k] = add_mod(t.a[k], shoup(t.b[k], t.c, t.csh, q), q);
-}
-
 // ---------------------------------------------------------------- linear combination
 // Element-wise ciphertext arithmetic: add/sub, multiplication by an integer
 // constant, addition of a constant (reference AddSub.cu:10-123 add_/sub_/
@@ -295,11 +286,6 @@ void fides_launch_ptdot_batch(const PtDotTask* tasks, const PtTerm* terms, u32 n
 void fides_launch_lin_batch(const LinTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
   if (ntasks == 0) return;
   k_lin_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
-}
-
-void fides_launch_axpy_batch(const AxpyTask* tasks, u32 ntasks, const PrimeC* prm, u32 logN, hipStream_t st) {
-  if (ntasks == 0) return;
-  k_axpy_batch<<<dim3((1u << logN) / 256, ntasks), 256, 0, st>>>(tasks, prm, logN);
 }
 
 int fides_launch_conv_batch(const ConvTask* tasks, u32 ntasks, u32 ns, bool split, const PrimeC* prm, u32 logN,

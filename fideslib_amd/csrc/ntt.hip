@@ -687,7 +687,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RowDotWaves<
 
 // ------------------------------------------------------------------ inverse
 // Row pass: global stages LOGN-1 .. L1 (descending).  [0, q) -> [0, 2q).
-template <int LOGN>
+// AXPY (task lists only): the input is src + x c0 mod q, element-wise.
+template <int LOGN, int AXPY = 0>
 __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
   using C = NttCfg<LOGN, 1>;
   __shared__ u64 lds[C::LPW * C::LDS_LINE];
@@ -721,6 +722,14 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
     // measured faster than a coalesced load through LDS)
 #pragma unroll
     for (int k = 0; k < EB; ++k) x[g * EB + k] = ldg(srow, (r1 << C::B) + k);
+    if constexpr (AXPY) {
+      const NttTask& t = span.list[blockIdx.y];
+      const u64* xrow = t.x + ((u64)row << C::L2);
+      const u64 c0 = t.c0, c0sh = t.c0sh;
+#pragma unroll
+      for (int k = 0; k < EB; ++k)
+        x[g * EB + k] = add_mod(x[g * EB + k], shoup(ldg(xrow, (r1 << C::B) + k), c0, c0sh, q), q);
+    }
   }
   intt_row_stages<LOGN>(row, r0, L, tw, tws, q, q2, x);
 #pragma unroll
@@ -728,7 +737,8 @@ __global__ __launch_bounds__(256) void k_intt_row(LimbSpan span, NttTables T) {
 }
 
 // Column pass: global stages L1-1 .. 0; N^-1 folded into stage 0.  [0, 2q) -> [0, q).
-template <int LOGN>
+// AXPY (task lists only): the output is the transform + src2 c1 mod q.
+template <int LOGN, int AXPY = 0>
 __global__ __launch_bounds__((NttCfg<LOGN, 0, kInvColW>::THREADS)) void k_intt_col(LimbSpan span, NttTables T) {
   using C = NttCfg<LOGN, 0, kInvColW>;
   // The exchange moves 32-bit halves (low words, then high words) through a
@@ -810,6 +820,13 @@ __global__ __launch_bounds__((NttCfg<LOGN, 0, kInvColW>::THREADS)) void k_intt_c
       x[k + half] = shoup(X + q2 - Y, P.ninv_w, P.ninv_w_sh, q);
     }
   }
+  if constexpr (AXPY) {
+    const NttTask& t = span.list[blockIdx.y];
+    const u64 c1 = t.c1, c1sh = t.c1sh;
+#pragma unroll
+    for (int k = 0; k < C::E; ++k)
+      x[k] = add_mod(x[k], shoup(ldg(t.src2, ((r0 + C::TPL * k) << C::L2) | col), c1, c1sh, q), q);
+  }
 #pragma unroll
   for (int k = 0; k < C::E; ++k) stg(data, ((r0 + C::TPL * k) << C::L2) | col, x[k]);
 }
@@ -876,13 +893,13 @@ static void launch_fwd(const LimbSpan& s, const NttTables& T, int pro, int epi, 
     launch_row_n<LOGN, 0>(s, T, epi, F, st);
 }
 
-template <int LOGN>
+template <int LOGN, int AXPY = 0>
 static void launch_inv(const LimbSpan& s, const NttTables& T, hipStream_t st) {
   using C0 = NttCfg<LOGN, 0, kInvColW>;
   using C1 = NttCfg<LOGN, 1>;
   dim3 g0(C0::NLINES / C0::LPW, s.total), g1(C1::NLINES / C1::LPW, s.total);
-  k_intt_row<LOGN><<<g1, C1::THREADS, 0, st>>>(s, T);
-  k_intt_col<LOGN><<<g0, C0::THREADS, 0, st>>>(s, T);
+  k_intt_row<LOGN, AXPY><<<g1, C1::THREADS, 0, st>>>(s, T);
+  k_intt_col<LOGN, AXPY><<<g0, C0::THREADS, 0, st>>>(s, T);
 }
 
 
@@ -999,6 +1016,21 @@ int fides_launch_ntt(int logN, const LimbSpan& s, const NttTables& T, int pro, i
   }
 }
 
+int fides_launch_intt_axpy(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st) {
+  if (s.total == 0) return 0;
+  if (!s.list) return -1;
+  switch (logN) {
+    case 10: launch_inv<10, 1>(s, T, st); return 0;
+    case 11: launch_inv<11, 1>(s, T, st); return 0;
+    case 12: launch_inv<12, 1>(s, T, st); return 0;
+    case 13: launch_inv<13, 1>(s, T, st); return 0;
+    case 14: launch_inv<14, 1>(s, T, st); return 0;
+    case 15: launch_inv<15, 1>(s, T, st); return 0;
+    case 16: launch_inv<16, 1>(s, T, st); return 0;
+    case 17: launch_inv<17, 1>(s, T, st); return 0;
+    default: return -1;
+  }
+}
 int fides_launch_intt(int logN, const LimbSpan& s, const NttTables& T, hipStream_t st) {
   if (s.total == 0) return 0;
   switch (logN) {
