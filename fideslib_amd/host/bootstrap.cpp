@@ -338,7 +338,20 @@ void placeholder_pts(BootPlan& P, const std::vector<uint64_t>& q) {
 // [switch back], [partial trace (sparse packing)], CoeffToSlot, Re/Im split
 // and two EvalMods (full packing) or one EvalMod on [Re | Im] (sparse),
 // recombine, SlotToCoeff, x 2^c.
-Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fides_key* up) {
+// The bootstrap program in two parts: the input-dependent prescale down to
+// one limb, then everything from the encapsulation and ModRaise on, which
+// no longer depends on the input's level or scaling factor (ModRaise
+// declares delta), so a batch of inputs shares it in lockstep.
+Reg boot_prescale(Exec& X, const BootPlan& P, Reg x) {
+  // prescale by 2^-c; under FLEXIBLEAUTO also by q_{l-1} / s_in, so that after
+  // the Rescale the message sits at delta whatever the input level's factor
+  // (the reference's adjustment factor, src/CKKS/Bootstrap.cu:441-470)
+  double pre = std::ldexp(1.0, -(int)P.cfg.correction);
+  if (P.flexible) pre = pre * (double)X.q[x.level - 1] / x.scale;
+  x = X.mul_real(x, pre);
+  return X.drop(x, 1);
+}
+Reg boot_core(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fides_key* up) {
   auto linear = [&](Reg v, const GroupPlan& G, const GroupPts& pts) {
     // rotation amounts modulo the plan's slot period (the data repeats with
     // it), the indices plan_rotations lists
@@ -367,13 +380,6 @@ Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fid
     }
     return c;
   };
-  // prescale by 2^-c; under FLEXIBLEAUTO also by q_{l-1} / s_in, so that after
-  // the Rescale the message sits at delta whatever the input level's factor
-  // (the reference's adjustment factor, src/CKKS/Bootstrap.cu:441-470)
-  double pre = std::ldexp(1.0, -(int)P.cfg.correction);
-  if (P.flexible) pre = pre * (double)X.q[x.level - 1] / x.scale;
-  x = X.mul_real(x, pre);
-  x = X.drop(x, 1);
   if (P.cfg.encapsulation) x = X.switch_key(x, down);
   Reg r = X.mod_raise(x, P.l_raise, X.delta);
   if (P.cfg.encapsulation) r = X.switch_key(r, up);
@@ -394,6 +400,9 @@ Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fid
   for (size_t i = 0; i < P.stc.size(); ++i) w = linear(w, P.stc[i], P.stc_pts[i]);
   const i128 corr = (i128)1 << P.cfg.correction;
   return X.lin(w, &corr, nullptr, false, nullptr, P.out_scale > 0 ? P.out_scale : w.scale);
+}
+Reg boot_run(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fides_key* up) {
+  return boot_core(X, P, boot_prescale(X, P, x), down, up);
 }
 
 }  // namespace
@@ -594,31 +603,53 @@ Ciphertext CryptoContextImpl::EvalBootstrap(const Ciphertext& ct0, uint32_t numI
   return r;
 }
 
-Ciphertext CryptoContextImpl::BootstrapOnce(const Ciphertext& ct0) {
+Ciphertext CryptoContextImpl::BootstrapOnce(const Ciphertext& ct0) { return EvalBootstrapMany({ct0}).at(0); }
+
+// Each input is prescaled to one limb on its own (a batch of one: the
+// constant depends on its level and scaling factor); the rest of the
+// program then runs once for all inputs, every step one batched call.
+std::vector<Ciphertext> CryptoContextImpl::EvalBootstrapMany(const std::vector<Ciphertext>& cts0) {
   if (!boot) throw std::runtime_error("EvalBootstrap: call EvalBootstrapSetup first");
-  Ciphertext ct = Regular(ct0);
-  if (ct->NoiseLevel == 2) ct = Rescale(ct);  // as the reference's ModRaise (Bootstrap.cu:423)
   if (!evalMultKey) throw std::runtime_error("EvalBootstrap: relinearization key not loaded");
   const BootPlan& P = *boot;
   if (P.cfg.encapsulation && (!encapDown || !encapUp))
     throw std::runtime_error("EvalBootstrap: sparse-secret encapsulation keys not loaded (AddEncapsulationKeys)");
+  const uint32_t n = (uint32_t)cts0.size();
+  if (n == 0) return {};
+  std::vector<Ciphertext> cts;
+  for (const Ciphertext& c0 : cts0) {
+    Ciphertext ct = Regular(c0);
+    if (ct->NoiseLevel == 2) ct = Rescale(ct);  // as the reference's ModRaise (Bootstrap.cu:423)
+    if (ct->limbs < 2) throw std::runtime_error("EvalBootstrap: the input needs at least 2 limbs");
+    cts.push_back(ct);
+  }
   Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
-  // the input as a register (its nominal scale is the tracked scaling factor)
-  Reg x;
-  x.buf = std::make_shared<DevBuf>((size_t)2 * ct->limbs * N);
-  chk(fides_copy_device(x.buf->d, ct->DeviceData(), (size_t)2 * ct->limbs * N * 8, nullptr), "copy");
-  x.level = ct->limbs;
-  x.scale = ct->NoiseFactor > 0 ? ct->NoiseFactor : boot_delta();
-  if (x.level < 2) throw std::runtime_error("EvalBootstrap: the input needs at least 2 limbs");
-  const Reg out = boot_run(X, P, x, encapDown, encapUp);
-  Ciphertext o = NewCiphertext(out.level);
-  chk(fides_copy_device(o->DeviceData(), out.buf->d, (size_t)2 * out.level * N * 8, nullptr), "copy");
+  X.n = n;
+  Reg x1 = X.fresh(1, 0.0);  // one limb per input
+  for (uint32_t i = 0; i < n; ++i) {
+    const Ciphertext& ct = cts[i];
+    Exec X1{*this, ctx, q, N, boot_delta(), evalMultKey};
+    X1.pool = X.pool;
+    // the input as a register (its nominal scale is the tracked scaling factor)
+    Reg x = X1.fresh(ct->limbs, ct->NoiseFactor > 0 ? ct->NoiseFactor : boot_delta());
+    chk(fides_copy_device(x.d(0), ct->DeviceData(), (size_t)2 * ct->limbs * N * 8, nullptr), "copy");
+    const Reg p = boot_prescale(X1, P, x);
+    chk(fides_copy_device(x1.d(i), p.d(0), (size_t)2 * N * 8, nullptr), "copy");
+    x1.scale = p.scale;
+  }
+  const Reg out = boot_core(X, P, x1, encapDown, encapUp);
+  std::vector<Ciphertext> res;
+  for (uint32_t i = 0; i < n; ++i) {
+    Ciphertext o = NewCiphertext(out.level);
+    chk(fides_copy_device(o->DeviceData(), out.d(i), (size_t)2 * out.level * N * 8, nullptr), "copy");
+    o->NoiseLevel = 1;
+    o->NoiseFactor = out.scale;
+    o->slots = cts[i]->slots;
+    o->keyid = cts[i]->keyid;
+    res.push_back(o);
+  }
   chk(fides_stream_sync(nullptr), "sync");
-  o->NoiseLevel = 1;
-  o->NoiseFactor = out.scale;
-  o->slots = ct->slots;
-  o->keyid = ct->keyid;
-  return o;
+  return res;
 }
 
 }  // namespace fideslib

@@ -2,7 +2,7 @@
 // line program executor over device ciphertext registers and the Chebyshev
 // series evaluator built on it.  It mirrors the Python program builder
 // (fideslib_amd/serve.py Program, chebyshev_eval): every operation is one
-// call of the batched C ABI with a batch of one, levels are tracked in Q
+// call of the batched C ABI over the executor's batch, levels are tracked in Q
 // limbs and scales nominally (a rescaled scale is scale / q_top), and
 // integer constants are rounded half to even exactly as Python's round().
 #ifndef FIDESLIB_AMD_HOST_PROGRAM_HPP
@@ -42,27 +42,65 @@ inline uint64_t powmod(uint64_t b, uint64_t e, uint64_t m) {
 }
 
 
-// A device buffer owned by a register (own == false: a view of caller memory).
+// Device memory of the registers.  A Pool keeps released blocks by size and
+// hands them out again, so a program that runs many steps allocates each
+// register shape once instead of once per step; blocks go back to the
+// device when the pool is destroyed (after the last register using it).
+struct Pool {
+  std::multimap<size_t, uint64_t*> free;
+  ~Pool() {
+    for (auto& kv : free) fides_device_free(kv.second);
+  }
+  uint64_t* get(size_t words) {
+    auto it = free.find(words);
+    if (it != free.end()) {
+      uint64_t* d = it->second;
+      free.erase(it);
+      return d;
+    }
+    void* p = nullptr;
+    chk(fides_device_alloc(words * 8, &p), "device buffer");
+    return (uint64_t*)p;
+  }
+};
+
+// A device buffer owned by a register (own == false: a view of caller
+// memory), returned to its pool (if any) when released.
 struct DevBuf {
   uint64_t* d = nullptr;
   bool own = true;
+  size_t words = 0;
+  std::shared_ptr<Pool> pool;
   DevBuf(uint64_t* view, bool) : d(view), own(false) {}
-  explicit DevBuf(size_t words) {
-    void* p = nullptr;
-    chk(fides_device_alloc(words * 8, &p), "device buffer");
-    d = (uint64_t*)p;
+  explicit DevBuf(size_t w, std::shared_ptr<Pool> p = nullptr) : words(w), pool(std::move(p)) {
+    if (pool) {
+      d = pool->get(w);
+    } else {
+      void* q = nullptr;
+      chk(fides_device_alloc(w * 8, &q), "device buffer");
+      d = (uint64_t*)q;
+    }
   }
   ~DevBuf() {
-    if (d && own) fides_device_free(d);
+    if (!d || !own) return;
+    if (pool)
+      pool->free.emplace(words, d);
+    else
+      fides_device_free(d);
   }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// A register holds one ciphertext for each job of the executor's batch, all
+// at the same level and nominal scale, in one buffer: job i's [2][level][N]
+// starts `stride` words after job i-1's.
 struct Reg {
   std::shared_ptr<DevBuf> buf;
   uint32_t level = 0;  // limbs
   double scale = 0;
+  size_t stride = 0;   // words per job
+  uint64_t* d(uint32_t i) const { return buf->d + (size_t)i * stride; }
 };
 
 
@@ -73,7 +111,9 @@ struct DevPt {
   double scale;
 };
 
-// The program builder/executor over one ciphertext (mirrors serve.Program).
+// The program builder/executor (mirrors serve.Program) over a batch of n
+// ciphertexts running the same program in lockstep: every operation is one
+// call of the batched C ABI with n jobs (the same constants for every job).
 // With `dry` set no device work is issued: registers carry levels and
 // nominal scales only (the structure of a program, e.g. its output scale).
 struct Exec {
@@ -84,15 +124,30 @@ struct Exec {
   double delta;
   const fides_key* rlk;
   bool dry = false;
+  uint32_t n = 1;
+  std::shared_ptr<Pool> pool = std::make_shared<Pool>();
 
   Reg fresh(uint32_t level, double scale) {
     if (level < 1) throw std::runtime_error("out of levels");
     Reg r;
-    if (!dry) r.buf = std::make_shared<DevBuf>((size_t)2 * level * N);
+    r.stride = (size_t)2 * level * N;
+    if (!dry) r.buf = std::make_shared<DevBuf>(r.stride * n, pool);
     r.level = level;
     r.scale = scale;
     return r;
   }
+  // one pointer per job
+  std::vector<const uint64_t*> in_ptrs(const Reg& a) const {
+    std::vector<const uint64_t*> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = a.d(i);
+    return v;
+  }
+  std::vector<uint64_t*> out_ptrs(const Reg& a) const {
+    std::vector<uint64_t*> v(n);
+    for (uint32_t i = 0; i < n; ++i) v[i] = a.d(i);
+    return v;
+  }
+  std::vector<uint32_t> same(uint32_t x) const { return std::vector<uint32_t>(n, x); }
   std::vector<uint64_t> residues(i128 x, uint32_t l) const {
     std::vector<uint64_t> r(l);
     for (uint32_t j = 0; j < l; ++j) {
@@ -108,9 +163,10 @@ struct Exec {
     if (level == a.level) return a;
     Reg o = fresh(level, a.scale);
     if (dry) return o;
-    const uint64_t* in = a.buf->d;
-    uint64_t* out = o.buf->d;
-    chk(fides_drop_limbs_batch(ctx, 1, &in, &a.level, &level, &out, nullptr), "drop");
+    const auto in = in_ptrs(a);
+    const auto out = out_ptrs(o);
+    const auto li = same(a.level), lo = same(level);
+    chk(fides_drop_limbs_batch(ctx, n, in.data(), li.data(), lo.data(), out.data(), nullptr), "drop");
     return o;
   }
   Reg hmul(Reg a, Reg b) {  // with Rescale
@@ -120,10 +176,10 @@ struct Exec {
     if (l < 2) throw std::runtime_error("HMult+Rescale needs 2 limbs");
     Reg o = fresh(l - 1, a.scale * b.scale / (double)q[l - 1]);
     if (dry) return o;
-    const uint64_t* pa = a.buf->d;
-    const uint64_t* pb = b.buf->d;
-    uint64_t* po = o.buf->d;
-    chk(fides_hmult_batch(ctx, 1, &pa, &pb, &l, rlk, 1, &po, nullptr), "HMult");
+    const auto pa = in_ptrs(a), pb = in_ptrs(b);
+    const auto po = out_ptrs(o);
+    const auto ls = same(l);
+    chk(fides_hmult_batch(ctx, n, pa.data(), pb.data(), ls.data(), rlk, 1, po.data(), nullptr), "HMult");
     return o;
   }
   // automorphism k with `key` (the rotation key of k by default; k = 1 with
@@ -131,10 +187,12 @@ struct Exec {
   Reg automorph(const Reg& a, uint32_t k, const fides_key* key = nullptr) {
     Reg o = fresh(a.level, a.scale);
     if (dry) return o;
-    const uint64_t* in = a.buf->d;
-    uint64_t* out = o.buf->d;
     if (!key) key = cc.RotationKeyFor(k);
-    chk(fides_rotate_batch(ctx, 1, &in, &a.level, &key, &k, &out, nullptr), "rotate");
+    const auto in = in_ptrs(a);
+    const auto out = out_ptrs(o);
+    const auto ls = same(a.level), ks = same(k);
+    const std::vector<const fides_key*> keys(n, key);
+    chk(fides_rotate_batch(ctx, n, in.data(), ls.data(), keys.data(), ks.data(), out.data(), nullptr), "rotate");
     return o;
   }
   Reg rotate(const Reg& a, int64_t r) {
@@ -162,13 +220,14 @@ struct Exec {
     std::vector<uint64_t> mr, ar;
     if (mul) mr = residues(*mul, l);
     if (add) ar = residues(*add, l);
-    const uint64_t* pa = a.buf->d;
-    const uint64_t* pb = b0 ? b.buf->d : nullptr;
-    const uint64_t* pm = mul ? mr.data() : nullptr;
-    const uint64_t* pd = add ? ar.data() : nullptr;
-    const uint32_t s = sub ? 1 : 0;
-    uint64_t* po = o.buf->d;
-    chk(fides_lin_batch(ctx, 1, &pa, &pb, &l, &pm, &pd, &s, &po, nullptr), "linear");
+    const auto pa = in_ptrs(a);
+    std::vector<const uint64_t*> pb(n, nullptr);
+    if (b0) pb = in_ptrs(b);
+    const std::vector<const uint64_t*> pm(n, mul ? mr.data() : nullptr), pd(n, add ? ar.data() : nullptr);
+    const auto ss = same(sub ? 1 : 0), ls = same(l);
+    const auto po = out_ptrs(o);
+    chk(fides_lin_batch(ctx, n, pa.data(), pb.data(), ls.data(), pm.data(), pd.data(), ss.data(), po.data(), nullptr),
+        "linear");
     return o;
   }
   Reg add(const Reg& a, const Reg& b) { return lin(a, nullptr, &b, false, nullptr, a.scale); }
@@ -177,9 +236,10 @@ struct Exec {
     if (a.level < 2) throw std::runtime_error("Rescale needs 2 limbs");
     Reg o = fresh(a.level - 1, a.scale / (double)q[a.level - 1]);
     if (dry) return o;
-    const uint64_t* in = a.buf->d;
-    uint64_t* out = o.buf->d;
-    chk(fides_rescale_batch(ctx, 1, &in, &a.level, &out, nullptr), "Rescale");
+    const auto in = in_ptrs(a);
+    const auto out = out_ptrs(o);
+    const auto ls = same(a.level);
+    chk(fides_rescale_batch(ctx, n, in.data(), ls.data(), out.data(), nullptr), "Rescale");
     return o;
   }
   Reg mul_real(const Reg& a, double x) {
@@ -198,22 +258,25 @@ struct Exec {
     for (const auto& t : terms) l = std::min(l, t.first.level);
     Reg o = fresh(l, scale);
     if (dry) return o;
-    const uint32_t cnt = (uint32_t)terms.size();
-    std::vector<const uint64_t*> xs;
-    std::vector<uint32_t> xl;
+    const auto cnt = same((uint32_t)terms.size());
     std::vector<std::vector<uint64_t>> mres;
-    std::vector<const uint64_t*> mp;
-    for (const auto& t : terms) {
-      xs.push_back(t.first.buf->d);
-      xl.push_back(t.first.level);
-      mres.push_back(residues(t.second, l));
-    }
-    for (auto& v : mres) mp.push_back(v.data());
+    for (const auto& t : terms) mres.push_back(residues(t.second, l));
+    std::vector<const uint64_t*> xs, mp;
+    std::vector<uint32_t> xl;
+    for (uint32_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < terms.size(); ++j) {
+        xs.push_back(terms[j].first.d(i));
+        xl.push_back(terms[j].first.level);
+        mp.push_back(mres[j].data());
+      }
     std::vector<uint64_t> ar;
     if (add) ar = residues(*add, l);
-    const uint64_t* pa = add ? ar.data() : nullptr;
-    uint64_t* po = o.buf->d;
-    chk(fides_lincomb_batch(ctx, 1, &cnt, xs.data(), xl.data(), mp.data(), &pa, &l, &po, nullptr), "lincomb");
+    const std::vector<const uint64_t*> pa(n, add ? ar.data() : nullptr);
+    const auto ls = same(l);
+    const auto po = out_ptrs(o);
+    chk(fides_lincomb_batch(ctx, n, cnt.data(), xs.data(), xl.data(), mp.data(), pa.data(), ls.data(), po.data(),
+                            nullptr),
+        "lincomb");
     return o;
   }
   Reg ptdot(const std::vector<std::pair<Reg, const DevPt*>>& terms) {
@@ -226,23 +289,27 @@ struct Exec {
     }
     Reg o = fresh(l, regs[0].scale * terms[0].second->scale);
     if (dry) return o;
-    const uint32_t cnt = (uint32_t)terms.size();
+    const auto cnt = same((uint32_t)terms.size());
     std::vector<const uint64_t*> cs, ps;
-    for (size_t i = 0; i < terms.size(); ++i) {
-      cs.push_back(regs[i].buf->d);
-      ps.push_back(terms[i].second->buf->d);
-    }
-    uint64_t* po = o.buf->d;
-    chk(fides_ptdot_batch(ctx, 1, &cnt, cs.data(), ps.data(), &l, &po, nullptr), "plaintext product");
+    for (uint32_t i = 0; i < n; ++i)
+      for (size_t j = 0; j < terms.size(); ++j) {
+        cs.push_back(regs[j].d(i));
+        ps.push_back(terms[j].second->buf->d);
+      }
+    const auto ls = same(l);
+    const auto po = out_ptrs(o);
+    chk(fides_ptdot_batch(ctx, n, cnt.data(), cs.data(), ps.data(), ls.data(), po.data(), nullptr),
+        "plaintext product");
     return o;
   }
   Reg mod_raise(Reg a, uint32_t level, double scale) {
     a = drop(a, 1);
     Reg o = fresh(level, scale);
     if (dry) return o;
-    const uint64_t* in = a.buf->d;
-    uint64_t* out = o.buf->d;
-    chk(fides_mod_raise_batch(ctx, 1, &in, &level, &out, nullptr), "ModRaise");
+    const auto in = in_ptrs(a);
+    const auto out = out_ptrs(o);
+    const auto ls = same(level);
+    chk(fides_mod_raise_batch(ctx, n, in.data(), ls.data(), out.data(), nullptr), "ModRaise");
     return o;
   }
 };

@@ -391,6 +391,12 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   void EvalBootstrapInPlace(Ciphertext& ct, uint32_t numIterations = 1, uint32_t precision = 0) {
     ct = EvalBootstrap(ct, numIterations, precision);
   }
+  // One bootstrap of each input (one iteration), the same results as
+  // EvalBootstrap on each; after each input's prescale to one limb, every
+  // step of the plan runs once for all of them (one batched call).  No
+  // reference counterpart: the reference bootstraps one ciphertext per call
+  // (api/CryptoContext.hpp:185).
+  std::vector<Ciphertext> EvalBootstrapMany(const std::vector<Ciphertext>& cts);
   // The key-switching key loaded for an automorphism index (throws if absent).
   const fides_key* RotationKeyFor(uint32_t autoIndex) const { return RotationKey(autoIndex); }
 

@@ -25,9 +25,13 @@
 //   bootsetup_pre <file>                EvalBootstrapSetup from a precomputation file
 //                                       (both write "rots ..." to meta.txt)
 //   boot <out> <a>                      EvalBootstrap
+//   bootmany <prefix> <a> <b> ...       EvalBootstrapMany -> prefix0, prefix1, ...
+//   boottime <reps> <n> <a>             ms for n EvalBootstrap calls, then for one
+//                                       EvalBootstrapMany of n (line "boottime n one many")
 //   store <name>                        -> name.bin [2][limbs][N], line in meta.txt:
 //                                          name limbs deg noiseFactor slots extended
 //   expect_throw <command...>           the command must throw (line "throws" in meta.txt)
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -274,6 +278,28 @@ struct State {
       meta << "\n";
     } else if (op == "boot") {  // boot out in [iterations precision]
       ct[w.at(1)] = cc->EvalBootstrap(C(w.at(2)), w.size() > 3 ? (uint32_t)I(3) : 1, w.size() > 4 ? (uint32_t)I(4) : 0);
+    } else if (op == "bootmany") {  // bootmany outprefix a b ...  -> outprefix0, outprefix1, ...
+      std::vector<Ciphertext> in;
+      for (size_t i = 2; i < w.size(); ++i) in.push_back(C(w[i]));
+      const auto out = cc->EvalBootstrapMany(in);
+      for (size_t i = 0; i < out.size(); ++i) ct[w.at(1) + std::to_string(i)] = out[i];
+    } else if (op == "boottime") {  // boottime reps n a: n bootstraps one by one, then in one EvalBootstrapMany
+      const int reps = (int)I(1), n = (int)I(2);
+      const Ciphertext a = C(w.at(3));
+      const std::vector<Ciphertext> in(n, a);
+      auto ms = [&](const std::function<void()>& f) {
+        f();  // warm-up
+        cc->Synchronize();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int r = 0; r < reps; ++r) f();
+        cc->Synchronize();
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+      };
+      const double one = ms([&] {
+        for (int i = 0; i < n; ++i) cc->EvalBootstrap(a);
+      });
+      const double many = ms([&] { cc->EvalBootstrapMany(in); });
+      meta << "boottime " << n << " " << one << " " << many << "\n";
     } else if (op == "setlevel") {
       Ciphertext a = C(w.at(2));
       CryptoContextImpl::SetLevel(a, (size_t)I(3));

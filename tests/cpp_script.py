@@ -80,6 +80,8 @@ class Script:
                 out["rots"] = [int(x) for x in w[1:]]
             elif w[0] == "throws":
                 out["throws"].append(w[1])
+            elif w[0] == "boottime":
+                out.setdefault("boottime", []).append((int(w[1]), float(w[2]), float(w[3])))
             else:
                 limbs = int(w[1])
                 data = np.fromfile(os.path.join(self.d, w[0] + ".bin"), np.uint64).reshape(2, limbs, N)

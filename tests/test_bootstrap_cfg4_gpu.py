@@ -203,6 +203,53 @@ def test_cpp_bootstrap_own_plan_flexible_encapsulated(tmp_path):
 
 
 @pytest.mark.parametrize("variant", ["fixed", "flexible_encapsulated"])
+def test_cpp_bootstrap_many_equals_single(tmp_path, variant):
+    """EvalBootstrapMany over inputs at different levels (and, under
+    FLEXIBLEAUTO, different scaling factors) gives each input's
+    EvalBootstrap result bit for bit; one of them equals the oracle running
+    the plan.  The same script times n single bootstraps against one batch."""
+    prm = params.boot_small()
+    N = prm.N
+    octx = O.Context(prm.logN, prm.q, prm.p, prm.dnum)
+    flex = variant != "fixed"
+    keys = C.Keys(octx, 13, h=None if flex else 64)
+    cfg = B.BootConfig.encapsulated(flexible=True) if flex else B.BootConfig()
+    bp = B.BootstrapPlan(prm, None, cfg, keep_host=True)
+    rot = host_keys(bp, keys, N)
+    sf = params.flexible_scaling_factors(prm.q) if flex else [2.0 ** prm.scale_bits] * prm.L
+    rng = np.random.default_rng(47)
+    lvls = [6, 3, 5]
+    zs = [rng.uniform(-1, 1, N // 2) + 1j * rng.uniform(-1, 1, N // 2) for _ in lvls]
+    cts = [keys.encrypt(C.encode(z, sf[l - 1], N), l) for z, l in zip(zs, lvls)]
+    rlk = keys.relin_key()
+    s = Script(tmp_path, prm, F.FLEXIBLEAUTO if flex else F.FIXEDMANUAL)
+    s.rlk(rlk)
+    for k, v in rot.items():
+        if isinstance(k, int):
+            s.autokey(k, v)
+    if flex:
+        s.encapkeys(rot[bp.ENCAPS_DOWN], rot[bp.ENCAPS_UP])
+    s.bootsetup_pre(bp.export_precomputation(), cfg)
+    for i, (ct, l) in enumerate(zip(cts, lvls)):
+        s.ct("x%d" % i, ct, 1, sf[l - 1], N // 2)
+        s.op("boot", "y%d" % i, "x%d" % i)
+    s.op("bootmany", "m", *["x%d" % i for i in range(len(lvls))])
+    s.op("boottime", 2, 8, "x0")
+    s.store(*["y%d" % i for i in range(len(lvls))], *["m%d" % i for i in range(len(lvls))])
+    res = s.run()
+    for i in range(len(lvls)):
+        y, m = res["y%d" % i], res["m%d" % i]
+        assert m["limbs"] == y["limbs"] and m["sf"] == y["sf"] and m["deg"] == y["deg"]
+        assert np.array_equal(m["data"], y["data"]), i
+    P = bp.program(lvls[0], sf[lvls[0] - 1])
+    want = run_oracle(octx, P, [cts[0]], rlk, rot)
+    assert np.array_equal(res["m0"]["data"], want)
+    n, one, many = res["boottime"][0]
+    say("C++ bootstrap (%s), N=2^%d: %d one by one %.1f ms, EvalBootstrapMany %.1f ms (%.2fx)"
+        % (variant, prm.logN, n, one, many, one / many))
+
+
+@pytest.mark.parametrize("variant", ["fixed", "flexible_encapsulated"])
 def test_cpp_bootstrap_two_iterations(tmp_path, variant):
     """EvalBootstrap(ct, 2, p): the error 2^p (x - BTS(x)) is bootstrapped
     and added back; the result carries the output factor times 2^p."""
