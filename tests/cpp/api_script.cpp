@@ -26,6 +26,10 @@
 //                                       (both write "rots ..." to meta.txt)
 //   boot <out> <a>                      EvalBootstrap
 //   bootmany <prefix> <a> <b> ...       EvalBootstrapMany -> prefix0, prefix1, ...
+//   synthkeys <seed>                    random keys (one shared value) for relinearization,
+//                                       every rotation of the last bootstrap setup,
+//                                       conjugation and encapsulation (timing only)
+//   synthct <name> <limbs> <seed> <noiseFactor> <slots>   random residues (timing only)
 //   boottime <reps> <n> <a>             ms for n EvalBootstrap calls, then for one
 //                                       EvalBootstrapMany of n (line "boottime n one many")
 //   store <name>                        -> name.bin [2][limbs][N], line in meta.txt:
@@ -74,6 +78,31 @@ struct State {
   std::map<std::string, Plaintext> pt;
   std::ofstream meta;
 
+  std::vector<int32_t> last_rots;
+  std::vector<uint64_t> moduli;  // Q then P
+
+  static uint64_t rnd(uint64_t& st) {  // xorshift64*
+    st ^= st >> 12;
+    st ^= st << 25;
+    st ^= st >> 27;
+    return st * 0x2545f4914f6cdd1dull;
+  }
+  RawKeySwitchKey synth_key(uint64_t seed) {
+    uint64_t st = seed * 0x9e3779b97f4a7c15ull + 7;
+    RawKeySwitchKey k;
+    k.r_key.resize(2);
+    for (uint32_t c = 0; c < 2; ++c) {
+      k.r_key[c].resize(dnum);
+      for (uint32_t d = 0; d < dnum; ++d) {
+        k.r_key[c][d].resize(L + K);
+        for (uint32_t t = 0; t < L + K; ++t) {
+          k.r_key[c][d][t].resize(N);
+          for (uint32_t j = 0; j < N; ++j) k.r_key[c][d][t][j] = rnd(st) % moduli[t];
+        }
+      }
+    }
+    return k;
+  }
   RawKeySwitchKey key(const std::string& file) {
     const auto flat = read_u64(file);
     RawKeySwitchKey k;
@@ -113,6 +142,7 @@ struct State {
       N = 1u << p[0];
       rp.moduli.assign(p.begin() + 6, p.begin() + 6 + L);
       rp.SPECIALmoduli.assign(p.begin() + 6 + L, p.begin() + 6 + L + K);
+      moduli.assign(p.begin() + 6, p.begin() + 6 + L + K);
       rp.dnum = (int)dnum;
       rp.scalingTechnique = (ScalingTechnique)I(2);
       cc = CryptoContextImpl::Create(rp);
@@ -217,6 +247,36 @@ struct State {
       std::ofstream o(g_dir + "/" + w.at(1) + ".txt");
       o << std::setprecision(17);
       for (double v : c) o << v << "\n";
+    } else if (op == "synthkeys") {  // synthkeys seed: random keys for rlk, every rotation of the last setup, conjugation, encapsulation
+      const RawKeySwitchKey k = synth_key((uint64_t)I(1));
+      cc->AddEvalMultKey(k);
+      for (int32_t r : last_rots) {
+        const uint64_t half = N / 2;
+        const uint64_t e = (uint64_t)((((int64_t)r % (int64_t)half) + (int64_t)half) % (int64_t)half);
+        uint64_t a = 1, b = 5;
+        for (uint64_t x = e; x; x >>= 1, b = b * b % (2ull * N))
+          if (x & 1) a = a * b % (2ull * N);
+        if (a != 1) cc->AddAutomorphismKey((uint32_t)a, k);
+      }
+      cc->AddAutomorphismKey(2 * N - 1, k);
+      cc->AddEncapsulationKeys(k, k);
+    } else if (op == "synthct") {  // synthct name limbs seed noiseFactor slots: random residues
+      const uint32_t l = (uint32_t)I(2);
+      uint64_t st = (uint64_t)I(3) * 0x9e3779b97f4a7c15ull + 1;
+      RawCipherText r;
+      r.sub_0.resize(l);
+      r.sub_1.resize(l);
+      for (auto* sub : {&r.sub_0, &r.sub_1})
+        for (uint32_t i = 0; i < l; ++i) {
+          (*sub)[i].resize(N);
+          for (uint32_t j = 0; j < N; ++j) (*sub)[i][j] = rnd(st) % moduli[i];
+        }
+      r.numRes = (int)l;
+      r.N = (int)N;
+      r.NoiseLevel = 1;
+      r.Noise = D(4);
+      r.slots = (int)I(5);
+      ct[w.at(1)] = cc->LoadCiphertext(r);
     } else if (op == "encapkeys") {
       cc->AddEncapsulationKeys(key(w.at(1)), key(w.at(2)));
     } else if (op == "bootsetup") {
@@ -230,6 +290,7 @@ struct State {
       c.encapsulation = I(7) != 0;
       if (w.size() > 8) c.slots = (uint32_t)I(8);
       const auto rots = cc->EvalBootstrapSetup(c);
+      last_rots = rots;
       meta << "rots";
       for (int32_t r : rots) meta << " " << r;
       meta << "\n";
@@ -273,6 +334,7 @@ struct State {
         (gi < ncts ? pre.coeffToSlot : pre.slotToCoeff).push_back(std::move(g));
       }
       const auto rots = cc->EvalBootstrapSetup(pre);
+      last_rots = rots;
       meta << "rots";
       for (int32_t r : rots) meta << " " << r;
       meta << "\n";

@@ -249,6 +249,28 @@ def test_cpp_bootstrap_many_equals_single(tmp_path, variant):
         % (variant, prm.logN, n, one, many, one / many))
 
 
+def test_cpp_bootstrap_many_rate_cfg4(tmp_path):
+    """The C++ API at the cfg4 shape (N = 2^16, 30 limbs, dnum 3, FLEXIBLEAUTO,
+    encapsulation, the library's own plan): 8 bootstraps of a 3-limb input
+    one by one and in one EvalBootstrapMany, with synthetic keys and
+    residues (the rate does not depend on their values).  Both must finish;
+    the rates are reported."""
+    prm = params.config3()
+    N = prm.N
+    cfg = B.BootConfig.encapsulated(flexible=True)
+    sf = params.flexible_scaling_factors(prm.q)
+    s = Script(tmp_path, prm, F.FLEXIBLEAUTO)
+    s.op("bootsetup", 3, 3, cfg.degree, cfg.double_angle, cfg.K, cfg.correction, 1)
+    s.op("synthkeys", 1)
+    s.op("synthct", "x", 3, 5, sf[2], N // 2)
+    s.op("boottime", 1, 8, "x")
+    res = s.run(timeout=900)
+    n, one, many = res["boottime"][0]
+    say("C++ bootstrap cfg4 (N=2^16, FLEXIBLEAUTO + encapsulation): %d one by one %.1f ms (%.1f/s), "
+        "EvalBootstrapMany %.1f ms (%.1f/s)" % (n, one, 1e3 * n / one, many, 1e3 * n / many))
+    assert many > 0 and one > 0
+
+
 @pytest.mark.parametrize("variant", ["fixed", "flexible_encapsulated"])
 def test_cpp_bootstrap_two_iterations(tmp_path, variant):
     """EvalBootstrap(ct, 2, p): the error 2^p (x - BTS(x)) is bootstrapped
