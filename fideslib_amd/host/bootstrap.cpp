@@ -357,15 +357,26 @@ Reg boot_core(Exec& X, const BootPlan& P, Reg x, const fides_key* down, const fi
     // it), the indices plan_rotations lists
     const int64_t n = (int64_t)G.n;
     auto amount = [n](int64_t a) { return ((a % n) + n) % n; };
+    // every baby rotation in one batched call (one shared ModUp), every
+    // giant's plaintext products in one, every giant rotation in one
+    std::vector<std::pair<Reg, int64_t>> brot;
+    for (int64_t b : G.babies) brot.emplace_back(v, b == 0 ? 0 : amount(b * (int64_t)G.stride));
+    const std::vector<Reg> bregs = X.rotate_many(brot);
     std::map<int64_t, Reg> babies;
-    for (int64_t b : G.babies) babies[b] = b == 0 ? v : X.rotate(v, amount(b * (int64_t)G.stride));
-    std::vector<Reg> parts;
+    for (size_t i = 0; i < G.babies.size(); ++i) babies[G.babies[i]] = bregs[i];
+    std::vector<std::vector<std::pair<Reg, const BootPlan::Pt*>>> groups;
+    std::vector<std::pair<Reg, int64_t>> grot;
     for (const auto& kv : pts) {
       std::vector<std::pair<Reg, const BootPlan::Pt*>> terms;
       for (const auto& bp : kv.second) terms.emplace_back(babies.at(bp.first), &bp.second);
-      Reg inner = X.ptdot(terms);
-      parts.push_back(kv.first == 0 ? inner : X.rotate(inner, amount(kv.first * (int64_t)G.g * (int64_t)G.stride)));
+      groups.push_back(std::move(terms));
     }
+    const std::vector<Reg> inners = X.ptdot_many(groups);
+    size_t gi = 0;
+    for (const auto& kv : pts) {
+      grot.emplace_back(inners[gi++], kv.first == 0 ? 0 : amount(kv.first * (int64_t)G.g * (int64_t)G.stride));
+    }
+    const std::vector<Reg> parts = X.rotate_many(grot);
     if (parts.size() == 1) return X.rescale(parts[0]);
     std::vector<std::pair<Reg, i128>> t;
     for (const Reg& p : parts) t.emplace_back(p, 1);

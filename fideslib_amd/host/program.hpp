@@ -201,6 +201,40 @@ struct Exec {
     if (rr == 0) return a;
     return automorph(a, (uint32_t)powmod(5, (uint64_t)rr, 2ull * N));
   }
+  // Several rotations in one batched call (rotations of the same register
+  // share one ModUp there); amount 0 returns the register itself.
+  std::vector<Reg> rotate_many(const std::vector<std::pair<Reg, int64_t>>& items) {
+    std::vector<Reg> outs(items.size());
+    std::vector<const uint64_t*> in;
+    std::vector<uint64_t*> out;
+    std::vector<uint32_t> ls, ks;
+    std::vector<const fides_key*> keys;
+    const int64_t half = N / 2;
+    for (size_t j = 0; j < items.size(); ++j) {
+      const Reg& a = items[j].first;
+      const int64_t rr = ((items[j].second % half) + half) % half;
+      if (rr == 0) {
+        outs[j] = a;
+        continue;
+      }
+      outs[j] = fresh(a.level, a.scale);
+      if (dry) continue;
+      const uint32_t k = (uint32_t)powmod(5, (uint64_t)rr, 2ull * N);
+      const fides_key* key = cc.RotationKeyFor(k);
+      for (uint32_t i = 0; i < n; ++i) {
+        in.push_back(a.d(i));
+        out.push_back(outs[j].d(i));
+        ls.push_back(a.level);
+        ks.push_back(k);
+        keys.push_back(key);
+      }
+    }
+    if (!in.empty())
+      chk(fides_rotate_batch(ctx, (uint32_t)in.size(), in.data(), ls.data(), keys.data(), ks.data(), out.data(),
+                             nullptr),
+          "rotate");
+    return outs;
+  }
   Reg conjugate(const Reg& a) { return automorph(a, 2 * N - 1); }
   Reg switch_key(const Reg& a, const fides_key* key) {
     if (!key && !dry) throw std::runtime_error("switching key not loaded");
@@ -301,6 +335,39 @@ struct Exec {
     chk(fides_ptdot_batch(ctx, n, cnt.data(), cs.data(), ps.data(), ls.data(), po.data(), nullptr),
         "plaintext product");
     return o;
+  }
+  // Several plaintext dot products in one batched call.
+  std::vector<Reg> ptdot_many(const std::vector<std::vector<std::pair<Reg, const DevPt*>>>& groups) {
+    std::vector<Reg> outs;
+    std::vector<uint32_t> cnt, ls;
+    std::vector<const uint64_t*> cs, ps;
+    std::vector<uint64_t*> po;
+    std::vector<std::vector<Reg>> keep;
+    for (const auto& terms : groups) {
+      uint32_t l = ~0u;
+      for (const auto& t : terms) l = std::min(l, t.first.level);
+      std::vector<Reg> regs;
+      for (const auto& t : terms) {
+        regs.push_back(drop(t.first, l));
+        if (t.second->level < l) throw std::runtime_error("plaintext below the ciphertext level");
+      }
+      outs.push_back(fresh(l, regs[0].scale * terms[0].second->scale));
+      if (dry) continue;
+      for (uint32_t i = 0; i < n; ++i) {
+        cnt.push_back((uint32_t)terms.size());
+        ls.push_back(l);
+        po.push_back(outs.back().d(i));
+        for (size_t j = 0; j < terms.size(); ++j) {
+          cs.push_back(regs[j].d(i));
+          ps.push_back(terms[j].second->buf->d);
+        }
+      }
+      keep.push_back(std::move(regs));
+    }
+    if (!dry && !cnt.empty())
+      chk(fides_ptdot_batch(ctx, (uint32_t)cnt.size(), cnt.data(), cs.data(), ps.data(), ls.data(), po.data(), nullptr),
+          "plaintext product");
+    return outs;
   }
   Reg mod_raise(Reg a, uint32_t level, double scale) {
     a = drop(a, 1);
