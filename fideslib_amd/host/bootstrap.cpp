@@ -637,15 +637,24 @@ std::vector<Ciphertext> CryptoContextImpl::EvalBootstrapMany(const std::vector<C
   Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
   X.n = n;
   Reg x1 = X.fresh(1, 0.0);  // one limb per input
+  // inputs of one level and scaling factor share their prescale (a batch)
+  std::map<std::pair<uint32_t, double>, std::vector<uint32_t>> groups;
   for (uint32_t i = 0; i < n; ++i) {
-    const Ciphertext& ct = cts[i];
-    Exec X1{*this, ctx, q, N, boot_delta(), evalMultKey};
-    X1.pool = X.pool;
-    // the input as a register (its nominal scale is the tracked scaling factor)
-    Reg x = X1.fresh(ct->limbs, ct->NoiseFactor > 0 ? ct->NoiseFactor : boot_delta());
-    chk(fides_copy_device(x.d(0), ct->DeviceData(), (size_t)2 * ct->limbs * N * 8, nullptr), "copy");
-    const Reg p = boot_prescale(X1, P, x);
-    chk(fides_copy_device(x1.d(i), p.d(0), (size_t)2 * N * 8, nullptr), "copy");
+    const double sc = cts[i]->NoiseFactor > 0 ? cts[i]->NoiseFactor : boot_delta();
+    groups[{cts[i]->limbs, sc}].push_back(i);
+  }
+  for (const auto& g : groups) {
+    const std::vector<uint32_t>& idx = g.second;
+    Exec Xg{*this, ctx, q, N, boot_delta(), evalMultKey};
+    Xg.n = (uint32_t)idx.size();
+    Xg.pool = X.pool;
+    // the inputs as a register (its nominal scale is the tracked scaling factor)
+    Reg x = Xg.fresh(g.first.first, g.first.second);
+    for (uint32_t j = 0; j < Xg.n; ++j)
+      chk(fides_copy_device(x.d(j), cts[idx[j]]->DeviceData(), (size_t)2 * g.first.first * N * 8, nullptr), "copy");
+    const Reg p = boot_prescale(Xg, P, x);
+    for (uint32_t j = 0; j < Xg.n; ++j)
+      chk(fides_copy_device(x1.d(idx[j]), p.d(j), (size_t)2 * N * 8, nullptr), "copy");
     x1.scale = p.scale;
   }
   const Reg out = boot_core(X, P, x1, encapDown, encapUp);
