@@ -636,6 +636,10 @@ std::vector<Ciphertext> CryptoContextImpl::EvalBootstrapMany(const std::vector<C
   }
   Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
   X.n = n;
+  // the register pool outlives the call, so repeated bootstraps of one batch
+  // size allocate no device memory after the first
+  if (!bootMem) bootMem = std::make_shared<Pool>();
+  X.pool = std::static_pointer_cast<Pool>(bootMem);
   Reg x1 = X.fresh(1, 0.0);  // one limb per input
   // inputs of one level and scaling factor share their prescale (a batch)
   std::map<std::pair<uint32_t, double>, std::vector<uint32_t>> groups;

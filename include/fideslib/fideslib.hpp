@@ -395,7 +395,9 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   // EvalBootstrap on each; after each input's prescale to one limb, every
   // step of the plan runs once for all of them (one batched call).  No
   // reference counterpart: the reference bootstraps one ciphertext per call
-  // (api/CryptoContext.hpp:185).
+  // (api/CryptoContext.hpp:185).  The device memory of the plan's registers
+  // is kept by the context between calls (no allocation once a batch size
+  // has run) and released with it.
   std::vector<Ciphertext> EvalBootstrapMany(const std::vector<Ciphertext>& cts);
   // The key-switching key loaded for an automorphism index (throws if absent).
   const fides_key* RotationKeyFor(uint32_t autoIndex) const { return RotationKey(autoIndex); }
@@ -446,6 +448,7 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   fides_key* encapUp = nullptr;    // s' -> s
   std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
   std::shared_ptr<BootPlan> boot;                // EvalBootstrapSetup
+  std::shared_ptr<void> bootMem;                 // device blocks kept between bootstraps (a register pool)
   // the nominal scale of a fresh ciphertext: 2^(bit length of q_1, rounded)
   double boot_delta() const { return std::ldexp(1.0, (int)std::lround(std::log2((double)q[q.size() > 1 ? 1 : 0]))); }
 };
