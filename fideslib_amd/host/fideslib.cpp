@@ -25,26 +25,49 @@ static void chk(int rc, const char* what) {
   if (rc != FIDES_OK) throw std::runtime_error(std::string(what) + ": " + fides_last_error());
 }
 
+// Device memory of ciphertexts and plaintexts comes from the context's cache
+// of idle blocks (one allocation per size; a released block waits for the
+// next operation of its size instead of being freed, which would
+// synchronize the device).  Every operation of the C++ API runs on the
+// default stream, so a block released on the host is reused only by work
+// queued after everything that used it.
 CiphertextImpl::CiphertextImpl(std::shared_ptr<CryptoContextImpl> c, uint32_t l, bool e)
     : limbs(l), cc(std::move(c)), ext(e) {
-  void* p = nullptr;
   const size_t nl = e ? l + cc->GetNumSpecialPrimes() : l;
-  chk(fides_device_alloc((size_t)2 * nl * cc->GetRingDimension() * 8, &p), "ciphertext allocation");
-  d = (uint64_t*)p;
+  words = (size_t)2 * nl * cc->GetRingDimension();
+  d = cc->TakeBlock(words);
 }
 
 CiphertextImpl::~CiphertextImpl() {
-  if (d) fides_device_free(d);
+  if (d) cc->GiveBlock(d, words);
 }
 
 PlaintextImpl::PlaintextImpl(std::shared_ptr<CryptoContextImpl> c, uint32_t l) : limbs(l), cc(std::move(c)) {
-  void* p = nullptr;
-  chk(fides_device_alloc((size_t)l * cc->GetRingDimension() * 8, &p), "plaintext allocation");
-  d = (uint64_t*)p;
+  words = (size_t)l * cc->GetRingDimension();
+  d = cc->TakeBlock(words);
 }
 
 PlaintextImpl::~PlaintextImpl() {
-  if (d) fides_device_free(d);
+  if (d) cc->GiveBlock(d, words);
+}
+
+uint64_t* CryptoContextImpl::TakeBlock(size_t w) {
+  if (!blockMem) blockMem = std::make_shared<detail::Pool>();
+  return std::static_pointer_cast<detail::Pool>(blockMem)->get(w);
+}
+
+void CryptoContextImpl::GiveBlock(uint64_t* d, size_t w) {
+  std::static_pointer_cast<detail::Pool>(blockMem)->free.emplace(w, d);
+}
+
+void CryptoContextImpl::ReleaseCachedMemory() {
+  Synchronize();
+  for (std::shared_ptr<void>* m : {&blockMem, &bootMem}) {
+    if (!*m) continue;
+    auto P = std::static_pointer_cast<detail::Pool>(*m);
+    for (auto& kv : P->free) fides_device_free(kv.second);
+    P->free.clear();
+  }
 }
 
 CryptoContext CryptoContextImpl::Create(const RawParams& P) {

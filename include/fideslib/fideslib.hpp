@@ -167,6 +167,7 @@ class CiphertextImpl {
   std::shared_ptr<CryptoContextImpl> cc;
   bool ext = false;
   uint64_t* d = nullptr;
+  size_t words = 0;  // size of d (returned to the context's memory cache)
 };
 using Ciphertext = std::shared_ptr<CiphertextImpl>;
 
@@ -194,6 +195,7 @@ class PlaintextImpl {
   explicit PlaintextImpl(std::shared_ptr<CryptoContextImpl> cc, uint32_t limbs);
   std::shared_ptr<CryptoContextImpl> cc;
   uint64_t* d = nullptr;
+  size_t words = 0;  // size of d (returned to the context's memory cache)
 };
 using Plaintext = std::shared_ptr<PlaintextImpl>;
 
@@ -350,6 +352,11 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   std::vector<Ciphertext> EvalRotateMany(const std::vector<Ciphertext>& a, const std::vector<int32_t>& indices);
 
   void Synchronize() const;
+  // Device memory of released ciphertexts, plaintexts and bootstrap
+  // registers is kept by the context for reuse (no allocation, and no
+  // device synchronization of a free, on the next operation of the same
+  // size).  This returns the idle blocks to the device.
+  void ReleaseCachedMemory();
 
   // Launch shapes (the C ABI's fides_tuning, per size class of a key-switch
   // batch); no setting changes a result bit.  No reference counterpart: the
@@ -449,6 +456,13 @@ class CryptoContextImpl : public std::enable_shared_from_this<CryptoContextImpl>
   std::map<uint32_t, fides_key*> rotationKeys;  // by automorphism index
   std::shared_ptr<BootPlan> boot;                // EvalBootstrapSetup
   std::shared_ptr<void> bootMem;                 // device blocks kept between bootstraps (a register pool)
+ public:
+  // (internal) the context's cache of device blocks; see ReleaseCachedMemory
+  uint64_t* TakeBlock(size_t words);
+  void GiveBlock(uint64_t* d, size_t words);
+
+ private:
+  std::shared_ptr<void> blockMem;                // idle ciphertext / plaintext blocks
   // the nominal scale of a fresh ciphertext: 2^(bit length of q_1, rounded)
   double boot_delta() const { return std::ldexp(1.0, (int)std::lround(std::log2((double)q[q.size() > 1 ? 1 : 0]))); }
 };

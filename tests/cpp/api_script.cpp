@@ -26,6 +26,7 @@
 //                                       (both write "rots ..." to meta.txt)
 //   boot <out> <a>                      EvalBootstrap
 //   bootmany <prefix> <a> <b> ...       EvalBootstrapMany -> prefix0, prefix1, ...
+//   releasemem                          ReleaseCachedMemory
 //   synthkeys <seed>                    random keys (one shared value) for relinearization,
 //                                       every rotation of the last bootstrap setup,
 //                                       conjugation and encapsulation (timing only)
@@ -277,6 +278,8 @@ struct State {
       r.Noise = D(4);
       r.slots = (int)I(5);
       ct[w.at(1)] = cc->LoadCiphertext(r);
+    } else if (op == "releasemem") {
+      cc->ReleaseCachedMemory();
     } else if (op == "encapkeys") {
       cc->AddEncapsulationKeys(key(w.at(1)), key(w.at(2)));
     } else if (op == "bootsetup") {

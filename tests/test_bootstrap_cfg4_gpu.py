@@ -233,6 +233,7 @@ def test_cpp_bootstrap_many_equals_single(tmp_path, variant):
     for i, (ct, l) in enumerate(zip(cts, lvls)):
         s.ct("x%d" % i, ct, 1, sf[l - 1], N // 2)
         s.op("boot", "y%d" % i, "x%d" % i)
+    s.op("releasemem")  # idle cached blocks back to the device; the next calls allocate again
     s.op("bootmany", "m", *["x%d" % i for i in range(len(lvls))])
     s.op("boottime", 2, 8, "x0")
     s.store(*["y%d" % i for i in range(len(lvls))], *["m%d" % i for i in range(len(lvls))])
