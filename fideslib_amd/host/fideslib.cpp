@@ -1064,6 +1064,8 @@ Ciphertext CryptoContextImpl::EvalChebyshevSeries(const Ciphertext& a0, std::vec
   if (a->NoiseLevel == 2) a = Rescale(a);
   if (a->NoiseLevel != 1) throw std::runtime_error("EvalChebyshevSeries: scale degree must be 1 or 2");
   detail::Exec X{*this, ctx, q, N, boot_delta(), evalMultKey};
+  if (!bootMem) bootMem = std::make_shared<detail::Pool>();
+  X.pool = std::static_pointer_cast<detail::Pool>(bootMem);  // registers from the context's cache
   detail::Reg x;
   x.buf = std::make_shared<detail::DevBuf>(a->d, false);
   x.level = a->limbs;
