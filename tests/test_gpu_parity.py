@@ -477,6 +477,26 @@ def test_cfg3_rotate_bit_exact(cfg3):
     assert C.precision_bits(dec, np.roll(z[0], -r)) > 30
 
 
+@pytest.mark.parametrize("l", [30, 7])
+def test_cfg3_conjugate_bit_exact(cfg3, l):
+    """Conjugation (automorphism 2N - 1) at the headline shape against the
+    oracle; decrypts to the conjugated slots."""
+    prm, octx, gctx, keys = cfg3
+    say("cfg3 conjugate l=%d" % l)
+    N = prm.N
+    k = 2 * N - 1
+    key = keys.rotation_key(k)
+    z, (c1, _) = _enc_pair(prm, keys, 303 + l, l)
+    g = gctx.upload_key(key)
+    a = capi.DeviceBuffer.from_array(c1)
+    o = capi.DeviceBuffer(2 * l * N)
+    gctx.rotate(a, l, g, k, o)
+    got = o.download((2, l, N))
+    assert np.array_equal(got, octx.rotate(c1, l, key, k))
+    dec = C.decode(keys.decrypt_coef(got, l), 2.0 ** prm.scale_bits, N)
+    assert C.precision_bits(dec, np.conj(z[0])) > 30
+
+
 # ------------------------------------------------------- batched = plain = oracle
 def test_cfg1_batch_equals_single_and_oracle(cfg1):
     prm, octx, gctx, keys = cfg1
