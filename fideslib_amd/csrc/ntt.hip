@@ -299,6 +299,84 @@ __global__ __launch_bounds__((NttCfg<LOGN, 0, kFwdColW>::THREADS)) void k_ntt_fw
   }
 }
 
+// Column pass at N = 2^16 with 8 elements per thread (32 threads per
+// 256-point column, 16 columns per 512-thread workgroup): radix-8 register
+// phases over stages 0-2, 3-5 and 6-7 with two LDS exchanges, twice the
+// waves of the 16-element pass for launches too small to fill the GPU.
+// Twiddle of stage s for the butterfly whose lower element is i:
+// psi[2^s + (i >> (8 - s))], as in the 16-element pass.  Plain transforms of
+// at most FIDES_COL8 limbs use it: 32 limbs 25.7-25.8 instead of
+// 26.2-27.3 us, bit-identical; for every plain launch (960 limbs, the
+// key-switch step's) it is no faster.
+#ifndef FIDES_COL8
+#define FIDES_COL8 32
+#endif
+template <int NARROW>
+__global__ __launch_bounds__(512) void k_ntt_fwd_col8(LimbSpan span, NttTables T, NttFuse F) {
+  constexpr int LOGN = 16, W = 16, LINE = 257;
+  __shared__ u64 lds[W * LINE];
+  const NttTask tk = resolve_task(span, F, blockIdx.y, LOGN);
+  u64* data = tk.data;
+  const u32 pid = tk.pid;
+  const u64 q = T.prm[pid].q, q2 = 2 * q, msub = (u64)FwdLazy<NARROW, LOGN>::SUB * q;
+  const u64* tw = T.psi + ((u64)pid << LOGN);
+  const u64* tws = T.psi_sh + ((u64)pid << LOGN);
+  const u32 tid = threadIdx.x;
+  const u32 ll = tid % W, t = tid / W;  // column in the workgroup, thread in the column (0..31)
+  const u32 col = blockIdx.x * W + ll;
+  const u64* in = tk.src ? tk.src : data;
+  u64* L = lds + ll * LINE;
+  u64 x[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) x[k] = ldg(in, ((t + 32 * k) << 8) | col);
+  // stages 0-2: elements t + 32k, pairs (k, k + half); wave-uniform twiddles
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const int half = 8 >> (st + 1);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (k & half) continue;
+      const u32 idx = (1u << st) + (k >> (3 - st));
+      fwd_bfly<true, NARROW, LOGN>(st, x[k], x[k + half], tw[idx], tws[idx], q, q2, msub);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) L[t + 32 * k] = x[k];
+  __syncthreads();
+  // stages 3-5: elements 32c + b0 + 4j (c = t / 4, b0 = t % 4; a wave holds
+  // t = 4w .. 4w + 3, so c and the twiddles are wave-uniform)
+  const u32 c = __builtin_amdgcn_readfirstlane(t >> 2), b0 = t & 3;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = L[32 * c + b0 + 4 * j];
+#pragma unroll
+  for (int v = 0; v < 3; ++v) {
+    const int st = 3 + v, half = 8 >> (v + 1);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (j & half) continue;
+      const u32 idx = (1u << st) + (c << v) + (j >> (3 - v));
+      fwd_bfly<true, NARROW, LOGN>(st, x[j], x[j + half], tw[idx], tws[idx], q, q2, msub);
+    }
+  }
+  __syncthreads();  // every thread has read its phase-2 elements
+#pragma unroll
+  for (int j = 0; j < 8; ++j) L[32 * c + b0 + 4 * j] = x[j];
+  __syncthreads();
+  // stages 6-7: two groups of 4 consecutive elements, 4g + m (g = 2t, 2t + 1)
+#pragma unroll
+  for (int e = 0; e < 8; ++e) x[e] = L[8 * t + e];
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi) {
+    const u32 g = 2 * t + gi;
+    fwd_bfly<false, NARROW, LOGN>(6, x[4 * gi], x[4 * gi + 2], tw[64 + g], tws[64 + g], q, q2, msub);
+    fwd_bfly<false, NARROW, LOGN>(6, x[4 * gi + 1], x[4 * gi + 3], tw[64 + g], tws[64 + g], q, q2, msub);
+    fwd_bfly<false, NARROW, LOGN>(7, x[4 * gi], x[4 * gi + 1], tw[128 + 2 * g], tws[128 + 2 * g], q, q2, msub);
+    fwd_bfly<false, NARROW, LOGN>(7, x[4 * gi + 2], x[4 * gi + 3], tw[129 + 2 * g], tws[129 + 2 * g], q, q2, msub);
+  }
+#pragma unroll
+  for (int e = 0; e < 8; ++e) stg(data, ((8 * t + e) << 8) | col, x[e]);
+}
+
 // Row pass: global stages [L1, LOGN).  Input lazy as the column pass left
 // it; output in [0, q).
 // Occupancy target of the row pass at N = 2^16 per epilogue (waves per
@@ -857,6 +935,12 @@ template <int LOGN, int NARROW>
 static void launch_col_n(const LimbSpan& s, const NttTables& T, int pro, const NttFuse& F, hipStream_t st) {
   using C0 = NttCfg<LOGN, 0, kFwdColW>;
   dim3 g0(C0::NLINES / C0::LPW, s.total);
+  if constexpr (LOGN == 16 && FIDES_COL8 > 0) {
+    if (pro == PRO_NONE && s.total <= FIDES_COL8) {
+      k_ntt_fwd_col8<NARROW><<<dim3(256 / 16, s.total), 512, 0, st>>>(s, T, F);
+      return;
+    }
+  }
   if (pro == PRO_SWITCH_MODULUS)
     k_ntt_fwd_col<LOGN, PRO_SWITCH_MODULUS, NARROW><<<g0, C0::THREADS, 0, st>>>(s, T, F);
   else if (pro == PRO_MDRS)
