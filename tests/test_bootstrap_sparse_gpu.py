@@ -108,8 +108,16 @@ def test_cpp_sparse_bootstrap(tmp_path, source):
         s.op("bootsetup", 3, 3, cfg.degree, cfg.double_angle, cfg.K, cfg.correction, 1, ns)
     s.ct("x", ct, 1, sf[lvl - 1], ns)
     s.op("boot", "y", "x")
-    s.store("y")
+    # a second input at another level: EvalBootstrapMany gives each one's bits
+    z2 = rng.uniform(-1, 1, ns) + 1j * rng.uniform(-1, 1, ns)
+    ct2 = keys.encrypt(C.encode(z2, sf[5], N), 6)
+    s.ct("x2", ct2, 1, sf[5], ns)
+    s.op("boot", "y2", "x2")
+    s.op("bootmany", "m", "x", "x2")
+    s.store("y", "y2", "m0", "m1")
     res = s.run()
+    assert np.array_equal(res["m0"]["data"], res["y"]["data"])
+    assert np.array_equal(res["m1"]["data"], res["y2"]["data"])
     assert res["rots"] == bp.rotations()
     y = res["y"]
     lo = prm.L - bp.depth
